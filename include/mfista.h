@@ -1,0 +1,153 @@
+/*
+ * mfista.h -- C ABI of libmfista_amd.so, the MI355X (gfx950) MFISTA imaging solver.
+ *
+ * Drop-in for the reference library ikeda46/sparseimaging: the three
+ * mfista_imaging_core_* entry points, their argument order and meaning, and
+ * struct RESULT are those of c/mfista.h:45-66 and :151-178 (the C++ build,
+ * c++/mfista.hpp:45-66 and :117-140, exports the same symbols plus
+ * x2y_nufft / y2x_nufft).  All pointers of the reference entry points are
+ * host pointers; the library uploads them, solves on the GPU and writes the
+ * results back.  No torch or HIP types appear in these signatures.
+ *
+ * Errors follow the reference: invalid combinations print a message and
+ * return with RESULT and xout unwritten.  The *_ex variants return a status
+ * code instead (0 = success, < 0 = MFISTA_E_*).
+ */
+#ifndef MFISTA_AMD_H
+#define MFISTA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* algorithm constants, c/mfista.h:36-41 */
+#define MAXITER 50000
+#define MINITER 100
+#define FGPITER 100
+#define TD 50
+#define ETA 1.1
+#define EPS 1.0e-5
+
+/* result record, field-for-field c/mfista.h:45-66 (128 bytes on LP64) */
+struct RESULT {
+  int M;
+  int N;
+  int NX;
+  int NY;
+  int N_active;
+  int maxiter;
+  int ITER;
+  int nonneg;
+  double lambda_l1;
+  double lambda_tv;
+  double lambda_tsv;
+  double sq_error;
+  double mean_sq_error;
+  double l1cost;
+  double tvcost;
+  double tsvcost;
+  double finalcost;
+  double comp_time;
+  double *residual;
+  double Lip_const;
+};
+
+/* ---- reference entry points -------------------------------------------- */
+
+/* replaces mfista_imaging_core_nufft, c/mfista.h:172-178 (c/mfista_nufft_lib.c:909);
+ * NUFFT preset C-24 (24 taps), as the C library. */
+void mfista_imaging_core_nufft(double *u_dx, double *v_dy, double *vis_r, double *vis_i,
+                               double *vis_std, int M, int NX, int NY, int maxiter, double eps,
+                               double lambda_l1, double lambda_tv, double lambda_tsv,
+                               double cinit, double *xinit, double *xout, int nonneg_flag,
+                               int box_flag, float *cl_box, struct RESULT *mfista_result);
+
+/* replaces mfista_imaging_core_fft, c/mfista.h:161-168 (c/mfista_fft_lib.c:701);
+ * fftw_plan_flag is accepted and ignored (there is no planner to tune). */
+void mfista_imaging_core_fft(int *u_idx, int *v_idx, double *y_r, double *y_i,
+                             double *noise_stdev, int M, int NX, int NY, int maxiter, double eps,
+                             double lambda_l1, double lambda_tv, double lambda_tsv, double cinit,
+                             double *xinit, double *xout, int nonneg_flag,
+                             unsigned int fftw_plan_flag, int box_flag, float *cl_box,
+                             struct RESULT *mfista_result);
+
+/* replaces mfista_imaging_core_dft, c/mfista.h:151-157 (c/mfista_dft_lib.c:540);
+ * A is M x N, column-major; looe_flag is accepted and ignored (as in the
+ * reference, whose LOOE call is commented out).  RESULT.residual is malloc'ed
+ * with M doubles and owned by the caller. */
+void mfista_imaging_core_dft(double *y, double *A, int *M, int *N, int NX, int NY, int maxiter,
+                             double eps, double lambda_l1, double lambda_tv, double lambda_tsv,
+                             double cinit, double *xinit, double *xout, int nonneg_flag,
+                             int looe_flag, int box_flag, float *cl_box,
+                             struct RESULT *mfista_result);
+
+/* replaces x2y_nufft, c++/mfista.hpp:136-137 (c++/mfista_nufft_lib.cpp:1076):
+ * model visibilities y = A x with the C++ 12-tap preset, FP64. */
+void x2y_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *xin, double *y_r,
+               double *y_i);
+
+/* replaces y2x_nufft, c++/mfista.hpp:139-140 (c++/mfista_nufft_lib.cpp:1151):
+ * x = Re(A^H y) with the C++ 12-tap preset, FP64. */
+void y2x_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *yin_r, double *yin_i,
+               double *xout);
+
+/* ---- extensions (no reference counterpart) ----------------------------- */
+
+#define MFISTA_PRESET_C24 0   /* c/mfista_nufft_lib.c: MSP=12, round, tau=12/N^2 */
+#define MFISTA_PRESET_CXX12 1 /* c++/mfista_nufft_lib.cpp: MSP=6, floor, tau=6/N^2, eta0=10 */
+
+#define MFISTA_OK 0
+#define MFISTA_E_BOTH_REG -1   /* lambda_tv and lambda_tsv both positive */
+#define MFISTA_E_TV_PRESET -2  /* TV with the C++ preset (not in that library) */
+#define MFISTA_E_FOV -3        /* a gridding tap leaves the 2NX x 2NY grid */
+#define MFISTA_E_SIZE -4       /* unsupported image size */
+#define MFISTA_E_ARG -5        /* invalid argument */
+#define MFISTA_E_NONNEG -6     /* nonneg_flag not in {0,1}: ITER = 0, as the reference */
+#define MFISTA_E_HIP -100      /* HIP runtime error */
+
+typedef struct mfista_opts {
+  int preset;       /* MFISTA_PRESET_* (NUFFT only); default C24 */
+  int fgp_iter;     /* FGP iterations of the TV prox; default FGPITER (100) */
+  double rel_tol;   /* > 0: stop when cost[t-TD]-cost[t] <= rel_tol*cost[t], t >= MINITER;
+                       0: the reference's absolute rule with eps (default) */
+  int verbose;      /* nonzero: print progress */
+  int max_trials;   /* out: largest number of backtracking trials in one iteration */
+  double solve_ms;  /* out: GPU wall time of the solve (excluding the FP64 RESULT pass) */
+} mfista_opts;
+
+void mfista_default_opts(mfista_opts *opts);
+
+/* same arguments as mfista_imaging_core_nufft plus options; returns a status */
+int mfista_imaging_core_nufft_ex(const double *u_dx, const double *v_dy, const double *vis_r,
+                                 const double *vis_i, const double *vis_std, int M, int NX,
+                                 int NY, int maxiter, double eps, double lambda_l1,
+                                 double lambda_tv, double lambda_tsv, double cinit,
+                                 const double *xinit, double *xout, int nonneg_flag,
+                                 int box_flag, const float *cl_box,
+                                 struct RESULT *mfista_result, mfista_opts *opts);
+
+/* Same as mfista_imaging_core_nufft_ex, but every array is a DEVICE pointer
+ * (HBM-resident inputs, e.g. from hipMalloc or a torch tensor's data_ptr) and
+ * the work is queued on `stream` (a hipStream_t, NULL = the library's own).
+ * RESULT and opts stay host structures. */
+int mfista_nufft_solve_device(const double *d_u, const double *d_v, const double *d_vis_r,
+                              const double *d_vis_i, const double *d_vis_std, int M, int NX,
+                              int NY, int maxiter, double eps, double lambda_l1,
+                              double lambda_tv, double lambda_tsv, double cinit,
+                              const double *d_xinit, double *d_xout, int nonneg_flag,
+                              int box_flag, const float *d_cl_box,
+                              struct RESULT *mfista_result, mfista_opts *opts, void *stream);
+
+/* standalone NUFFT operators with a chosen preset and precision (0 = FP32,
+ * 1 = FP64); host pointers; return a status */
+int mfista_nufft_forward(int preset, int fp64, const double *u, const double *v, int M, int NX,
+                         int NY, const double *x, double *y_r, double *y_i);
+int mfista_nufft_adjoint(int preset, int fp64, const double *u, const double *v, int M, int NX,
+                         int NY, const double *y_r, const double *y_i, double *x);
+
+const char *mfista_amd_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

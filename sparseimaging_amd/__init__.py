@@ -1,0 +1,251 @@
+"""sparseimaging_amd -- MI355X-native MFISTA imaging solver (ctypes mirror of the C ABI).
+
+The native library ``lib/libmfista_amd.so`` (HIP, gfx950) exports the
+reference library's entry points (ikeda46/sparseimaging ``c/mfista.h:151-178``,
+``c++/mfista.hpp:117-140``).  This module binds them with ctypes and adds
+numpy-friendly wrappers whose argument names and meaning follow the reference
+(``mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, M, NX, NY,
+maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit, xinit, xout,
+nonneg_flag, box_flag, cl_box, mfista_result)``).
+
+There is no CPU fallback: importing works without a GPU (for ABI checks), but
+every solve needs the HIP library and a visible gfx950 device and raises
+otherwise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmfista_amd.so")
+
+PRESET_C24 = 0
+PRESET_CXX12 = 1
+MAXITER, MINITER, FGPITER, TD, ETA, EPS = 50000, 100, 100, 50, 1.1, 1.0e-5
+
+STATUS = {
+    0: "ok", -1: "lambda_tv and lambda_tsv both positive", -2: "TV needs the C-24 preset",
+    -3: "field of view too small for the (u,v) coverage", -4: "unsupported image size",
+    -5: "invalid argument", -6: "nonneg_flag must be 0 or 1", -100: "HIP runtime error",
+}
+
+EXPORTED = [
+    "mfista_imaging_core_nufft", "mfista_imaging_core_fft", "mfista_imaging_core_dft",
+    "x2y_nufft", "y2x_nufft", "mfista_default_opts", "mfista_imaging_core_nufft_ex",
+    "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
+    "mfista_amd_version",
+]
+
+
+class RESULT(C.Structure):
+    """struct RESULT, field for field c/mfista.h:45-66."""
+
+    _fields_ = [(n, C.c_int) for n in ("M", "N", "NX", "NY", "N_active", "maxiter", "ITER", "nonneg")] + [
+        (n, C.c_double)
+        for n in (
+            "lambda_l1", "lambda_tv", "lambda_tsv", "sq_error", "mean_sq_error", "l1cost",
+            "tvcost", "tsvcost", "finalcost", "comp_time",
+        )
+    ] + [("residual", C.POINTER(C.c_double)), ("Lip_const", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "residual"}
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("preset", C.c_int), ("fgp_iter", C.c_int), ("rel_tol", C.c_double), ("verbose", C.c_int),
+        ("max_trials", C.c_int), ("solve_ms", C.c_double),
+    ]
+
+
+class MfistaError(RuntimeError):
+    def __init__(self, status):
+        super().__init__(f"mfista_amd status {status}: {STATUS.get(status, 'error')}")
+        self.status = status
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
+
+
+def lib():
+    """Load libmfista_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    i, d, u = C.c_int, C.c_double, C.c_uint
+    L.mfista_imaging_core_nufft.restype = None
+    L.mfista_imaging_core_nufft.argtypes = [_dp, _dp, _dp, _dp, _dp, i, i, i, i, d, d, d, d, d, _dp, _dp,
+                                            i, i, _fp, C.POINTER(RESULT)]
+    L.mfista_imaging_core_fft.restype = None
+    L.mfista_imaging_core_fft.argtypes = [_ip, _ip, _dp, _dp, _dp, i, i, i, i, d, d, d, d, d, _dp, _dp,
+                                          i, u, i, _fp, C.POINTER(RESULT)]
+    L.mfista_imaging_core_dft.restype = None
+    L.mfista_imaging_core_dft.argtypes = [_dp, _dp, _ip, _ip, i, i, i, d, d, d, d, d, _dp, _dp, i, i, i,
+                                          _fp, C.POINTER(RESULT)]
+    L.x2y_nufft.restype = None
+    L.x2y_nufft.argtypes = [_dp, _dp, i, i, i, _dp, _dp, _dp]
+    L.y2x_nufft.restype = None
+    L.y2x_nufft.argtypes = [_dp, _dp, i, i, i, _dp, _dp, _dp]
+    L.mfista_default_opts.restype = None
+    L.mfista_default_opts.argtypes = [C.POINTER(Opts)]
+    L.mfista_imaging_core_nufft_ex.restype = i
+    L.mfista_imaging_core_nufft_ex.argtypes = [_dp, _dp, _dp, _dp, _dp, i, i, i, i, d, d, d, d, d, _dp,
+                                               _dp, i, i, _fp, C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_nufft_solve_device.restype = i
+    L.mfista_nufft_solve_device.argtypes = [C.c_void_p] * 5 + [i, i, i, i, d, d, d, d, d] + [
+        C.c_void_p, C.c_void_p, i, i, C.c_void_p, C.POINTER(RESULT), C.POINTER(Opts), C.c_void_p]
+    L.mfista_nufft_forward.restype = i
+    L.mfista_nufft_forward.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
+    L.mfista_nufft_adjoint.restype = i
+    L.mfista_nufft_adjoint.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
+    L.mfista_amd_version.restype = C.c_char_p
+    L.mfista_amd_version.argtypes = []
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _box(cl_box, n):
+    if cl_box is None:
+        return 0, None, C.cast(None, _fp)
+    b = np.ascontiguousarray(cl_box, dtype=np.float32).ravel()
+    if b.size != n:
+        raise ValueError("cl_box must hold NX*NY floats")
+    return 1, b, b.ctypes.data_as(_fp)
+
+
+def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0):
+    o = Opts()
+    lib().mfista_default_opts(C.byref(o))
+    o.preset, o.fgp_iter, o.rel_tol, o.verbose = preset, fgp_iter, rel_tol, verbose
+    return o
+
+
+# ---------------------------------------------------------------- solvers
+def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
+                              lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
+                              nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
+                              rel_tol=0.0):
+    """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts)."""
+    u, pu = _d(u_dx)
+    v, pv = _d(v_dy)
+    vr, pvr = _d(vis_r)
+    vi, pvi = _d(vis_i)
+    vs, pvs = _d(vis_std)
+    M = u.size
+    x0 = np.zeros(NX * NY) if xinit is None else np.asarray(xinit, dtype=np.float64).ravel()
+    x0, px0 = _d(x0)
+    xout = np.zeros(NX * NY)
+    bflag, b, pb = _box(cl_box, NX * NY)
+    res = RESULT()
+    o = make_opts(preset, fgp_iter, rel_tol)
+    rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
+                                            lambda_tv, lambda_tsv, cinit, px0,
+                                            xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
+                                            C.byref(res), C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    return xout.reshape(NX, NY), res.as_dict(), {"max_trials": o.max_trials, "solve_ms": o.solve_ms}
+
+
+def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
+                       lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
+                       t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None):
+    """HBM-resident solve: every array argument is a device pointer (int) or a
+    torch CUDA tensor (float64; cl_box float32).  Returns (RESULT dict, opts dict)."""
+
+    def ptr(t):
+        if t is None:
+            return None
+        return t if isinstance(t, int) else t.data_ptr()
+
+    M = t_u.numel() if hasattr(t_u, "numel") else None
+    if M is None:
+        raise ValueError("pass torch tensors (M is taken from t_u)")
+    res = RESULT()
+    o = make_opts(preset, fgp_iter, rel_tol)
+    rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
+                                         maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
+                                         ptr(t_xinit), ptr(t_xout), nonneg_flag,
+                                         0 if t_box is None else 1, ptr(t_box), C.byref(res),
+                                         C.byref(o), stream)
+    if rc != 0:
+        raise MfistaError(rc)
+    return res.as_dict(), {"max_trials": o.max_trials, "solve_ms": o.solve_ms}
+
+
+def nufft_forward(u, v, x, preset=PRESET_C24, fp64=False):
+    """Model visibilities A.x of an NX x NY image (NUFFT2d2); complex array of M."""
+    u, pu = _d(u)
+    v, pv = _d(v)
+    x, px = _d(x)
+    NX, NY = x.shape
+    yr = np.zeros(u.size)
+    yi = np.zeros(u.size)
+    rc = lib().mfista_nufft_forward(preset, int(fp64), pu, pv, u.size, NX, NY, px,
+                                    yr.ctypes.data_as(_dp), yi.ctypes.data_as(_dp))
+    if rc != 0:
+        raise MfistaError(rc)
+    return yr + 1j * yi
+
+
+def nufft_adjoint(u, v, y, NX, NY, preset=PRESET_C24, fp64=False):
+    """Re(A^H y) (NUFFT2d1) as an NX x NY image."""
+    u, pu = _d(u)
+    v, pv = _d(v)
+    yr, pyr = _d(np.real(y))
+    yi, pyi = _d(np.imag(y))
+    x = np.zeros((NX, NY))
+    rc = lib().mfista_nufft_adjoint(preset, int(fp64), pu, pv, u.size, NX, NY, pyr, pyi,
+                                    x.ctypes.data_as(_dp))
+    if rc != 0:
+        raise MfistaError(rc)
+    return x
+
+
+def x2y_nufft(u_dx, v_dy, xin):
+    """c++/mfista_nufft_lib.cpp:1076 -- 12-tap model visibilities, FP64."""
+    u, pu = _d(u_dx)
+    v, pv = _d(v_dy)
+    x, px = _d(xin)
+    NX, NY = x.shape
+    yr = np.zeros(u.size)
+    yi = np.zeros(u.size)
+    lib().x2y_nufft(pu, pv, u.size, NX, NY, px, yr.ctypes.data_as(_dp), yi.ctypes.data_as(_dp))
+    return yr, yi
+
+
+def y2x_nufft(u_dx, v_dy, yin_r, yin_i, NX, NY):
+    """c++/mfista_nufft_lib.cpp:1151 -- 12-tap adjoint image, FP64."""
+    u, pu = _d(u_dx)
+    v, pv = _d(v_dy)
+    yr, pyr = _d(yin_r)
+    yi, pyi = _d(yin_i)
+    x = np.zeros((NX, NY))
+    lib().y2x_nufft(pu, pv, u.size, NX, NY, pyr, pyi, x.ctypes.data_as(_dp))
+    return x
+
+
+def version():
+    return lib().mfista_amd_version().decode()

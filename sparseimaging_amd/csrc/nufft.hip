@@ -1,0 +1,589 @@
+// nufft.hip -- Gaussian-gridding NUFFT kernels for gfx950.
+//
+// Forward A.x  (reference NUFFT2d2, c/mfista_nufft_lib.c:187-243 and
+//               c++/mfista_nufft_lib.cpp:440-504):
+//   k_rowfft_fwd  E4-weighted, centred rows -> r2c along y (two rows per complex
+//                 FFT) -> T[ky][i]  (transposed through LDS)
+//   k_colfft_fwd  c2c along x per ky -> Gf[ky][kx] (fft-shifted columns, plus
+//                 MSP-1 conjugate-mirrored rows of negative ky)
+//   k_degrid      G lanes per visibility gather the (2MSP)^2 taps, then the
+//                 weighted residual and its FP64 chi^2 partial are fused in
+// Adjoint Re(A^H r) (reference NUFFT2d1, c/mfista_nufft_lib.c:123-185 and
+//               c++/mfista_nufft_lib.cpp:189-438):
+//   k_grid        one workgroup per LDS tile of the half-plane grid; each of the
+//                 4 wavefronts accumulates into its own LDS copy (lanes own
+//                 distinct taps, so no atomics), copies summed in fixed order
+//   k_colfft_inv  c2c backward along x per ky -> U[i][ky]
+//   k_rowfft_c2r  c2r along y (FFTW semantics), crop, * E4 / (8 NX NY)
+#include <hipcub/hipcub.hpp>
+
+#include "nufft.h"
+
+namespace mf {
+
+// --------------------------------------------------------------------------
+// forward row pass: real rows -> half spectra along y
+template <typename R>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_rowfft_fwd(const R* __restrict__ x, const R* __restrict__ e4x, const R* __restrict__ e4y,
+             cplx<R>* __restrict__ T, int NX, int NY, int Ly, int shift, FFTPlan P,
+             const cplx<R>* __restrict__ tw, int nseq, const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  extern __shared__ double4 smem_d4[];
+  C* b0 = reinterpret_cast<C*>(smem_d4);
+  C* b1 = b0 + nseq * Ly;
+  const int row0 = blockIdx.x * 2 * nseq;
+  for (int t = threadIdx.x; t < nseq * Ly; t += blockDim.x) {
+    const int q = t / Ly, b = t - q * Ly;
+    int j = b + shift;
+    if (j >= Ly) j -= Ly;
+    const int i0 = row0 + 2 * q, i1 = i0 + 1;
+    R re = 0, im = 0;
+    if (j < NY) {
+      const R ey = e4y ? e4y[j] : (R)1;
+      if (i0 < NX) re = x[(size_t)i0 * NY + j] * (e4x ? e4x[i0] : (R)1) * ey;
+      if (i1 < NX) im = x[(size_t)i1 * NY + j] * (e4x ? e4x[i1] : (R)1) * ey;
+    }
+    b0[t] = mk<R>(re, im);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
+  const int Lh = Ly / 2 + 1, nr = 2 * nseq;
+  for (int t = threadIdx.x; t < Lh * nr; t += blockDim.x) {
+    const int k = t / nr, r = t - k * nr;
+    const int i = row0 + r;
+    if (i >= NX) continue;
+    const int q = r >> 1;
+    const C z = res[q * Ly + k], zm = res[q * Ly + (k == 0 ? 0 : Ly - k)];
+    C o;
+    if (!(r & 1))
+      o = mk<R>((R)0.5 * (z.x + zm.x), (R)0.5 * (z.y - zm.y));
+    else
+      o = mk<R>((R)0.5 * (z.y + zm.y), (R)-0.5 * (z.x - zm.x));
+    T[(size_t)k * NX + i] = o;
+  }
+}
+
+// forward column pass: c2c along x for each ky row of T
+// mode 0: NUFFT layout Gf[ky + MSP - 1][(kx + NX) mod Lx] with mirrored rows
+// mode 1: plain layout out[ky][kx]
+template <typename R>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_colfft_fwd(const cplx<R>* __restrict__ T, cplx<R>* __restrict__ out, int NX, int nky, int Lx,
+             int shift, int mode, int MSP, FFTPlan P, const cplx<R>* __restrict__ tw, int nseq,
+             const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  extern __shared__ double4 smem_d4[];
+  C* b0 = reinterpret_cast<C*>(smem_d4);
+  C* b1 = b0 + nseq * Lx;
+  const int ky0 = blockIdx.x * nseq;
+  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
+    const int q = t / Lx, a = t - q * Lx;
+    int i = a + shift;
+    if (i >= Lx) i -= Lx;
+    const int ky = ky0 + q;
+    b0[t] = (ky < nky && i < NX) ? T[(size_t)ky * NX + i] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
+  const int NXh = Lx / 2;
+  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
+    const int q = t / Lx, k = t - q * Lx;
+    const int ky = ky0 + q;
+    if (ky >= nky) continue;
+    const C v = res[t];
+    if (mode == 0) {
+      int col = k + NXh;
+      if (col >= Lx) col -= Lx;
+      out[(size_t)(ky + MSP - 1) * Lx + col] = v;
+      if (ky >= 1 && ky <= MSP - 1) {
+        int colm = NXh - k;
+        if (colm < 0) colm += Lx;
+        out[(size_t)(MSP - 1 - ky) * Lx + colm] = cconj(v);
+      }
+    } else {
+      out[(size_t)ky * Lx + k] = v;
+    }
+  }
+}
+
+// adjoint column pass: c2c backward along x for each ky, transposed output U[i][ky]
+// mode 0: input in NUFFT layout Hg[ky][(kx + NX) mod Lx]; mode 1: plain in[ky][kx]
+template <typename R>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_colfft_inv(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ U, int NX, int nky, int Lx,
+             int shift, int mode, FFTPlan P, const cplx<R>* __restrict__ tw, int nseq,
+             const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  extern __shared__ double4 smem_d4[];
+  C* b0 = reinterpret_cast<C*>(smem_d4);
+  C* b1 = b0 + nseq * Lx;
+  const int ky0 = blockIdx.x * nseq;
+  const int NXh = Lx / 2;
+  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
+    const int q = t / Lx, k = t - q * Lx;
+    const int ky = ky0 + q;
+    int col = k;
+    if (mode == 0) {
+      col = k + NXh;
+      if (col >= Lx) col -= Lx;
+    }
+    b0[t] = (ky < nky) ? in[(size_t)ky * Lx + col] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, nseq, P, tw, +1);
+  for (int t = threadIdx.x; t < NX * nseq; t += blockDim.x) {
+    const int i = t / nseq, q = t - i * nseq;
+    const int ky = ky0 + q;
+    if (ky >= nky) continue;
+    int a = i - shift;
+    if (a < 0) a += Lx;
+    U[(size_t)i * nky + ky] = res[q * Lx + a];
+  }
+}
+
+// adjoint row pass: 1-D c2r along y for two image rows per complex FFT.  Only
+// the real parts of the DC and Nyquist entries are used (FFTW c2r semantics).
+template <typename R>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_rowfft_c2r(const cplx<R>* __restrict__ U, const R* __restrict__ e4x, const R* __restrict__ e4y,
+             R scale, R* __restrict__ g, int NX, int NY, int Ly, int shift, FFTPlan P,
+             const cplx<R>* __restrict__ tw, int nseq, const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  extern __shared__ double4 smem_d4[];
+  C* b0 = reinterpret_cast<C*>(smem_d4);
+  C* b1 = b0 + nseq * Ly;
+  const int row0 = blockIdx.x * 2 * nseq;
+  const int Lh = Ly / 2, nky = Ly / 2 + 1;
+  for (int t = threadIdx.x; t < nseq * Ly; t += blockDim.x) {
+    const int q = t / Ly, b = t - q * Ly;
+    const int i0 = row0 + 2 * q, i1 = i0 + 1;
+    const int k = (b <= Lh) ? b : Ly - b;
+    C A = (i0 < NX) ? U[(size_t)i0 * nky + k] : mk<R>(0, 0);
+    C B = (i1 < NX) ? U[(size_t)i1 * nky + k] : mk<R>(0, 0);
+    if (k == 0 || k == Lh) {
+      A.y = 0;
+      B.y = 0;
+    }
+    b0[t] = (b <= Lh) ? mk<R>(A.x - B.y, A.y + B.x) : mk<R>(A.x + B.y, -A.y + B.x);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, nseq, P, tw, +1);
+  const int nr = 2 * nseq;
+  for (int t = threadIdx.x; t < nr * NY; t += blockDim.x) {
+    const int r = t / NY, j = t - r * NY;
+    const int i = row0 + r;
+    if (i >= NX) continue;
+    int b = j - shift;
+    if (b < 0) b += Ly;
+    const C z = res[(r >> 1) * Ly + b];
+    R val = (r & 1) ? z.y : z.x;
+    val *= scale;
+    if (e4x) val *= e4x[i];
+    if (e4y) val *= e4y[j];
+    g[(size_t)i * NY + j] = val;
+  }
+}
+
+// --------------------------------------------------------------------------
+// degridding: G lanes per visibility; lane l owns tap column kx = mx + l.
+template <typename R, int G>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rw,
+         double* __restrict__ chi_part, cplx<R>* __restrict__ model, const int* __restrict__ perm,
+         const int* __restrict__ yneg, const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  __shared__ double red[16];
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = gid / G, lane = threadIdx.x & (G - 1);
+  const int S = 2 * g.MSP;
+  const R t3x = (R)g.t3x, t3y = (R)g.t3y;
+  C acc = mk<R>(0, 0);
+  if (s < g.M && lane < S) {
+    const int2 m = vp.mxy[s];
+    const C a = vp.a[s];
+    const R l = (R)(lane - g.MSP + 1);
+    const R gx = dev_exp(l * a.x - l * l * t3x);
+    const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - g.MSP + 1);
+    for (int b = 0; b < S; b++) {
+      const R lp = (R)(b - g.MSP + 1);
+      const R gy = dev_exp(lp * a.y - lp * lp * t3y);
+      const C gv = col[(size_t)b * g.Lx];
+      acc.x += gy * gv.x;
+      acc.y += gy * gv.y;
+    }
+    acc.x *= gx;
+    acc.y *= gx;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    acc.x += __shfl_xor(acc.x, off, 64);
+    acc.y += __shfl_xor(acc.y, off, 64);
+  }
+  double chi[1] = {0.0};
+  if (s < g.M && lane == 0) {
+    const R e1s = vp.e1[s] * (R)g.inv_MM;
+    const C F = mk<R>(e1s * acc.x, -e1s * acc.y);
+    const C V = vp.vis[s];
+    const R w = vp.w[s];
+    const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
+    chi[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+    if (rw) rw[s] = mk<R>(r.x * w, r.y * w);
+    if (model) model[perm[s]] = yneg[s] ? cconj(F) : F;
+  }
+  if (chi_part) {
+    block_sum<1>(chi, red);
+    if (threadIdx.x == 0) chi_part[blockIdx.x] = chi[0];
+  }
+}
+
+// --------------------------------------------------------------------------
+// gridding: one workgroup (4 wavefronts) per TY x TX tile of Hg.
+template <typename R>
+__global__ void __launch_bounds__(MF_BLOCK)
+k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rw,
+       const int* __restrict__ tstart, const int* __restrict__ tend,
+       const unsigned* __restrict__ ent, cplx<R>* __restrict__ Hg, const int* skip) {
+  if (skip && *skip) return;
+  using C = cplx<R>;
+  extern __shared__ double4 smem_d4[];
+  C* acc = reinterpret_cast<C*>(smem_d4);
+  const int TX = g.TX, TY = g.TY, TT = TX * TY;
+  const int tile = blockIdx.x;
+  const int tr = tile / g.ntx, tc = tile - tr * g.ntx;
+  const int rb = tr * TY, cb = tc * TX;
+  for (int t = threadIdx.x; t < 4 * TT; t += blockDim.x) acc[t] = mk<R>(0, 0);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  C* my = acc + wave * TT;
+  const int MSP = g.MSP, NX = g.NX;
+  const R t3x = (R)g.t3x, t3y = (R)g.t3y;
+  const int e_end = tend[tile];
+  for (int e = tstart[tile] + wave; e < e_end; e += 4) {
+    const unsigned en = ent[e];
+    const int s = (int)(en & 0x7fffffffu);
+    const bool mir = (en >> 31) != 0;
+    const int2 m = vp.mxy[s];
+    const C a = vp.a[s];
+    const R e1 = vp.e1[s];
+    const C r = rw[s];
+    C d = mk<R>(e1 * r.x, -e1 * r.y);
+    int lp0, lp1, l0, l1;
+    if (!mir) {
+      lp0 = max(-MSP + 1, rb - m.y);
+      lp1 = min(MSP + 1, rb + TY - m.y);
+      l0 = max(-MSP + 1, cb - m.x - NX);
+      l1 = min(MSP + 1, cb + TX - m.x - NX);
+    } else {
+      d.y = -d.y;
+      lp0 = max(-MSP + 1, -m.y - rb - TY + 1);
+      lp1 = min(MSP + 1, -m.y - rb + 1);
+      l0 = max(-MSP + 1, NX - m.x - cb - TX + 1);
+      l1 = min(MSP + 1, NX - m.x - cb + 1);
+    }
+    const int nl = l1 - l0, nlp = lp1 - lp0;
+    if (nl <= 0 || nlp <= 0) continue;
+    const int ntap = nl * nlp;
+    for (int t = lane; t < ntap; t += 64) {
+      const int ia = t / nl, ib = t - ia * nl;
+      const int lp = lp0 + ia, l = l0 + ib;
+      const R fl = (R)l, flp = (R)lp;
+      const R wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
+      int row, col;
+      if (!mir) {
+        row = m.y + lp - rb;
+        col = m.x + NX + l - cb;
+      } else {
+        row = -(m.y + lp) - rb;
+        col = NX - m.x - l - cb;
+      }
+      C& c = my[row * TX + col];
+      c.x += wgt * d.x;
+      c.y += wgt * d.y;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < TT; t += blockDim.x) {
+    const int rr = t / TX, cc = t - rr * TX;
+    const int row = rb + rr, col = cb + cc;
+    C sum = acc[t];
+    sum = cadd(sum, acc[TT + t]);
+    sum = cadd(sum, acc[2 * TT + t]);
+    sum = cadd(sum, acc[3 * TT + t]);
+    if (row <= g.NY && col < g.Lx) Hg[(size_t)row * g.Lx + col] = sum;
+  }
+}
+
+// --------------------------------------------------------------------------
+// setup: grid cell of every visibility (FP64 maths, as preNUFFT computes it)
+__device__ inline void vis_cell(double u, double v, int NX, int NY, int preset, bool& flip,
+                                double& uu, double& vv, int& mx, int& my) {
+  const double pi = 3.14159265358979323846;
+  flip = v < 0;
+  uu = flip ? -u : u;
+  vv = flip ? -v : v;
+  const double fx = uu * (2.0 * NX) / (2.0 * pi), fy = vv * (2.0 * NY) / (2.0 * pi);
+  if (preset == 0) {
+    mx = (int)round(fx);
+    my = (int)round(fy);
+  } else {
+    mx = (int)floor(fx);
+    my = (int)floor(fy);
+  }
+}
+
+__global__ void k_vis_keys(const double* __restrict__ u, const double* __restrict__ v, int M,
+                           int NX, int NY, int preset, int MSP, int TX, int TY, int ntx,
+                           int* __restrict__ keys, int* __restrict__ idx, int* err) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  bool flip;
+  double uu, vv;
+  int mx, my;
+  vis_cell(u[k], v[k], NX, NY, preset, flip, uu, vv, mx, my);
+  const bool ok = (mx >= -NX + MSP) && (mx <= NX - MSP - 1) && (my >= -NY + MSP) &&
+                  (my <= NY - MSP - 1) && isfinite(u[k]) && isfinite(v[k]);
+  if (!ok) atomicOr(err, 1);
+  int tr = ok ? max(my, 0) / TY : 0, tc = ok ? (mx + NX) / TX : 0;
+  keys[k] = tr * ntx + tc;
+  idx[k] = k;
+}
+
+__global__ void k_vis_params(const int* __restrict__ perm, const double* __restrict__ u,
+                             const double* __restrict__ v, const double* __restrict__ vr,
+                             const double* __restrict__ vi, const double* __restrict__ sd, int M,
+                             int NX, int NY, int preset, int MSP, VisParams<float> pf,
+                             VisParams<double> pd, int* __restrict__ yneg) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const int k = perm[s];
+  const double pi = 3.14159265358979323846;
+  bool flip;
+  double uu, vv;
+  int mx, my;
+  vis_cell(u[k], v[k], NX, NY, preset, flip, uu, vv, mx, my);
+  const double Mrx = 2.0 * NX, Mry = 2.0 * NY;
+  double taux, tauy;
+  if (preset == 0) {
+    taux = 12.0 / ((double)NX * NX);
+    tauy = 12.0 / ((double)NY * NY);
+  } else {
+    taux = (double)MSP / ((double)NX * NX);
+    tauy = taux;  // the C++ preset uses one tau for both axes
+  }
+  const double xix = 2.0 * pi * mx / Mrx, xiy = 2.0 * pi * my / Mry;
+  const double dx = uu - xix, dy = vv - xiy;
+  const double e1 = exp(-dx * dx / (4.0 * taux) - dy * dy / (4.0 * tauy));
+  const double ax = pi * dx / (Mrx * taux), ay = pi * dy / (Mry * tauy);
+  const double Vr = vr[k], Vi = flip ? -vi[k] : vi[k];
+  const double w = 1.0 / sd[k];
+  pd.mxy[s] = make_int2(mx, my);
+  pd.a[s] = make_double2(ax, ay);
+  pd.e1[s] = e1;
+  pd.vis[s] = make_double2(Vr, Vi);
+  pd.w[s] = w;
+  pf.mxy[s] = make_int2(mx, my);
+  pf.a[s] = make_float2((float)ax, (float)ay);
+  pf.e1[s] = (float)e1;
+  pf.vis[s] = make_float2((float)Vr, (float)Vi);
+  pf.w[s] = (float)w;
+  yneg[s] = flip ? 1 : 0;
+}
+
+// up to 6 gridding-tile entries per visibility: <= 4 for the direct footprint
+// (clipped to ky >= 0) and <= 2 for the mirrored part of taps at ky <= 0.
+__global__ void k_tile_entries(const int2* __restrict__ mxy, int M, int NX, int NY, int MSP,
+                               int TX, int TY, int ntx, int nty, int* __restrict__ keys,
+                               unsigned* __restrict__ vals) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const int2 m = mxy[s];
+  const int sentinel = ntx * nty;
+  int n = 0;
+  int* kk = keys + (size_t)6 * s;
+  unsigned* vv = vals + (size_t)6 * s;
+  {
+    const int r0 = max(0, m.y - MSP + 1), r1 = m.y + MSP;
+    const int c0 = m.x + NX - MSP + 1, c1 = m.x + NX + MSP;
+    for (int tr = r0 / TY; tr <= r1 / TY && tr < nty; tr++)
+      for (int tc = c0 / TX; tc <= c1 / TX && tc < ntx; tc++)
+        if (n < 6) {
+          kk[n] = tr * ntx + tc;
+          vv[n] = (unsigned)s;
+          n++;
+        }
+  }
+  if (m.y - MSP + 1 <= 0) {
+    const int r1 = MSP - 1 - m.y;
+    const int c0 = NX - m.x - MSP, c1 = NX - m.x + MSP - 1;
+    for (int tr = 0; tr <= r1 / TY && tr < nty; tr++)
+      for (int tc = c0 / TX; tc <= c1 / TX && tc < ntx; tc++)
+        if (n < 6) {
+          kk[n] = tr * ntx + tc;
+          vv[n] = (unsigned)s | 0x80000000u;
+          n++;
+        }
+  }
+  for (; n < 6; n++) {
+    kk[n] = sentinel;
+    vv[n] = 0;
+  }
+}
+
+__global__ void k_tile_ranges(const int* __restrict__ keys, int n, int ntiles,
+                              int* __restrict__ start, int* __restrict__ end) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int k = keys[e];
+  if (k >= ntiles) return;
+  if (e == 0 || keys[e - 1] != k) start[k] = e;
+  if (e == n - 1 || keys[e + 1] != k) end[k] = e + 1;
+}
+
+// --------------------------------------------------------------------------
+static inline int fft_nseq(int L, size_t csize, int cap) {
+  int n = (int)(65536 / (2 * (size_t)L * csize));
+  if (n < 1) n = 1;
+  if (n > cap) n = cap;
+  return n;
+}
+
+template <typename R>
+void launch_rowfft_fwd(const R* x, const R* e4x, const R* e4y, cplx<R>* T, int NX, int NY, int Ly,
+                       int shift, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
+                       const int* skip) {
+  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), 8);
+  const int nblk = (NX + 2 * nseq - 1) / (2 * nseq);
+  const size_t sh = 2 * (size_t)nseq * Ly * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_rowfft_fwd<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, x, e4x, e4y, T, NX, NY,
+                     Ly, shift, P, tw, nseq, skip);
+}
+
+template <typename R>
+void launch_colfft_fwd(const cplx<R>* T, cplx<R>* out, int NX, int nky, int Lx, int shift,
+                       int mode, int MSP, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
+                       const int* skip) {
+  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), 8);
+  const int nblk = (nky + nseq - 1) / nseq;
+  const size_t sh = 2 * (size_t)nseq * Lx * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_colfft_fwd<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, T, out, NX, nky, Lx,
+                     shift, mode, MSP, P, tw, nseq, skip);
+}
+
+template <typename R>
+void launch_colfft_inv(const cplx<R>* in, cplx<R>* U, int NX, int nky, int Lx, int shift,
+                       int mode, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
+                       const int* skip) {
+  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), 8);
+  const int nblk = (nky + nseq - 1) / nseq;
+  const size_t sh = 2 * (size_t)nseq * Lx * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_colfft_inv<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, in, U, NX, nky, Lx,
+                     shift, mode, P, tw, nseq, skip);
+}
+
+template <typename R>
+void launch_rowfft_c2r(const cplx<R>* U, const R* e4x, const R* e4y, R scale, R* g, int NX,
+                       int NY, int Ly, int shift, const FFTPlan& P, const cplx<R>* tw,
+                       hipStream_t st, const int* skip) {
+  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), 8);
+  const int nblk = (NX + 2 * nseq - 1) / (2 * nseq);
+  const size_t sh = 2 * (size_t)nseq * Ly * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_rowfft_c2r<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, U, e4x, e4y, scale, g,
+                     NX, NY, Ly, shift, P, tw, nseq, skip);
+}
+
+static inline int degrid_lanes(int MSP) { return (2 * MSP <= 16) ? 16 : 32; }
+
+int degrid_blocks(int M, int MSP) {
+  const int G = degrid_lanes(MSP);
+  const int per = MF_BLOCK / G;
+  return (M + per - 1) / per;
+}
+
+template <typename R>
+void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rw,
+                   double* chi_part, int nblk, cplx<R>* model, const int* perm,
+                   const int* yneg_sorted, hipStream_t st, const int* skip) {
+  if (degrid_lanes(g.MSP) == 16)
+    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rw,
+                       chi_part, model, perm, yneg_sorted, skip);
+  else
+    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rw,
+                       chi_part, model, perm, yneg_sorted, skip);
+}
+
+template <typename R>
+void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rw,
+                 const int* tile_start, const int* tile_end, const unsigned* entries,
+                 cplx<R>* Hg, hipStream_t st, const int* skip) {
+  const size_t sh = 4 * (size_t)g.TX * g.TY * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_grid<R>, dim3(g.ntx * g.nty), dim3(MF_BLOCK), sh, st, g, vp, rw,
+                     tile_start, tile_end, entries, Hg, skip);
+}
+
+void launch_vis_keys(const double* u, const double* v, int M, int NX, int NY, int preset,
+                     int MSP, int TX, int TY, int ntx, int* keys, int* idx, int* err,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_vis_keys, dim3((M + 255) / 256), dim3(256), 0, st, u, v, M, NX, NY, preset,
+                     MSP, TX, TY, ntx, keys, idx, err);
+}
+
+void launch_vis_params(const int* perm, const double* u, const double* v, const double* vr,
+                       const double* vi, const double* sd, int M, int NX, int NY, int preset,
+                       int MSP, VisParams<float> pf, VisParams<double> pd, int* yneg,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_vis_params, dim3((M + 255) / 256), dim3(256), 0, st, perm, u, v, vr, vi, sd,
+                     M, NX, NY, preset, MSP, pf, pd, yneg);
+}
+
+void launch_tile_entries(const int2* mxy, int M, int NX, int NY, int MSP, int TX, int TY,
+                         int ntx, int nty, int* keys, unsigned* vals, hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_entries, dim3((M + 255) / 256), dim3(256), 0, st, mxy, M, NX, NY, MSP,
+                     TX, TY, ntx, nty, keys, vals);
+}
+
+void launch_tile_ranges(const int* keys_sorted, int n, int ntiles, int* start, int* end,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_ranges, dim3((n + 255) / 256), dim3(256), 0, st, keys_sorted, n,
+                     ntiles, start, end);
+}
+
+// radix sort of (key, value) pairs; temp storage sized by the first call
+int sort_pairs(void* temp, size_t* temp_bytes, const int* kin, int* kout, const int* vin, int* vout,
+               int n, int end_bit, hipStream_t st) {
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, n, 0,
+                                                    end_bit, st);
+  return e == hipSuccess ? 0 : -1;
+}
+int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, const unsigned* vin,
+                 unsigned* vout, int n, int end_bit, hipStream_t st) {
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, n, 0,
+                                                    end_bit, st);
+  return e == hipSuccess ? 0 : -1;
+}
+
+#define MF_INST(R)                                                                               \
+  template void launch_rowfft_fwd<R>(const R*, const R*, const R*, cplx<R>*, int, int, int, int, \
+                                     const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
+  template void launch_colfft_fwd<R>(const cplx<R>*, cplx<R>*, int, int, int, int, int, int,     \
+                                     const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
+  template void launch_colfft_inv<R>(const cplx<R>*, cplx<R>*, int, int, int, int, int,          \
+                                     const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
+  template void launch_rowfft_c2r<R>(const cplx<R>*, const R*, const R*, R, R*, int, int, int,   \
+                                     int, const FFTPlan&, const cplx<R>*, hipStream_t,           \
+                                     const int*);                                                \
+  template void launch_degrid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, cplx<R>*, \
+                                 double*, int, cplx<R>*, const int*, const int*, hipStream_t,    \
+                                 const int*);                                                    \
+  template void launch_grid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, const int*, \
+                               const int*, const unsigned*, cplx<R>*, hipStream_t, const int*);
+
+MF_INST(float)
+MF_INST(double)
+
+}  // namespace mf

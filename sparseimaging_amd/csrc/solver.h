@@ -1,0 +1,77 @@
+// solver.h -- host-side context and workspaces shared by the solver files.
+#pragma once
+
+#include <mutex>
+
+#include "../../include/mfista.h"
+#include "fft_lds.h"
+#include "mfista_kernels.h"
+#include "nufft.h"
+
+namespace mf {
+
+// grow-only device allocation
+struct DevMem {
+  void* p = nullptr;
+  size_t n = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+struct NufftState {
+  bool ready = false, have64 = false;
+  int preset = -1, NX = 0, NY = 0, M = 0, MSP = 0;
+  GridGeom geom;
+  FFTPlan Px, Py;
+  DevMem twxf, twyf, twxd, twyd, e4xf, e4yf, e4xd, e4yd;
+  DevMem mxy, af, e1f, visf, wf, ad, e1d, visd, wd, perm, yneg;
+  DevMem keys_a, keys_b, idx_a, vals_a, ent, tstart, tend, sort_tmp, err, ones, zero_vis;
+  DevMem T, Gf, Hg, U, rw, T64, Gf64, Hg64, U64, rw64;
+  VisParams<float> pf() {
+    return {(int2*)mxy.p, (float2*)af.p, (float*)e1f.p, (float2*)visf.p, (float*)wf.p};
+  }
+  VisParams<double> pd() {
+    return {(int2*)mxy.p, (double2*)ad.p, (double*)e1d.p, (double2*)visd.p, (double*)wd.p};
+  }
+};
+
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int* hflags = nullptr;  // pinned
+  long last_steps = 0;
+  NufftState nu;
+  // solver workspaces
+  DevMem x, z, xnew, g, x64, part, stat, stat64, chi64, state, cost;
+  DevMem fgp;  // TV prox duals
+  // uploads of the host-pointer entry points
+  DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
+};
+
+struct SolveArgs {
+  const double *u = nullptr, *v = nullptr, *vr = nullptr, *vi = nullptr, *sd = nullptr;
+  int M = 0, NX = 0, NY = 0, maxiter = 0, nonneg = 0, box_flag = 0;
+  double eps = 0, eps_abs = 0, lambda_l1 = 0, lambda_tv = 0, lambda_tsv = 0, cinit = 0;
+  const double* xinit = nullptr;
+  double* xout = nullptr;
+  const float* box = nullptr;
+};
+
+Context* get_context();
+std::mutex& global_mutex();
+int upload(Context& C, DevMem& m, const void* h, size_t bytes);
+int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, const double* d_vr,
+                const double* d_vi, const double* d_sd, int M, int NX, int NY);
+int nufft_ensure64(Context& C);
+template <typename R>
+void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* model,
+               const int* skip);
+template <typename R>
+void nufft_adj(Context& C, const cplx<R>* rw, R* g, const int* skip);
+int image_stats64(Context& C, const double* d_x64, int NX, int NY, double out[4]);
+int nufft_chi2_64(Context& C, const double* d_x64, double* chi2);
+int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts);
+int solve_nufft_tv(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts);
+
+}  // namespace mf

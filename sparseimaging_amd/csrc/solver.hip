@@ -1,0 +1,748 @@
+// solver.hip -- host orchestration and the C ABI of libmfista_amd.so.
+//
+// Entry points replace c/mfista.h:151-178 and c++/mfista.hpp:117-140.  A
+// per-process context owns one HIP stream and grow-only device workspaces,
+// so repeated solves of the same shape allocate nothing.  The NUFFT set-up
+// (reference preNUFFT, c/mfista_nufft_lib.c:50-121) runs on the GPU: cell
+// indices and tap slopes per visibility, a radix sort by gridding tile for
+// locality, and the per-tile visibility lists of the LDS gridding kernel.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/mfista.h"
+#include "mfista_kernels.h"
+#include "nufft.h"
+#include "solver.h"
+
+namespace mf {
+
+int sort_pairs(void* temp, size_t* temp_bytes, const int* kin, int* kout, const int* vin, int* vout,
+               int n, int end_bit, hipStream_t st);
+int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, const unsigned* vin,
+                 unsigned* vout, int n, int end_bit, hipStream_t st);
+
+int DevMem::ensure(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (bytes <= n) return 0;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    p = nullptr;
+    return -1;
+  }
+  n = bytes;
+  return 0;
+}
+
+void DevMem::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+static int upload_twiddles(const FFTPlan& P, DevMem& tf, DevMem& td, hipStream_t st) {
+  const int L = P.L;
+  std::vector<float2> hf(L);
+  std::vector<double2> hd(L);
+  for (int t = 0; t < L; t++) {
+    const double a = -2.0 * M_PI * (double)t / (double)L;
+    hd[t] = make_double2(cos(a), sin(a));
+    hf[t] = make_float2((float)cos(a), (float)sin(a));
+  }
+  if (tf.ensure(L * sizeof(float2)) || td.ensure(L * sizeof(double2))) return -1;
+  MF_CHECK(hipMemcpyAsync(tf.p, hf.data(), L * sizeof(float2), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipMemcpyAsync(td.p, hd.data(), L * sizeof(double2), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+static int bits_for(int v) {
+  int b = 1;
+  while ((1 << b) <= v) b++;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// NUFFT set-up on the device.  vr/vi/sd may be null (operator-only use).
+int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, const double* d_vr,
+                const double* d_vi, const double* d_sd, int M, int NX, int NY) {
+  hipStream_t st = C.stream;
+  // even sizes; 2N-point LDS transforms need 2 * 2N * 16 B <= 64 KiB in FP64
+  if (M <= 0 || NX < 4 || NY < 4 || (NX % 2) || (NY % 2) || NX > 1024 || NY > 1024)
+    return MFISTA_E_SIZE;
+  const int MSP = (preset == MFISTA_PRESET_CXX12) ? 6 : 12;
+  NufftState& N = C.nu;
+  N.preset = preset;
+  N.NX = NX;
+  N.NY = NY;
+  N.M = M;
+  N.MSP = MSP;
+  GridGeom& g = N.geom;
+  g.NX = NX;
+  g.NY = NY;
+  g.M = M;
+  g.MSP = MSP;
+  g.Lx = 2 * NX;
+  g.Ly = 2 * NY;
+  g.TX = 32;
+  g.TY = (MSP <= 8) ? 16 : 32;
+  g.ntx = (g.Lx + g.TX - 1) / g.TX;
+  g.nty = (NY + 1 + g.TY - 1) / g.TY;
+  const double pi = M_PI, Mrx = 2.0 * NX, Mry = 2.0 * NY;
+  double taux, tauy, coeff;
+  if (preset == MFISTA_PRESET_C24) {
+    taux = 12.0 / ((double)NX * NX);
+    tauy = 12.0 / ((double)NY * NY);
+    coeff = pi / sqrt(taux * tauy);
+  } else {
+    taux = (double)MSP / ((double)NX * NX);
+    tauy = taux;
+    coeff = ((double)NX * NY) * pi / ((double)MSP);
+  }
+  g.t3x = (pi / Mrx) * (pi / Mrx) / taux;
+  g.t3y = (pi / Mry) * (pi / Mry) / tauy;
+  g.inv_MM = 1.0 / (4.0 * (double)NX * NY);
+  if (!make_fft_plan(g.Lx, &N.Px) || !make_fft_plan(g.Ly, &N.Py)) return MFISTA_E_SIZE;
+  if (upload_twiddles(N.Px, N.twxf, N.twxd, st) || upload_twiddles(N.Py, N.twyf, N.twyd, st))
+    return MFISTA_E_HIP;
+  {  // separable deconvolution E4(i,j) = e4x[i] * e4y[j]
+    std::vector<double> ex(NX), ey(NY);
+    std::vector<float> exf(NX), eyf(NY);
+    for (int i = 0; i < NX; i++) {
+      const double ti = (double)(i - NX / 2);
+      ex[i] = coeff * exp(taux * ti * ti);
+      exf[i] = (float)ex[i];
+    }
+    // the C library centres the second axis with Nx/2 (c/mfista_nufft_lib.c:82)
+    const int cy = (preset == MFISTA_PRESET_C24) ? NX / 2 : NY / 2;
+    for (int j = 0; j < NY; j++) {
+      const double tj = (double)(j - cy);
+      ey[j] = exp(tauy * tj * tj);
+      eyf[j] = (float)ey[j];
+    }
+    if (N.e4xf.ensure(NX * 4) || N.e4yf.ensure(NY * 4) || N.e4xd.ensure(NX * 8) ||
+        N.e4yd.ensure(NY * 8))
+      return MFISTA_E_HIP;
+    MF_CHECK(hipMemcpyAsync(N.e4xf.p, exf.data(), NX * 4, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemcpyAsync(N.e4yf.p, eyf.data(), NY * 4, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemcpyAsync(N.e4xd.p, ex.data(), NX * 8, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemcpyAsync(N.e4yd.p, ey.data(), NY * 8, hipMemcpyHostToDevice, st));
+  }
+  const size_t Ms = (size_t)M;
+  const int ntiles = g.ntx * g.nty;
+  const size_t nent = 6 * Ms;
+  if (N.mxy.ensure(Ms * sizeof(int2)) || N.af.ensure(Ms * sizeof(float2)) ||
+      N.e1f.ensure(Ms * 4) || N.visf.ensure(Ms * sizeof(float2)) || N.wf.ensure(Ms * 4) ||
+      N.ad.ensure(Ms * sizeof(double2)) || N.e1d.ensure(Ms * 8) ||
+      N.visd.ensure(Ms * sizeof(double2)) || N.wd.ensure(Ms * 8) || N.perm.ensure(Ms * 4) ||
+      N.yneg.ensure(Ms * 4) || N.keys_a.ensure(nent * 4) || N.keys_b.ensure(nent * 4) ||
+      N.idx_a.ensure(Ms * 4) || N.vals_a.ensure(nent * 4) || N.ent.ensure(nent * 4) ||
+      N.tstart.ensure((size_t)ntiles * 4) || N.tend.ensure((size_t)ntiles * 4) ||
+      N.err.ensure(16) || N.zero_vis.ensure(Ms * sizeof(double2)))
+    return MFISTA_E_HIP;
+  const size_t nky = (size_t)NY + 1;
+  if (N.T.ensure(nky * NX * sizeof(float2)) ||
+      N.Gf.ensure(((size_t)NY + MSP) * g.Lx * sizeof(float2)) ||
+      N.Hg.ensure(nky * g.Lx * sizeof(float2)) || N.U.ensure((size_t)NX * nky * sizeof(float2)) ||
+      N.rw.ensure(Ms * sizeof(float2)))
+    return MFISTA_E_HIP;
+  N.have64 = false;
+  int* err = (int*)N.err.p;
+  MF_CHECK(hipMemsetAsync(err, 0, 16, st));
+  launch_vis_keys(d_u, d_v, M, NX, NY, preset, MSP, g.TX, g.TY, g.ntx, (int*)N.keys_a.p,
+                  (int*)N.idx_a.p, err, st);
+  const int kbits = bits_for(ntiles + 1);
+  size_t b1 = 0, b2 = 0;
+  sort_pairs(nullptr, &b1, (int*)N.keys_a.p, (int*)N.keys_b.p, (int*)N.idx_a.p, (int*)N.perm.p,
+             M, kbits, st);
+  sort_pairs_u(nullptr, &b2, (int*)N.keys_a.p, (int*)N.keys_b.p, (unsigned*)N.vals_a.p,
+               (unsigned*)N.ent.p, (int)nent, kbits, st);
+  size_t tb = b1 > b2 ? b1 : b2;
+  if (N.sort_tmp.ensure(tb)) return MFISTA_E_HIP;
+  tb = N.sort_tmp.n;
+  if (sort_pairs(N.sort_tmp.p, &tb, (int*)N.keys_a.p, (int*)N.keys_b.p, (int*)N.idx_a.p,
+                 (int*)N.perm.p, M, kbits, st))
+    return MFISTA_E_HIP;
+  // visibility data: missing arrays are read as zeros / unit sigma
+  const double* vr = d_vr;
+  const double* vi = d_vi;
+  const double* sd = d_sd;
+  if (!vr || !vi || !sd) {
+    if (N.ones.ensure(Ms * 8)) return MFISTA_E_HIP;
+    std::vector<double> one(Ms, 1.0);
+    MF_CHECK(hipMemcpyAsync(N.ones.p, one.data(), Ms * 8, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemsetAsync(N.zero_vis.p, 0, Ms * sizeof(double2), st));
+    vr = (const double*)N.zero_vis.p;
+    vi = (const double*)N.zero_vis.p;
+    sd = (const double*)N.ones.p;
+  }
+  launch_vis_params((const int*)N.perm.p, d_u, d_v, vr, vi, sd, M, NX, NY, preset, MSP, N.pf(),
+                    N.pd(), (int*)N.yneg.p, st);
+  launch_tile_entries((const int2*)N.mxy.p, M, NX, NY, MSP, g.TX, g.TY, g.ntx, g.nty,
+                      (int*)N.keys_a.p, (unsigned*)N.vals_a.p, st);
+  tb = N.sort_tmp.n;
+  if (sort_pairs_u(N.sort_tmp.p, &tb, (int*)N.keys_a.p, (int*)N.keys_b.p, (unsigned*)N.vals_a.p,
+                   (unsigned*)N.ent.p, (int)nent, kbits, st))
+    return MFISTA_E_HIP;
+  MF_CHECK(hipMemsetAsync(N.tstart.p, 0, (size_t)ntiles * 4, st));
+  MF_CHECK(hipMemsetAsync(N.tend.p, 0, (size_t)ntiles * 4, st));
+  launch_tile_ranges((const int*)N.keys_b.p, (int)nent, ntiles, (int*)N.tstart.p,
+                     (int*)N.tend.p, st);
+  int herr = 0;
+  MF_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+  MF_CHECK(hipStreamSynchronize(st));
+  MF_CHECK(hipGetLastError());
+  if (herr) return MFISTA_E_FOV;
+  N.ready = true;
+  return 0;
+}
+
+int nufft_ensure64(Context& C) {
+  NufftState& N = C.nu;
+  if (N.have64) return 0;
+  const GridGeom& g = N.geom;
+  const size_t nky = (size_t)N.NY + 1;
+  if (N.T64.ensure(nky * N.NX * sizeof(double2)) ||
+      N.Gf64.ensure(((size_t)N.NY + N.MSP) * g.Lx * sizeof(double2)) ||
+      N.Hg64.ensure(nky * g.Lx * sizeof(double2)) ||
+      N.U64.ensure((size_t)N.NX * nky * sizeof(double2)) ||
+      N.rw64.ensure((size_t)N.M * sizeof(double2)))
+    return MFISTA_E_HIP;
+  N.have64 = true;
+  return 0;
+}
+
+template <typename R>
+struct Sel;
+template <>
+struct Sel<float> {
+  static VisParams<float> vp(NufftState& N) { return N.pf(); }
+  static const float2* tx(NufftState& N) { return (const float2*)N.twxf.p; }
+  static const float2* ty(NufftState& N) { return (const float2*)N.twyf.p; }
+  static const float* ex(NufftState& N) { return (const float*)N.e4xf.p; }
+  static const float* ey(NufftState& N) { return (const float*)N.e4yf.p; }
+  static float2* T(NufftState& N) { return (float2*)N.T.p; }
+  static float2* Gf(NufftState& N) { return (float2*)N.Gf.p; }
+  static float2* Hg(NufftState& N) { return (float2*)N.Hg.p; }
+  static float2* U(NufftState& N) { return (float2*)N.U.p; }
+};
+template <>
+struct Sel<double> {
+  static VisParams<double> vp(NufftState& N) { return N.pd(); }
+  static const double2* tx(NufftState& N) { return (const double2*)N.twxd.p; }
+  static const double2* ty(NufftState& N) { return (const double2*)N.twyd.p; }
+  static const double* ex(NufftState& N) { return (const double*)N.e4xd.p; }
+  static const double* ey(NufftState& N) { return (const double*)N.e4yd.p; }
+  static double2* T(NufftState& N) { return (double2*)N.T64.p; }
+  static double2* Gf(NufftState& N) { return (double2*)N.Gf64.p; }
+  static double2* Hg(NufftState& N) { return (double2*)N.Hg64.p; }
+  static double2* U(NufftState& N) { return (double2*)N.U64.p; }
+};
+
+// A x: residual rw = w^2 (V' - A x) (optional), chi^2 partials (optional),
+// model visibilities in input order (optional)
+template <typename R>
+void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* model,
+               const int* skip) {
+  NufftState& N = C.nu;
+  const GridGeom& g = N.geom;
+  launch_rowfft_fwd<R>(x, Sel<R>::ex(N), Sel<R>::ey(N), Sel<R>::T(N), N.NX, N.NY, g.Ly, N.NY / 2,
+                       N.Py, Sel<R>::ty(N), C.stream, skip);
+  launch_colfft_fwd<R>(Sel<R>::T(N), Sel<R>::Gf(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.MSP,
+                       N.Px, Sel<R>::tx(N), C.stream, skip);
+  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), rw, chi_part, degrid_blocks(N.M, N.MSP),
+                   model, (const int*)N.perm.p, (const int*)N.yneg.p, C.stream, skip);
+}
+
+// g = Re(A^H r) from the reflected-frame weighted residual rw
+template <typename R>
+void nufft_adj(Context& C, const cplx<R>* rw, R* gout, const int* skip) {
+  NufftState& N = C.nu;
+  const GridGeom& g = N.geom;
+  launch_grid<R>(g, Sel<R>::vp(N), rw, (const int*)N.tstart.p, (const int*)N.tend.p,
+                 (const unsigned*)N.ent.p, Sel<R>::Hg(N), C.stream, skip);
+  launch_colfft_inv<R>(Sel<R>::Hg(N), Sel<R>::U(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.Px,
+                       Sel<R>::tx(N), C.stream, skip);
+  launch_rowfft_c2r<R>(Sel<R>::U(N), Sel<R>::ex(N), Sel<R>::ey(N),
+                       (R)(1.0 / (8.0 * (double)N.NX * N.NY)), gout, N.NX, N.NY, g.Ly, N.NY / 2,
+                       N.Py, Sel<R>::ty(N), C.stream, skip);
+}
+
+template void nufft_fwd<float>(Context&, const float*, float2*, double*, float2*, const int*);
+template void nufft_fwd<double>(Context&, const double*, double2*, double*, double2*, const int*);
+template void nufft_adj<float>(Context&, const float2*, float*, const int*);
+template void nufft_adj<double>(Context&, const double2*, double*, const int*);
+
+// gather y (input order) into the reflected frame, sorted order
+template <typename R>
+__global__ void k_gather_vis(const double* __restrict__ yr, const double* __restrict__ yi,
+                             const int* __restrict__ perm, const int* __restrict__ yneg, int M,
+                             cplx<R>* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const int k = perm[s];
+  const double im = yneg[s] ? -yi[k] : yi[k];
+  out[s] = mk<R>((R)yr[k], (R)im);
+}
+
+template <typename R>
+__global__ void k_split_model(const cplx<R>* __restrict__ m, double* __restrict__ yr,
+                              double* __restrict__ yi, int M) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  yr[k] = (double)m[k].x;
+  yi[k] = (double)m[k].y;
+}
+
+// ---------------------------------------------------------------------------
+static double host_sum(const std::vector<double>& v, int off, int cnt, int stride) {
+  double s = 0;
+  for (int i = 0; i < cnt; i++) s += v[(size_t)off + (size_t)i * stride];
+  return s;
+}
+
+// FP64 RESULT fields from a device FP64 image (the reference's calc_result_*).
+// chi2 = sum_k w_k^2 |V_k - (A x)_k|^2 is supplied by the caller.
+static void fill_result_common(RESULT* R, int NX, int NY, double lambda_l1, double lambda_tv,
+                               double lambda_tsv, double chi2, const double stats[4]) {
+  R->N = NX * NY;
+  R->NX = NX;
+  R->NY = NY;
+  R->lambda_l1 = lambda_l1;
+  R->lambda_tv = lambda_tv;
+  R->lambda_tsv = lambda_tsv;
+  R->sq_error = chi2;
+  R->l1cost = stats[0];
+  R->N_active = (int)llround(stats[3]);
+  R->finalcost = chi2 / 2;
+  if (lambda_l1 > 0) R->finalcost += lambda_l1 * R->l1cost;
+  if (lambda_tsv > 0) {
+    R->tsvcost = stats[1];
+    R->finalcost += lambda_tsv * R->tsvcost;
+  } else if (lambda_tv > 0) {
+    R->tvcost = stats[2];
+    R->finalcost += lambda_tv * R->tvcost;
+  }
+}
+
+int image_stats64(Context& C, const double* d_x64, int NX, int NY, double out[4]) {
+  const int nb = elem_blocks(NX * NY);
+  if (C.stat64.ensure((size_t)4 * nb * 8)) return MFISTA_E_HIP;
+  launch_stats<double>(d_x64, NX, NY, (double*)C.stat64.p, C.stream);
+  std::vector<double> h((size_t)4 * nb);
+  MF_CHECK(hipMemcpyAsync(h.data(), C.stat64.p, h.size() * 8, hipMemcpyDeviceToHost, C.stream));
+  MF_CHECK(hipStreamSynchronize(C.stream));
+  for (int q = 0; q < 4; q++) out[q] = host_sum(h, q, nb, 4);
+  return 0;
+}
+
+int nufft_chi2_64(Context& C, const double* d_x64, double* chi2) {
+  NufftState& N = C.nu;
+  if (nufft_ensure64(C)) return MFISTA_E_HIP;
+  const int nb = degrid_blocks(N.M, N.MSP);
+  if (C.chi64.ensure((size_t)nb * 8)) return MFISTA_E_HIP;
+  nufft_fwd<double>(C, d_x64, nullptr, (double*)C.chi64.p, nullptr, nullptr);
+  std::vector<double> h(nb);
+  MF_CHECK(hipMemcpyAsync(h.data(), C.chi64.p, nb * 8, hipMemcpyDeviceToHost, C.stream));
+  MF_CHECK(hipStreamSynchronize(C.stream));
+  *chi2 = host_sum(h, 0, nb, 1);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// MFISTA L1 + TSV with the NUFFT operator, FP32 operators, FP64 control.
+static int solve_nufft_tsv(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
+  NufftState& N = C.nu;
+  hipStream_t st = C.stream;
+  const int NX = A.NX, NY = A.NY, n = NX * NY;
+  const int W = NX, H = NY;  // TSV view of the flat image: NY rows of NX (x[NX*j+i])
+  PartialLayout L;
+  L.n_degrid = degrid_blocks(N.M, N.MSP);
+  L.n_elem = elem_blocks(n);
+  L.chi_z = 0;
+  L.chi_x = L.n_degrid;
+  L.tsv_z = 2 * L.n_degrid;
+  L.prox = L.tsv_z + L.n_elem;
+  L.total = L.prox + 4 * L.n_elem;
+  if (C.x.ensure(n * 4) || C.z.ensure(n * 4) || C.xnew.ensure(n * 4) || C.g.ensure(n * 4) ||
+      C.x64.ensure(n * 8) || C.part.ensure((size_t)L.total * 8) ||
+      C.stat.ensure((size_t)4 * L.n_elem * 8) || C.state.ensure(sizeof(DevState)) ||
+      C.cost.ensure((size_t)(A.maxiter > 0 ? A.maxiter : 1) * 8))
+    return MFISTA_E_HIP;
+  float* x = (float*)C.x.p;
+  float* z = (float*)C.z.p;
+  float* xnew = (float*)C.xnew.p;
+  float* g = (float*)C.g.p;
+  double* part = (double*)C.part.p;
+  DevState* S = (DevState*)C.state.p;
+  float2* rw = (float2*)N.rw.p;
+  launch_cast<double, float>(A.xinit, x, n, st);
+  MF_CHECK(hipMemcpyAsync(z, x, n * 4, hipMemcpyDeviceToDevice, st));
+  DevState h;
+  memset(&h, 0, sizeof(h));
+  h.c = A.cinit;
+  h.lambda_l1 = A.lambda_l1;
+  h.lambda_tsv = A.lambda_tsv;
+  h.lambda_tv = 0.0;
+  h.eps_abs = A.eps_abs;
+  h.rel_tol = opts ? opts->rel_tol : 0.0;
+  h.eta_first = (N.preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
+  h.maxiter = A.maxiter;
+  h.nonneg = A.nonneg;
+  h.tv_mode = 0;
+  MF_CHECK(hipMemcpyAsync(S, &h, sizeof(h), hipMemcpyHostToDevice, st));
+  // initial cost
+  nufft_fwd<float>(C, x, nullptr, part + L.chi_x, nullptr, nullptr);
+  launch_stats<float>(x, W, H, (double*)C.stat.p, st);
+  launch_init_control(S, part, L, (const double*)C.stat.p, L.n_elem, st);
+  const int* gp = &S->gate_prologue;
+  const int* gt = &S->gate_trial;
+  const float* box = A.box_flag ? A.box : nullptr;
+  const int K = 16;
+  int* hflag = C.hflags;
+  int pending = -1;
+  const long cap = (long)A.maxiter * 64 + 100000;
+  long step = 0;
+  for (;; step++) {
+    // iteration prologue: A z, residual, A^H r, TSV gradient
+    nufft_fwd<float>(C, z, rw, part + L.chi_z, nullptr, gp);
+    nufft_adj<float>(C, rw, g, gp);
+    if (A.lambda_tsv > 0) launch_tsv_grad<float>(z, g, W, H, A.lambda_tsv, part + L.tsv_z, st, gp);
+    // one backtracking trial
+    launch_prox<float>(z, g, xnew, box, W, H, S, part + L.prox, st, gt);
+    nufft_fwd<float>(C, xnew, nullptr, part + L.chi_x, nullptr, gt);
+    launch_control(S, (double*)C.cost.p, part, L, st);
+    launch_momentum<float>(x, xnew, z, n, S, st);
+    if ((step + 1) % K == 0) {
+      const int slot = (int)((step / K) & 1);
+      MF_CHECK(hipMemcpyAsync(&hflag[slot], &S->done, 4, hipMemcpyDeviceToHost, st));
+      MF_CHECK(hipEventRecord(C.ev[slot], st));
+      if (pending >= 0) {
+        MF_CHECK(hipEventSynchronize(C.ev[pending]));
+        if (hflag[pending]) break;
+      }
+      pending = slot;
+    }
+    if (step > cap) {
+      fprintf(stderr, "[mfista_amd] solver did not terminate\n");
+      return MFISTA_E_ARG;
+    }
+  }
+  MF_CHECK(hipStreamSynchronize(st));
+  MF_CHECK(hipGetLastError());
+  MF_CHECK(hipMemcpy(&h, S, sizeof(h), hipMemcpyDeviceToHost));
+  res->ITER = h.ITER;
+  res->Lip_const = h.c;
+  if (opts) opts->max_trials = h.max_trials;
+  C.last_steps = step + 1;
+  launch_cast<float, double>(x, A.xout, n, st);
+  return 0;
+}
+
+int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
+  const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
+  int tv_mode;
+  if (A.lambda_tv == 0)
+    tv_mode = 0;
+  else if (A.lambda_tsv == 0)
+    tv_mode = 1;
+  else {
+    printf("You cannot set both of lambda_TV and lambda_TSV positive.\n");
+    return MFISTA_E_BOTH_REG;
+  }
+  if (tv_mode && preset == MFISTA_PRESET_CXX12) {
+    printf("We have not implemented TV option.\n");
+    return MFISTA_E_TV_PRESET;
+  }
+  if (A.maxiter <= 0) return MFISTA_E_ARG;
+  auto t0 = std::chrono::steady_clock::now();
+  int rc = nufft_setup(C, preset, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY);
+  if (rc) {
+    if (rc == MFISTA_E_FOV) printf("Error: the field of view is too small for the (u,v) coverage.\n");
+    return rc;
+  }
+  // eps scaled by the mean |V|^2 (c/mfista_nufft_lib.c:923-926)
+  SolveArgs B = A;
+  {
+    double s = 0;
+    std::vector<double> hr(A.M), hi(A.M);
+    MF_CHECK(hipMemcpyAsync(hr.data(), A.vr, A.M * 8, hipMemcpyDeviceToHost, C.stream));
+    MF_CHECK(hipMemcpyAsync(hi.data(), A.vi, A.M * 8, hipMemcpyDeviceToHost, C.stream));
+    MF_CHECK(hipStreamSynchronize(C.stream));
+    for (int i = 0; i < A.M; i++) s += hr[i] * hr[i];
+    for (int i = 0; i < A.M; i++) s += hi[i] * hi[i];
+    B.eps_abs = A.eps * s / (double)A.M;
+  }
+  int iter_ok = 1;
+  if (tv_mode) {
+    rc = solve_nufft_tv(C, B, res, opts);
+  } else if (A.nonneg != 0 && A.nonneg != 1) {
+    printf("nonneg_flag must be chosen properly.\n");
+    iter_ok = 0;
+    res->ITER = 0;
+    res->Lip_const = A.cinit;
+    rc = 0;
+  } else {
+    rc = solve_nufft_tsv(C, B, res, opts);
+  }
+  if (rc) return rc;
+  MF_CHECK(hipStreamSynchronize(C.stream));
+  auto t1 = std::chrono::steady_clock::now();
+  const double secs = std::chrono::duration<double>(t1 - t0).count();
+  if (opts) opts->solve_ms = secs * 1e3;
+  res->comp_time = secs;
+  res->nonneg = A.nonneg;
+  res->maxiter = A.maxiter;
+  // FP64 result pass (calc_result_nufft, c/mfista_nufft_lib.c:791-905)
+  double chi2 = 0, stats[4] = {0, 0, 0, 0};
+  const double* xo = iter_ok ? A.xout : A.xout;
+  if ((rc = nufft_chi2_64(C, xo, &chi2))) return rc;
+  if ((rc = image_stats64(C, xo, A.NX, A.NY, stats))) return rc;
+  fill_result_common(res, A.NX, A.NY, A.lambda_l1, A.lambda_tv, A.lambda_tsv, chi2, stats);
+  res->M = (preset == MFISTA_PRESET_CXX12) ? A.M : (int)(A.M / 2);
+  res->mean_sq_error = res->sq_error / (double)A.M;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+static std::mutex g_mu;
+static std::vector<Context*> g_ctx;
+
+Context* get_context() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  if ((int)g_ctx.size() <= dev) g_ctx.resize(dev + 1, nullptr);
+  if (!g_ctx[dev]) {
+    Context* C = new Context();
+    C->device = dev;
+    if (hipStreamCreateWithFlags(&C->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete C;
+      return nullptr;
+    }
+    if (hipHostMalloc((void**)&C->hflags, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+      delete C;
+      return nullptr;
+    }
+    (void)hipEventCreateWithFlags(&C->ev[0], hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&C->ev[1], hipEventDisableTiming);
+    g_ctx[dev] = C;
+  }
+  return g_ctx[dev];
+}
+
+std::mutex& global_mutex() { return g_mu; }
+
+// upload helper for the host-pointer entry points
+int upload(Context& C, DevMem& m, const void* h, size_t bytes) {
+  if (m.ensure(bytes)) return MFISTA_E_HIP;
+  if (h) MF_CHECK(hipMemcpyAsync(m.p, h, bytes, hipMemcpyHostToDevice, C.stream));
+  return 0;
+}
+
+}  // namespace mf
+
+using namespace mf;
+
+extern "C" {
+
+const char* mfista_amd_version(void) { return "mfista_amd 0.1 (gfx950)"; }
+
+void mfista_default_opts(mfista_opts* o) {
+  memset(o, 0, sizeof(*o));
+  o->preset = MFISTA_PRESET_C24;
+  o->fgp_iter = FGPITER;
+  o->rel_tol = 0.0;
+}
+
+int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double* d_vis_r,
+                              const double* d_vis_i, const double* d_vis_std, int M, int NX,
+                              int NY, int maxiter, double eps, double lambda_l1,
+                              double lambda_tv, double lambda_tsv, double cinit,
+                              const double* d_xinit, double* d_xout, int nonneg_flag,
+                              int box_flag, const float* d_cl_box, struct RESULT* res,
+                              mfista_opts* opts, void* stream) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (!d_u || !d_v || !d_vis_r || !d_vis_i || !d_vis_std || !d_xinit || !d_xout || !res)
+    return MFISTA_E_ARG;
+  if (box_flag && !d_cl_box) return MFISTA_E_ARG;
+  hipStream_t saved = C->stream;
+  if (stream) C->stream = (hipStream_t)stream;
+  SolveArgs A;
+  A.u = d_u;
+  A.v = d_v;
+  A.vr = d_vis_r;
+  A.vi = d_vis_i;
+  A.sd = d_vis_std;
+  A.M = M;
+  A.NX = NX;
+  A.NY = NY;
+  A.maxiter = maxiter;
+  A.eps = eps;
+  A.lambda_l1 = lambda_l1;
+  A.lambda_tv = lambda_tv;
+  A.lambda_tsv = lambda_tsv;
+  A.cinit = cinit;
+  A.xinit = d_xinit;
+  A.xout = d_xout;
+  A.nonneg = nonneg_flag;
+  A.box_flag = box_flag;
+  A.box = d_cl_box;
+  int rc = solve_nufft(*C, A, res, opts);
+  C->stream = saved;
+  return rc;
+}
+
+int mfista_imaging_core_nufft_ex(const double* u_dx, const double* v_dy, const double* vis_r,
+                                 const double* vis_i, const double* vis_std, int M, int NX,
+                                 int NY, int maxiter, double eps, double lambda_l1,
+                                 double lambda_tv, double lambda_tsv, double cinit,
+                                 const double* xinit, double* xout, int nonneg_flag,
+                                 int box_flag, const float* cl_box, struct RESULT* res,
+                                 mfista_opts* opts) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (M <= 0 || NX <= 0 || NY <= 0 || !u_dx || !v_dy || !vis_r || !vis_i || !vis_std || !xinit ||
+      !xout || !res)
+    return MFISTA_E_ARG;
+  const size_t Mb = (size_t)M * 8, nb = (size_t)NX * NY * 8;
+  int rc;
+  if ((rc = upload(*C, C->h_u, u_dx, Mb)) || (rc = upload(*C, C->h_v, v_dy, Mb)) ||
+      (rc = upload(*C, C->h_vr, vis_r, Mb)) || (rc = upload(*C, C->h_vi, vis_i, Mb)) ||
+      (rc = upload(*C, C->h_sd, vis_std, Mb)) || (rc = upload(*C, C->h_x0, xinit, nb)) ||
+      (rc = upload(*C, C->h_x1, nullptr, nb)))
+    return rc;
+  if (box_flag) {
+    if (!cl_box) return MFISTA_E_ARG;
+    if ((rc = upload(*C, C->h_box, cl_box, (size_t)NX * NY * 4))) return rc;
+  }
+  SolveArgs A;
+  A.u = (const double*)C->h_u.p;
+  A.v = (const double*)C->h_v.p;
+  A.vr = (const double*)C->h_vr.p;
+  A.vi = (const double*)C->h_vi.p;
+  A.sd = (const double*)C->h_sd.p;
+  A.M = M;
+  A.NX = NX;
+  A.NY = NY;
+  A.maxiter = maxiter;
+  A.eps = eps;
+  A.lambda_l1 = lambda_l1;
+  A.lambda_tv = lambda_tv;
+  A.lambda_tsv = lambda_tsv;
+  A.cinit = cinit;
+  A.xinit = (const double*)C->h_x0.p;
+  A.xout = (double*)C->h_x1.p;
+  A.nonneg = nonneg_flag;
+  A.box_flag = box_flag;
+  A.box = box_flag ? (const float*)C->h_box.p : nullptr;
+  RESULT tmp = *res;
+  rc = solve_nufft(*C, A, &tmp, opts);
+  if (rc) return rc;
+  if (hipMemcpy(xout, C->h_x1.p, nb, hipMemcpyDeviceToHost) != hipSuccess) return MFISTA_E_HIP;
+  *res = tmp;
+  return 0;
+}
+
+void mfista_imaging_core_nufft(double* u_dx, double* v_dy, double* vis_r, double* vis_i,
+                               double* vis_std, int M, int NX, int NY, int maxiter, double eps,
+                               double lambda_l1, double lambda_tv, double lambda_tsv,
+                               double cinit, double* xinit, double* xout, int nonneg_flag,
+                               int box_flag, float* cl_box, struct RESULT* mfista_result) {
+  mfista_opts o;
+  mfista_default_opts(&o);
+  int rc = mfista_imaging_core_nufft_ex(u_dx, v_dy, vis_r, vis_i, vis_std, M, NX, NY, maxiter,
+                                        eps, lambda_l1, lambda_tv, lambda_tsv, cinit, xinit, xout,
+                                        nonneg_flag, box_flag, cl_box, mfista_result, &o);
+  if (rc && rc != MFISTA_E_BOTH_REG)
+    fprintf(stderr, "[mfista_amd] mfista_imaging_core_nufft failed with status %d\n", rc);
+}
+
+static int nufft_operator(int preset, int fp64, const double* u, const double* v, int M, int NX,
+                          int NY, const double* x_or_null, const double* yr, const double* yi,
+                          double* out_yr, double* out_yi, double* out_x) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (M <= 0 || NX <= 0 || NY <= 0 || !u || !v) return MFISTA_E_ARG;
+  const size_t Mb = (size_t)M * 8, nb = (size_t)NX * NY * 8;
+  int rc;
+  if ((rc = upload(*C, C->h_u, u, Mb)) || (rc = upload(*C, C->h_v, v, Mb))) return rc;
+  if ((rc = nufft_setup(*C, preset, (const double*)C->h_u.p, (const double*)C->h_v.p, nullptr,
+                        nullptr, nullptr, M, NX, NY)))
+    return rc;
+  NufftState& N = C->nu;
+  hipStream_t st = C->stream;
+  if (fp64 && nufft_ensure64(*C)) return MFISTA_E_HIP;
+  if (x_or_null) {  // forward
+    if ((rc = upload(*C, C->h_x0, x_or_null, nb))) return rc;
+    if (C->h_model.ensure((size_t)M * sizeof(double2)) || C->h_vr.ensure(Mb) ||
+        C->h_vi.ensure(Mb) || C->x.ensure(nb))
+      return MFISTA_E_HIP;
+    if (fp64) {
+      nufft_fwd<double>(*C, (const double*)C->h_x0.p, nullptr, nullptr, (double2*)C->h_model.p,
+                        nullptr);
+      hipLaunchKernelGGL(k_split_model<double>, dim3((M + 255) / 256), dim3(256), 0, st,
+                         (const double2*)C->h_model.p, (double*)C->h_vr.p, (double*)C->h_vi.p, M);
+    } else {
+      launch_cast<double, float>((const double*)C->h_x0.p, (float*)C->x.p, NX * NY, st);
+      nufft_fwd<float>(*C, (const float*)C->x.p, nullptr, nullptr, (float2*)C->h_model.p,
+                       nullptr);
+      hipLaunchKernelGGL(k_split_model<float>, dim3((M + 255) / 256), dim3(256), 0, st,
+                         (const float2*)C->h_model.p, (double*)C->h_vr.p, (double*)C->h_vi.p, M);
+    }
+    MF_CHECK(hipMemcpyAsync(out_yr, C->h_vr.p, Mb, hipMemcpyDeviceToHost, st));
+    MF_CHECK(hipMemcpyAsync(out_yi, C->h_vi.p, Mb, hipMemcpyDeviceToHost, st));
+  } else {  // adjoint
+    if ((rc = upload(*C, C->h_vr, yr, Mb)) || (rc = upload(*C, C->h_vi, yi, Mb))) return rc;
+    if (C->h_x1.ensure(nb) || C->x.ensure(nb)) return MFISTA_E_HIP;
+    if (fp64) {
+      hipLaunchKernelGGL(k_gather_vis<double>, dim3((M + 255) / 256), dim3(256), 0, st,
+                         (const double*)C->h_vr.p, (const double*)C->h_vi.p,
+                         (const int*)N.perm.p, (const int*)N.yneg.p, M, (double2*)N.rw64.p);
+      nufft_adj<double>(*C, (const double2*)N.rw64.p, (double*)C->h_x1.p, nullptr);
+    } else {
+      hipLaunchKernelGGL(k_gather_vis<float>, dim3((M + 255) / 256), dim3(256), 0, st,
+                         (const double*)C->h_vr.p, (const double*)C->h_vi.p,
+                         (const int*)N.perm.p, (const int*)N.yneg.p, M, (float2*)N.rw.p);
+      nufft_adj<float>(*C, (const float2*)N.rw.p, (float*)C->x.p, nullptr);
+      launch_cast<float, double>((const float*)C->x.p, (double*)C->h_x1.p, NX * NY, st);
+    }
+    MF_CHECK(hipMemcpyAsync(out_x, C->h_x1.p, nb, hipMemcpyDeviceToHost, st));
+  }
+  MF_CHECK(hipStreamSynchronize(st));
+  MF_CHECK(hipGetLastError());
+  return 0;
+}
+
+int mfista_nufft_forward(int preset, int fp64, const double* u, const double* v, int M, int NX,
+                         int NY, const double* x, double* y_r, double* y_i) {
+  if (!x || !y_r || !y_i) return MFISTA_E_ARG;
+  return nufft_operator(preset, fp64, u, v, M, NX, NY, x, nullptr, nullptr, y_r, y_i, nullptr);
+}
+
+int mfista_nufft_adjoint(int preset, int fp64, const double* u, const double* v, int M, int NX,
+                         int NY, const double* y_r, const double* y_i, double* x) {
+  if (!y_r || !y_i || !x) return MFISTA_E_ARG;
+  return nufft_operator(preset, fp64, u, v, M, NX, NY, nullptr, y_r, y_i, nullptr, nullptr, x);
+}
+
+void x2y_nufft(double* u_dx, double* v_dy, int M, int Nx, int Ny, double* xin, double* y_r,
+               double* y_i) {
+  int rc = mfista_nufft_forward(MFISTA_PRESET_CXX12, 1, u_dx, v_dy, M, Nx, Ny, xin, y_r, y_i);
+  if (rc) fprintf(stderr, "[mfista_amd] x2y_nufft failed with status %d\n", rc);
+}
+
+void y2x_nufft(double* u_dx, double* v_dy, int M, int Nx, int Ny, double* yin_r, double* yin_i,
+               double* xout) {
+  int rc = mfista_nufft_adjoint(MFISTA_PRESET_CXX12, 1, u_dx, v_dy, M, Nx, Ny, yin_r, yin_i, xout);
+  if (rc) fprintf(stderr, "[mfista_amd] y2x_nufft failed with status %d\n", rc);
+}
+
+}  // extern "C"

@@ -1,0 +1,60 @@
+"""GPU MFISTA (NUFFT, L1+TSV) against the FP64 oracle, both run to convergence.
+
+Parity bar: image relative L2 <= 1e-3 and final objective relative error
+<= 1e-6 (FP32 operators, FP64 objective/backtracking and RESULT pass).
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sparseimaging_amd as sa
+from sparseimaging_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(N, M, seed, amax):
+    return synth.make_nufft_problem(N=N, M=M, seed=seed, amax=amax)
+
+
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_solve_l1_tsv_matches_oracle(preset):
+    P = _problem(64, 2000, 1234, 0.35)
+    kw = dict(maxiter=20000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=10.0, cinit=1e4,
+              nonneg_flag=1, rel_tol=1e-9)
+    print(f"case preset={preset} N=64 M=2000", flush=True)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"],
+                                  nonneg=1, preset=preset, rel_tol=kw["rel_tol"])
+    xg, rg, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                                preset=preset, **kw)
+    xo, ro = o["x"], o["result"]
+    l2 = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    dcost = abs(rg["finalcost"] - ro["finalcost"]) / abs(ro["finalcost"])
+    print(f"  ITER gpu={rg['ITER']} oracle={ro['ITER']}  image relL2={l2:.3e}  cost rel={dcost:.3e}"
+          f"  cost gpu={rg['finalcost']:.10e} oracle={ro['finalcost']:.10e}")
+    assert l2 <= 1e-3
+    assert dcost <= 1e-6
+    assert rg["M"] == (2000 if preset == sa.PRESET_CXX12 else 1000)
+    assert rg["N"] == 64 * 64 and rg["NX"] == 64
+
+
+def test_result_fields_consistent():
+    P = _problem(64, 1500, 99, 0.35)
+    x, r, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                           maxiter=300, eps=1e-5, lambda_l1=1.0, lambda_tsv=10.0,
+                                           cinit=1e4, nonneg_flag=1, preset=sa.PRESET_CXX12)
+    ro = oracle.result_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], x, 1.0, 0.0, 10.0,
+                             preset=oracle.PRESET_CXX12)
+    for k in ("sq_error", "l1cost", "tsvcost", "finalcost"):
+        assert abs(r[k] - ro[k]) <= 1e-9 * abs(ro[k]), k
+    assert r["N_active"] == ro["N_active"]
+    assert (x >= 0).all()
+
+
+def test_both_regularisers_rejected():
+    P = _problem(64, 500, 3, 0.35)
+    with pytest.raises(sa.MfistaError) as e:
+        sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                     maxiter=10, lambda_tv=1.0, lambda_tsv=1.0)
+    assert e.value.status == -1
