@@ -113,6 +113,7 @@ typedef struct mfista_opts {
   int verbose;      /* nonzero: print progress */
   int max_trials;   /* out: largest number of backtracking trials in one iteration */
   double solve_ms;  /* out: GPU wall time of the solve (excluding the FP64 RESULT pass) */
+  int total_trials; /* out: backtracking trials over the whole solve (forward NUFFTs of xnew) */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);
@@ -138,12 +139,37 @@ int mfista_nufft_solve_device(const double *d_u, const double *d_v, const double
                               int box_flag, const float *d_cl_box,
                               struct RESULT *mfista_result, mfista_opts *opts, void *stream);
 
+/* same arguments as mfista_imaging_core_fft (without fftw_plan_flag) plus options */
+int mfista_imaging_core_fft_ex(const int *u_idx, const int *v_idx, const double *y_r,
+                               const double *y_i, const double *noise_stdev, int M, int NX, int NY,
+                               int maxiter, double eps, double lambda_l1, double lambda_tv,
+                               double lambda_tsv, double cinit, const double *xinit, double *xout,
+                               int nonneg_flag, int box_flag, const float *cl_box,
+                               struct RESULT *mfista_result, mfista_opts *opts);
+
+/* same arguments as mfista_imaging_core_dft (M, N by value, no looe_flag) plus options */
+int mfista_imaging_core_dft_ex(const double *y, const double *A, int M, int N, int NX, int NY,
+                               int maxiter, double eps, double lambda_l1, double lambda_tv,
+                               double lambda_tsv, double cinit, const double *xinit, double *xout,
+                               int nonneg_flag, int box_flag, const float *cl_box,
+                               struct RESULT *mfista_result, mfista_opts *opts);
+
 /* standalone NUFFT operators with a chosen preset and precision (0 = FP32,
  * 1 = FP64); host pointers; return a status */
 int mfista_nufft_forward(int preset, int fp64, const double *u, const double *v, int M, int NX,
                          int NY, const double *x, double *y_r, double *y_i);
 int mfista_nufft_adjoint(int preset, int fp64, const double *u, const double *v, int M, int NX,
                          int NY, const double *y_r, const double *y_i, double *x);
+
+/* Tracing hook: re-runs each kernel of the last NUFFT solve `reps` times on
+ * the library stream with one HIP event pair per launch and writes the mean
+ * duration in ms of stage s to ms[s]: 0 forward row FFT, 1 forward column
+ * FFT, 2 degridding, 3 gridding, 4 adjoint column FFT, 5 adjoint row c2r,
+ * 6 TSV gradient, 7 prox.  Returns a status. */
+int mfista_profile_stages(int reps, double *ms, int nstage);
+
+/* number of trial steps the last NUFFT solve enqueued */
+long mfista_last_steps(void);
 
 const char *mfista_amd_version(void);
 

@@ -36,8 +36,10 @@ EXPORTED = [
     "mfista_imaging_core_nufft", "mfista_imaging_core_fft", "mfista_imaging_core_dft",
     "x2y_nufft", "y2x_nufft", "mfista_default_opts", "mfista_imaging_core_nufft_ex",
     "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
-    "mfista_amd_version",
+    "mfista_amd_version", "mfista_profile_stages", "mfista_last_steps",
+    "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex",
 ]
+STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
 
 class RESULT(C.Structure):
@@ -58,7 +60,7 @@ class RESULT(C.Structure):
 class Opts(C.Structure):
     _fields_ = [
         ("preset", C.c_int), ("fgp_iter", C.c_int), ("rel_tol", C.c_double), ("verbose", C.c_int),
-        ("max_trials", C.c_int), ("solve_ms", C.c_double),
+        ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
     ]
 
 
@@ -112,6 +114,16 @@ def _declare(L):
     L.mfista_nufft_forward.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
     L.mfista_nufft_adjoint.restype = i
     L.mfista_nufft_adjoint.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
+    L.mfista_imaging_core_fft_ex.restype = i
+    L.mfista_imaging_core_fft_ex.argtypes = [_ip, _ip, _dp, _dp, _dp, i, i, i, i, d, d, d, d, d, _dp, _dp,
+                                             i, i, _fp, C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_imaging_core_dft_ex.restype = i
+    L.mfista_imaging_core_dft_ex.argtypes = [_dp, _dp, i, i, i, i, i, d, d, d, d, d, _dp, _dp, i, i, _fp,
+                                             C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_profile_stages.restype = i
+    L.mfista_profile_stages.argtypes = [i, _dp, i]
+    L.mfista_last_steps.restype = C.c_long
+    L.mfista_last_steps.argtypes = []
     L.mfista_amd_version.restype = C.c_char_p
     L.mfista_amd_version.argtypes = []
 
@@ -142,6 +154,10 @@ def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0):
     return o
 
 
+def _info(o):
+    return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials}
+
+
 # ---------------------------------------------------------------- solvers
 def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
@@ -166,7 +182,56 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
                                             C.byref(res), C.byref(o))
     if rc != 0:
         raise MfistaError(rc)
-    return xout.reshape(NX, NY), res.as_dict(), {"max_trials": o.max_trials, "solve_ms": o.solve_ms}
+    return xout.reshape(NX, NY), res.as_dict(), _info(o)
+
+
+def mfista_imaging_core_fft(u_idx, v_idx, y_r, y_i, noise_stdev, NX, NY, maxiter=MAXITER, eps=EPS,
+                            lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
+                            nonneg_flag=1, cl_box=None, fgp_iter=FGPITER, rel_tol=0.0):
+    """FFT-mode MFISTA on the GPU, FP64 (c/mfista_fft_lib.c:701).  u_idx/v_idx are
+    0-based grid indices.  Returns (x[NX,NY], RESULT dict, info)."""
+    ui, pui = _i(u_idx)
+    vi_, pvi_ = _i(v_idx)
+    yr, pyr = _d(y_r)
+    yi, pyi = _d(y_i)
+    ns, pns = _d(noise_stdev)
+    x0 = np.zeros(NX * NY) if xinit is None else np.asarray(xinit, dtype=np.float64).ravel()
+    x0, px0 = _d(x0)
+    xout = np.zeros(NX * NY)
+    bflag, b, pb = _box(cl_box, NX * NY)
+    res = RESULT()
+    o = make_opts(PRESET_C24, fgp_iter, rel_tol)
+    rc = lib().mfista_imaging_core_fft_ex(pui, pvi_, pyr, pyi, pns, ui.size, NX, NY, maxiter, eps, lambda_l1,
+                                          lambda_tv, lambda_tsv, cinit, px0, xout.ctypes.data_as(_dp),
+                                          nonneg_flag, bflag, pb, C.byref(res), C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    return xout.reshape(NX, NY), res.as_dict(), _info(o)
+
+
+def mfista_imaging_core_dft(y, A, NX, NY, maxiter=MAXITER, eps=EPS, lambda_l1=1.0, lambda_tv=0.0,
+                            lambda_tsv=0.0, cinit=1e4, xinit=None, nonneg_flag=1, cl_box=None,
+                            fgp_iter=FGPITER, rel_tol=0.0):
+    """DFT-mode MFISTA on the GPU, FP64 (c/mfista_dft_lib.c:540).  A is an M x N
+    array (passed column-major).  Returns (x[NX,NY], RESULT dict, info, residual)."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    M, N = A.shape
+    y, py = _d(y)
+    x0 = np.zeros(N) if xinit is None else np.asarray(xinit, dtype=np.float64).ravel()
+    x0, px0 = _d(x0)
+    xout = np.zeros(N)
+    bflag, b, pb = _box(cl_box, N)
+    res = RESULT()
+    o = make_opts(PRESET_C24, fgp_iter, rel_tol)
+    rc = lib().mfista_imaging_core_dft_ex(py, A.ctypes.data_as(_dp), M, N, NX, NY, maxiter, eps, lambda_l1,
+                                          lambda_tv, lambda_tsv, cinit, px0, xout.ctypes.data_as(_dp),
+                                          nonneg_flag, bflag, pb, C.byref(res), C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    resid = np.ctypeslib.as_array(res.residual, shape=(M,)).copy() if res.residual else None
+    if res.residual:
+        C.CDLL(None).free(C.cast(res.residual, C.c_void_p))
+    return xout.reshape(NX, NY), res.as_dict(), _info(o), resid
 
 
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
@@ -192,7 +257,7 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
                                          C.byref(o), stream)
     if rc != 0:
         raise MfistaError(rc)
-    return res.as_dict(), {"max_trials": o.max_trials, "solve_ms": o.solve_ms}
+    return res.as_dict(), _info(o)
 
 
 def nufft_forward(u, v, x, preset=PRESET_C24, fp64=False):
@@ -245,6 +310,20 @@ def y2x_nufft(u_dx, v_dy, yin_r, yin_i, NX, NY):
     x = np.zeros((NX, NY))
     lib().y2x_nufft(pu, pv, u.size, NX, NY, pyr, pyi, x.ctypes.data_as(_dp))
     return x
+
+
+def profile_stages(reps=20):
+    """Mean per-launch duration (ms) of each kernel of the last NUFFT solve,
+    timed with HIP events on the library's stream."""
+    ms = np.zeros(len(STAGES))
+    rc = lib().mfista_profile_stages(reps, ms.ctypes.data_as(_dp), len(STAGES))
+    if rc != 0:
+        raise MfistaError(rc)
+    return dict(zip(STAGES, ms.tolist()))
+
+
+def last_steps():
+    return int(lib().mfista_last_steps())
 
 
 def version():

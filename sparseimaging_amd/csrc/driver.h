@@ -1,0 +1,55 @@
+// driver.h -- the MFISTA driver shared by the NUFFT, FFT and DFT modes.
+#pragma once
+
+#include "solver.h"
+
+namespace mf {
+
+// A data-term operator.  fwd(x) writes per-block partial sums whose total is
+// 2 F(x) (F = chi^2 / 2 in NUFFT/DFT mode, the half-plane norm / 4 in FFT
+// mode); with keep = true it also keeps what adj() needs.  adj(g) writes
+// g = -dF/dx at the last kept point.  Every launch is gated by `skip`.
+template <typename R>
+struct Operator {
+  virtual ~Operator() = default;
+  virtual int npart() const = 0;
+  virtual void fwd(const R* x, bool keep, double* part, const int* skip) = 0;
+  virtual void adj(R* g, const int* skip) = 0;
+  // optional: forward of a step d from the kept point; partials of 2 F(z + d)
+  // and of |W A d|^2 (enables the cancellation-free backtracking test)
+  virtual bool has_delta() const { return false; }
+  virtual void fwd_delta(const R*, double*, double*, const int*) {}
+};
+
+struct RunParams {
+  int NX = 0, NY = 0, maxiter = 0, nonneg = 0, box_flag = 0, tv_mode = 0, fgp_iter = 100;
+  double eps_abs = 0, rel_tol = 0, lambda_l1 = 0, lambda_tv = 0, lambda_tsv = 0, cinit = 0;
+  double eta_first = 1.1;
+  const double* xinit = nullptr;  // device, FP64
+  double* xout = nullptr;         // device, FP64
+  const float* box = nullptr;     // device
+};
+
+struct RunStats {
+  int ITER = 0, max_trials = 0, total_trials = 0, status = 0;
+  double c = 0;
+  long steps = 0;
+};
+
+template <typename R>
+int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
+
+// FGP TV prox kernels (reference c/mfista_TV_lib.c:189-384)
+template <typename R>
+void launch_tv_pre(const R* z, const R* g, R* b, int n, const DevState* S, hipStream_t st,
+                   const int* skip);
+template <typename R>
+void launch_fgp_iter(const R* b, const float* box, const R* rin, const R* sin, R* rout, R* sout,
+                     R* p, R* q, int W, int H, const DevState* S, double c1, double c2, int first,
+                     hipStream_t st, const int* skip);
+template <typename R>
+void launch_fgp_final(const R* b, const float* box, const R* p, const R* q, const R* z,
+                      const R* g, R* xnew, R* d, int W, int H, const DevState* S, double* part,
+                      int have_dual, hipStream_t st, const int* skip);
+
+}  // namespace mf

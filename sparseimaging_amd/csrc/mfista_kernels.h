@@ -7,6 +7,15 @@
 // kernel reduces the FP64 partial sums, applies the F <= Q test, updates c, mu
 // and the cost history in FP64 and writes the gates that make the next step's
 // kernels run or return at once.  The host only polls `done` every few steps.
+//
+// The backtracking test F(xnew) <= Q(xnew, z) of the reference is, because the
+// adjoint used for the gradient is the exact adjoint of the forward operator,
+// algebraically the same as  1/2 |W A d|^2 + lambda_tsv TSV(d) <= c/2 |d|^2
+// with d = xnew - z (TSV is quadratic).  With FP32 operators the second form
+// is used (lip_mode = 1): it is a comparison of two sums of squares, free of
+// the cancellation between chi^2(xnew) and chi^2(z) that makes the first form
+// flip on rounding noise near convergence.  FP64 operators use the reference
+// form unchanged (lip_mode = 0).
 #pragma once
 
 #include "common.h"
@@ -21,13 +30,14 @@ struct DevState {
   double c, mu, costtmp, Qcore, lambda_l1, lambda_tsv, lambda_tv, eps_abs, rel_tol, eta_first;
   double mom_a, mom_b, l1_x, last_F, last_Q;
   int iter, trial, done, ITER, maxiter, nonneg, tv_mode, max_trials, accept_x, status;
+  int total_trials, lip_mode;
   int gate_prologue, gate_trial, gate_momentum;  // nonzero: the kernel returns at once
-  int pad;
 };
 
 // offsets (in doubles) of the partial-sum regions written by the kernels
+constexpr int kProxW = 6;  // per-block partials of the trial kernel
 struct PartialLayout {
-  int chi_z, chi_x, tsv_z, prox;  // prox: 4 values per block
+  int chi_z, chi_x, lip, tsv_z, prox;  // prox: kProxW values per block
   int n_degrid, n_elem, total;
 };
 
@@ -35,8 +45,8 @@ template <typename R>
 void launch_tsv_grad(const R* z, R* g, int W, int H, double lambda_tsv, double* part,
                      hipStream_t st, const int* skip);
 template <typename R>
-void launch_prox(const R* z, const R* g, R* xnew, const float* box, int W, int H,
-                 const DevState* S, double* part4, hipStream_t st, const int* skip);
+void launch_prox(const R* z, const R* g, R* xnew, R* d, const float* box, int W, int H,
+                 const DevState* S, double* part, hipStream_t st, const int* skip);
 template <typename R>
 void launch_stats(const R* x, int W, int H, double* part, hipStream_t st);  // 4 per block
 template <typename R>

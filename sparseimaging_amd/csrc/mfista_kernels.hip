@@ -57,14 +57,15 @@ __device__ inline R prox1(R v, R eta, int nonneg, bool boxed_out) {
   return 0;
 }
 
-// xnew = prox(z + g/c); partials: <xnew-z, g>, |xnew-z|^2, |xnew|_1, TSV(xnew)
+// xnew = prox(z + g/c), d = xnew - z; partials: <d, g>, |d|^2, |xnew|_1,
+// TSV(xnew), TSV(d)
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
-k_prox(const R* __restrict__ z, const R* __restrict__ g, R* __restrict__ xnew,
+k_prox(const R* __restrict__ z, const R* __restrict__ g, R* __restrict__ xnew, R* __restrict__ dout,
        const float* __restrict__ box, int W, int H, const DevState* __restrict__ S, double* part,
        const int* skip) {
   if (skip && *skip) return;
-  __shared__ double red[64];
+  __shared__ double red[16 * kProxW];
   const int n = W * H;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const double c = S->c;
@@ -72,12 +73,13 @@ k_prox(const R* __restrict__ z, const R* __restrict__ g, R* __restrict__ xnew,
   const R eta = (R)(S->lambda_l1 / c);
   const int nonneg = S->nonneg;
   const bool tsv = S->lambda_tsv > 0.0;
-  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  double v[kProxW] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (f < n) {
     const R gf = g[f], zf = z[f];
     const R xn = prox1<R>(gf * cinv + zf, eta, nonneg, box && box[f] == 0.0f);
     xnew[f] = xn;
     const R d = xn - zf;
+    if (dout) dout[f] = d;
     v[0] = (double)gf * (double)d;
     v[1] = (double)d * (double)d;
     v[2] = fabs((double)xn);
@@ -86,18 +88,22 @@ k_prox(const R* __restrict__ z, const R* __restrict__ g, R* __restrict__ xnew,
       if (cc < W - 1) {
         const R xr = prox1<R>(g[f + 1] * cinv + z[f + 1], eta, nonneg, box && box[f + 1] == 0.0f);
         const double t = (double)xn - (double)xr;
+        const double u = (double)d - ((double)xr - (double)z[f + 1]);
         v[3] += t * t;
+        v[4] += u * u;
       }
       if (r < H - 1) {
         const R xd = prox1<R>(g[f + W] * cinv + z[f + W], eta, nonneg, box && box[f + W] == 0.0f);
         const double t = (double)xn - (double)xd;
+        const double u = (double)d - ((double)xd - (double)z[f + W]);
         v[3] += t * t;
+        v[4] += u * u;
       }
     }
   }
-  block_sum<4>(v, red);
+  block_sum<kProxW>(v, red);
   if (threadIdx.x == 0)
-    for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = v[q];
+    for (int q = 0; q < kProxW; q++) part[kProxW * blockIdx.x + q] = v[q];
 }
 
 // |x|_1, TSV(x), TV(x), #(x != 0) partials (FP64 arithmetic on R inputs)
@@ -175,10 +181,12 @@ k_control(DevState* S, double* cost, const double* __restrict__ part, PartialLay
     tsv_z = block_reduce_range(part + L.tsv_z, L.n_elem, 1, red);
   }
   const double chi_x = block_reduce_range(part + L.chi_x, L.n_degrid, 1, red);
-  const double s_dg = block_reduce_range(part + L.prox + 0, L.n_elem, 4, red);
-  const double s_d2 = block_reduce_range(part + L.prox + 1, L.n_elem, 4, red);
-  const double l1n = block_reduce_range(part + L.prox + 2, L.n_elem, 4, red);
-  const double tsvn = block_reduce_range(part + L.prox + 3, L.n_elem, 4, red);
+  const double lip = S->lip_mode ? block_reduce_range(part + L.lip, L.n_degrid, 1, red) : 0.0;
+  const double s_dg = block_reduce_range(part + L.prox + 0, L.n_elem, kProxW, red);
+  const double s_d2 = block_reduce_range(part + L.prox + 1, L.n_elem, kProxW, red);
+  const double l1n = block_reduce_range(part + L.prox + 2, L.n_elem, kProxW, red);
+  const double tsvn = block_reduce_range(part + L.prox + 3, L.n_elem, kProxW, red);
+  const double tsvd = block_reduce_range(part + L.prox + 4, L.n_elem, kProxW, red);
   if (threadIdx.x != 0) return;
   const bool tsv = S->lambda_tsv > 0.0;
   const int it = S->iter;
@@ -191,6 +199,7 @@ k_control(DevState* S, double* cost, const double* __restrict__ part, PartialLay
   S->last_F = F;
   S->last_Q = Q;
   S->trial += 1;
+  S->total_trials += 1;
   if (!isfinite(F) || !isfinite(Q)) {  // non-finite objective: stop instead of backtracking forever
     S->status = 1;
     S->done = 1;
@@ -201,7 +210,23 @@ k_control(DevState* S, double* cost, const double* __restrict__ part, PartialLay
     return;
   }
   const double eta = (it == 0) ? S->eta_first : kEta;
-  const bool ok = (F <= Q);
+  bool ok;
+  if (S->lip_mode) {
+    const double lhs = lip / 2 + (tsv ? S->lambda_tsv * tsvd : 0.0);
+    const double rhs = S->c * s_d2 / 2;
+    if (!isfinite(lhs)) {
+      S->status = 1;
+      S->done = 1;
+      S->ITER = it + 1;
+      S->gate_prologue = 1;
+      S->gate_trial = 1;
+      S->gate_momentum = 1;
+      return;
+    }
+    ok = (lhs <= rhs);
+  } else {
+    ok = (F <= Q);
+  }
   if (!ok && S->trial < S->maxiter) {
     S->c *= eta;
     S->gate_prologue = 1;
@@ -215,6 +240,7 @@ k_control(DevState* S, double* cost, const double* __restrict__ part, PartialLay
   const double mu = S->mu;
   const double munew = (1 + sqrt(1 + 4 * mu * mu)) / 2;
   F += S->lambda_l1 * l1n;
+  if (S->tv_mode) F += S->lambda_tv * tsvn;  // slot 3 holds TV(xnew) in TV mode
   bool stop = false;
   if (F < cost[it]) {
     S->costtmp = F;
@@ -272,6 +298,7 @@ k_init_control(DevState* S, const double* __restrict__ part, PartialLayout L,
   S->done = 0;
   S->ITER = 0;
   S->max_trials = 0;
+  S->total_trials = 0;
   S->gate_prologue = 0;
   S->gate_trial = 0;
   S->gate_momentum = 1;
@@ -284,10 +311,10 @@ void launch_tsv_grad(const R* z, R* g, int W, int H, double lambda_tsv, double* 
                      (R)lambda_tsv, part, skip);
 }
 template <typename R>
-void launch_prox(const R* z, const R* g, R* xnew, const float* box, int W, int H,
-                 const DevState* S, double* part4, hipStream_t st, const int* skip) {
-  hipLaunchKernelGGL(k_prox<R>, dim3(elem_blocks(W * H)), dim3(MF_BLOCK), 0, st, z, g, xnew, box, W,
-                     H, S, part4, skip);
+void launch_prox(const R* z, const R* g, R* xnew, R* d, const float* box, int W, int H,
+                 const DevState* S, double* part, hipStream_t st, const int* skip) {
+  hipLaunchKernelGGL(k_prox<R>, dim3(elem_blocks(W * H)), dim3(MF_BLOCK), 0, st, z, g, xnew, d, box,
+                     W, H, S, part, skip);
 }
 template <typename R>
 void launch_stats(const R* x, int W, int H, double* part, hipStream_t st) {
@@ -313,7 +340,7 @@ void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st) {
 #define MF_INST2(R)                                                                          \
   template void launch_tsv_grad<R>(const R*, R*, int, int, double, double*, hipStream_t,     \
                                    const int*);                                              \
-  template void launch_prox<R>(const R*, const R*, R*, const float*, int, int,               \
+  template void launch_prox<R>(const R*, const R*, R*, R*, const float*, int, int,           \
                                const DevState*, double*, hipStream_t, const int*);           \
   template void launch_stats<R>(const R*, int, int, double*, hipStream_t);                   \
   template void launch_momentum<R>(R*, const R*, R*, int, const DevState*, hipStream_t);
@@ -321,5 +348,7 @@ MF_INST2(float)
 MF_INST2(double)
 template void launch_cast<float, double>(const float*, double*, int, hipStream_t);
 template void launch_cast<double, float>(const double*, float*, int, hipStream_t);
+template void launch_cast<double, double>(const double*, double*, int, hipStream_t);
+template void launch_cast<float, float>(const float*, float*, int, hipStream_t);
 
 }  // namespace mf

@@ -2,15 +2,16 @@
 //
 // Maths (Greengard & Lee 2004, as the reference computes it):
 //   forward  (A x)_k   = conj( E1_k / (4 NX NY) * sum_{l,l'} g_x(l) g_y(l') G[m_k+l, n_k+l'] )
-//   adjoint  Re(A^H r) = E4 / (8 NX NY) * c2r( S ),  S = G_r + conj(G_r(-k)) on ky >= 0
+//   adjoint  Re(A^H w r) = E4 / (8 NX NY) * c2r( S ),  S = G_r + conj(G_r(-k)) on ky >= 0
 // with G the unnormalised r2c spectrum of the centred, E4-weighted image padded
 // to 2NX x 2NY, g(l) = exp(l a - l^2 t3) the Gaussian tap weights, and G_r the
-// grid of E1 conj(w r) deposits.  Every visibility is first reflected to v >= 0
+// grid of E1 w conj(r) deposits (r: residual already weighted once by w = 1/sigma).  Every visibility is first reflected to v >= 0
 // (u,v -> -u,-v, vis -> conj vis), so all taps lie at ky >= -(MSP-1).
 //
 // HBM layout (complex = float2/double2):
 //   Gf [ky + MSP - 1][kx + NX]   forward spectrum, ky in [-(MSP-1), NY], 2NX columns
-//   Hg [ky][kx + NX]              adjoint grid, ky in [0, NY], 2NX columns
+//   Hg [ky][kx + NX]              adjoint grid, ky in [0, NY], 2NX columns (kept zero
+//                                 between adjoints: the column pass clears it)
 //   T  [ky][i]                    forward row-pass output (NY+1 rows of NX)
 //   U  [i][ky]                    adjoint column-pass output (NX rows of NY+1)
 #pragma once
@@ -47,7 +48,7 @@ void launch_colfft_fwd(const cplx<R>* T, cplx<R>* out, int NX, int nky, int Lx, 
                        int mode, int MSP, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
                        const int* skip);
 template <typename R>
-void launch_colfft_inv(const cplx<R>* in, cplx<R>* U, int NX, int nky, int Lx, int shift,
+void launch_colfft_inv(cplx<R>* in, cplx<R>* U, int NX, int nky, int Lx, int shift,
                        int mode, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
                        const int* skip);
 template <typename R>
@@ -55,13 +56,14 @@ void launch_rowfft_c2r(const cplx<R>* U, const R* e4x, const R* e4y, R scale, R*
                        int NY, int Ly, int shift, const FFTPlan& P, const cplx<R>* tw,
                        hipStream_t st, const int* skip);
 template <typename R>
-void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rw,
-                   double* chi_part, int nblk, cplx<R>* model, const int* perm,
-                   const int* yneg_sorted, hipStream_t st, const int* skip);
+void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rz,
+                   double* chi_part, double* lip_part, int mode, int nblk, cplx<R>* model,
+                   const int* perm, const int* yneg_sorted, hipStream_t st, const int* skip);
 template <typename R>
-void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rw,
-                 const int* tile_start, const int* tile_end, const unsigned* entries,
-                 cplx<R>* Hg, hipStream_t st, const int* skip);
+void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rz, const int4* chunks,
+                 int nchunks, const unsigned* entries, cplx<R>* Hg, hipStream_t st,
+                 const int* skip);
+constexpr int kGridChunk = 32;  // gridding entries per workgroup
 
 int degrid_blocks(int M, int MSP);
 

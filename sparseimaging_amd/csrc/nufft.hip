@@ -10,9 +10,10 @@
 //                 weighted residual and its FP64 chi^2 partial are fused in
 // Adjoint Re(A^H r) (reference NUFFT2d1, c/mfista_nufft_lib.c:123-185 and
 //               c++/mfista_nufft_lib.cpp:189-438):
-//   k_grid        one workgroup per LDS tile of the half-plane grid; each of the
-//                 4 wavefronts accumulates into its own LDS copy (lanes own
-//                 distinct taps, so no atomics), copies summed in fixed order
+//   k_grid        one workgroup per chunk of a tile's visibility list; each of
+//                 the 4 wavefronts accumulates into its own LDS copy (lanes own
+//                 distinct taps, so no LDS atomics), then one global atomic
+//                 add per touched cell
 //   k_colfft_inv  c2c backward along x per ky -> U[i][ky]
 //   k_rowfft_c2r  c2r along y (FFTW semantics), crop, * E4 / (8 NX NY)
 #include <hipcub/hipcub.hpp>
@@ -113,7 +114,7 @@ k_colfft_fwd(const cplx<R>* __restrict__ T, cplx<R>* __restrict__ out, int NX, i
 // mode 0: input in NUFFT layout Hg[ky][(kx + NX) mod Lx]; mode 1: plain in[ky][kx]
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
-k_colfft_inv(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ U, int NX, int nky, int Lx,
+k_colfft_inv(cplx<R>* __restrict__ in, cplx<R>* __restrict__ U, int NX, int nky, int Lx,
              int shift, int mode, FFTPlan P, const cplx<R>* __restrict__ tw, int nseq,
              const int* skip) {
   if (skip && *skip) return;
@@ -131,7 +132,12 @@ k_colfft_inv(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ U, int NX, in
       col = k + NXh;
       if (col >= Lx) col -= Lx;
     }
-    b0[t] = (ky < nky) ? in[(size_t)ky * Lx + col] : mk<R>(0, 0);
+    C v = mk<R>(0, 0);
+    if (ky < nky) {
+      v = in[(size_t)ky * Lx + col];
+      if (mode == 0) in[(size_t)ky * Lx + col] = mk<R>(0, 0);  // the gridder accumulates into zeros
+    }
+    b0[t] = v;
   }
   __syncthreads();
   C* res = lds_fft<R>(b0, b1, nseq, P, tw, +1);
@@ -191,10 +197,14 @@ k_rowfft_c2r(const cplx<R>* __restrict__ U, const R* __restrict__ e4x, const R* 
 
 // --------------------------------------------------------------------------
 // degridding: G lanes per visibility; lane l owns tap column kx = mx + l.
+// mode 0: r = (V' - A x) w, stored in rz, partial sum_k |r_k|^2
+// mode 1: x is a step d; t = (A d) w, r_new = rz - t; partials sum |r_new|^2 and
+//         sum |t|^2 (the Lipschitz-test term), rz left unchanged
 template <typename R, int G>
 __global__ void __launch_bounds__(MF_BLOCK)
-k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rw,
-         double* __restrict__ chi_part, cplx<R>* __restrict__ model, const int* __restrict__ perm,
+k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rz,
+         double* __restrict__ chi_part, double* __restrict__ lip_part, int mode,
+         cplx<R>* __restrict__ model, const int* __restrict__ perm,
          const int* __restrict__ yneg, const int* skip) {
   if (skip && *skip) return;
   using C = cplx<R>;
@@ -225,36 +235,52 @@ k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* _
     acc.x += __shfl_xor(acc.x, off, 64);
     acc.y += __shfl_xor(acc.y, off, 64);
   }
-  double chi[1] = {0.0};
+  double v[2] = {0.0, 0.0};
   if (s < g.M && lane == 0) {
     const R e1s = vp.e1[s] * (R)g.inv_MM;
     const C F = mk<R>(e1s * acc.x, -e1s * acc.y);
-    const C V = vp.vis[s];
     const R w = vp.w[s];
-    const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
-    chi[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-    if (rw) rw[s] = mk<R>(r.x * w, r.y * w);
+    if (mode == 0) {
+      const C V = vp.vis[s];
+      const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
+      v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+      if (rz) rz[s] = r;
+    } else {
+      const C t = mk<R>(F.x * w, F.y * w);
+      const C r0 = rz[s];
+      const C rn = mk<R>(r0.x - t.x, r0.y - t.y);
+      v[0] = (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+      v[1] = (double)t.x * (double)t.x + (double)t.y * (double)t.y;
+    }
     if (model) model[perm[s]] = yneg[s] ? cconj(F) : F;
   }
   if (chi_part) {
-    block_sum<1>(chi, red);
-    if (threadIdx.x == 0) chi_part[blockIdx.x] = chi[0];
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) {
+      chi_part[blockIdx.x] = v[0];
+      if (lip_part) lip_part[blockIdx.x] = v[1];
+    }
   }
 }
 
 // --------------------------------------------------------------------------
-// gridding: one workgroup (4 wavefronts) per TY x TX tile of Hg.
+// gridding: one workgroup (4 wavefronts) per chunk of at most kGridChunk
+// entries of one TY x TX tile of Hg.  Each wavefront accumulates into its own
+// LDS copy (its lanes own distinct taps, so no LDS atomics); the copies are
+// summed and the nonzero cells added to Hg with one global atomic each.  Heavy
+// tiles (dense uv tracks) are spread over many workgroups.  Hg is zeroed by
+// the adjoint column pass after it has read the grid.
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
-k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rw,
-       const int* __restrict__ tstart, const int* __restrict__ tend,
+k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rz, const int4* __restrict__ chunks,
        const unsigned* __restrict__ ent, cplx<R>* __restrict__ Hg, const int* skip) {
   if (skip && *skip) return;
   using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
   C* acc = reinterpret_cast<C*>(smem_d4);
   const int TX = g.TX, TY = g.TY, TT = TX * TY;
-  const int tile = blockIdx.x;
+  const int4 ch = chunks[blockIdx.x];
+  const int tile = ch.x;
   const int tr = tile / g.ntx, tc = tile - tr * g.ntx;
   const int rb = tr * TY, cb = tc * TX;
   for (int t = threadIdx.x; t < 4 * TT; t += blockDim.x) acc[t] = mk<R>(0, 0);
@@ -263,16 +289,15 @@ k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rw,
   C* my = acc + wave * TT;
   const int MSP = g.MSP, NX = g.NX;
   const R t3x = (R)g.t3x, t3y = (R)g.t3y;
-  const int e_end = tend[tile];
-  for (int e = tstart[tile] + wave; e < e_end; e += 4) {
+  for (int e = ch.y + wave; e < ch.z; e += 4) {
     const unsigned en = ent[e];
     const int s = (int)(en & 0x7fffffffu);
     const bool mir = (en >> 31) != 0;
     const int2 m = vp.mxy[s];
     const C a = vp.a[s];
-    const R e1 = vp.e1[s];
-    const C r = rw[s];
-    C d = mk<R>(e1 * r.x, -e1 * r.y);
+    const R ew = vp.e1[s] * vp.w[s];
+    const C r = rz[s];
+    C d = mk<R>(ew * r.x, -ew * r.y);
     int lp0, lp1, l0, l1;
     if (!mir) {
       lp0 = max(-MSP + 1, rb - m.y);
@@ -315,7 +340,11 @@ k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rw,
     sum = cadd(sum, acc[TT + t]);
     sum = cadd(sum, acc[2 * TT + t]);
     sum = cadd(sum, acc[3 * TT + t]);
-    if (row <= g.NY && col < g.Lx) Hg[(size_t)row * g.Lx + col] = sum;
+    if (row <= g.NY && col < g.Lx && (sum.x != (R)0 || sum.y != (R)0)) {
+      R* dst = reinterpret_cast<R*>(Hg + (size_t)row * g.Lx + col);
+      unsafeAtomicAdd(dst, sum.x);
+      unsafeAtomicAdd(dst + 1, sum.y);
+    }
   }
 }
 
@@ -446,10 +475,15 @@ __global__ void k_tile_ranges(const int* __restrict__ keys, int n, int ntiles,
 }
 
 // --------------------------------------------------------------------------
-static inline int fft_nseq(int L, size_t csize, int cap) {
+// sequences per workgroup: bounded by 64 KiB of LDS (two buffers) and chosen
+// so that a pass launches about 256 workgroups or more (one per CU)
+static inline int fft_nseq(int L, size_t csize, int nseq_total) {
   int n = (int)(65536 / (2 * (size_t)L * csize));
   if (n < 1) n = 1;
-  if (n > cap) n = cap;
+  int want = nseq_total / 256;
+  if (want < 1) want = 1;
+  if (n > want) n = want;
+  if (n > 8) n = 8;
   return n;
 }
 
@@ -457,7 +491,7 @@ template <typename R>
 void launch_rowfft_fwd(const R* x, const R* e4x, const R* e4y, cplx<R>* T, int NX, int NY, int Ly,
                        int shift, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
                        const int* skip) {
-  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), 8);
+  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), (NX + 1) / 2);
   const int nblk = (NX + 2 * nseq - 1) / (2 * nseq);
   const size_t sh = 2 * (size_t)nseq * Ly * sizeof(cplx<R>);
   hipLaunchKernelGGL(k_rowfft_fwd<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, x, e4x, e4y, T, NX, NY,
@@ -468,7 +502,7 @@ template <typename R>
 void launch_colfft_fwd(const cplx<R>* T, cplx<R>* out, int NX, int nky, int Lx, int shift,
                        int mode, int MSP, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
                        const int* skip) {
-  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), 8);
+  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), nky);
   const int nblk = (nky + nseq - 1) / nseq;
   const size_t sh = 2 * (size_t)nseq * Lx * sizeof(cplx<R>);
   hipLaunchKernelGGL(k_colfft_fwd<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, T, out, NX, nky, Lx,
@@ -476,10 +510,10 @@ void launch_colfft_fwd(const cplx<R>* T, cplx<R>* out, int NX, int nky, int Lx, 
 }
 
 template <typename R>
-void launch_colfft_inv(const cplx<R>* in, cplx<R>* U, int NX, int nky, int Lx, int shift,
+void launch_colfft_inv(cplx<R>* in, cplx<R>* U, int NX, int nky, int Lx, int shift,
                        int mode, const FFTPlan& P, const cplx<R>* tw, hipStream_t st,
                        const int* skip) {
-  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), 8);
+  const int nseq = fft_nseq(Lx, sizeof(cplx<R>), nky);
   const int nblk = (nky + nseq - 1) / nseq;
   const size_t sh = 2 * (size_t)nseq * Lx * sizeof(cplx<R>);
   hipLaunchKernelGGL(k_colfft_inv<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, in, U, NX, nky, Lx,
@@ -490,7 +524,7 @@ template <typename R>
 void launch_rowfft_c2r(const cplx<R>* U, const R* e4x, const R* e4y, R scale, R* g, int NX,
                        int NY, int Ly, int shift, const FFTPlan& P, const cplx<R>* tw,
                        hipStream_t st, const int* skip) {
-  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), 8);
+  const int nseq = fft_nseq(Ly, sizeof(cplx<R>), (NX + 1) / 2);
   const int nblk = (NX + 2 * nseq - 1) / (2 * nseq);
   const size_t sh = 2 * (size_t)nseq * Ly * sizeof(cplx<R>);
   hipLaunchKernelGGL(k_rowfft_c2r<R>, dim3(nblk), dim3(MF_BLOCK), sh, st, U, e4x, e4y, scale, g,
@@ -506,24 +540,25 @@ int degrid_blocks(int M, int MSP) {
 }
 
 template <typename R>
-void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rw,
-                   double* chi_part, int nblk, cplx<R>* model, const int* perm,
-                   const int* yneg_sorted, hipStream_t st, const int* skip) {
+void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rz,
+                   double* chi_part, double* lip_part, int mode, int nblk, cplx<R>* model,
+                   const int* perm, const int* yneg_sorted, hipStream_t st, const int* skip) {
   if (degrid_lanes(g.MSP) == 16)
-    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rw,
-                       chi_part, model, perm, yneg_sorted, skip);
+    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, chi_part,
+                       lip_part, mode, model, perm, yneg_sorted, skip);
   else
-    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rw,
-                       chi_part, model, perm, yneg_sorted, skip);
+    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, chi_part,
+                       lip_part, mode, model, perm, yneg_sorted, skip);
 }
 
 template <typename R>
-void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rw,
-                 const int* tile_start, const int* tile_end, const unsigned* entries,
-                 cplx<R>* Hg, hipStream_t st, const int* skip) {
+void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rz, const int4* chunks,
+                 int nchunks, const unsigned* entries, cplx<R>* Hg, hipStream_t st,
+                 const int* skip) {
+  if (nchunks <= 0) return;
   const size_t sh = 4 * (size_t)g.TX * g.TY * sizeof(cplx<R>);
-  hipLaunchKernelGGL(k_grid<R>, dim3(g.ntx * g.nty), dim3(MF_BLOCK), sh, st, g, vp, rw,
-                     tile_start, tile_end, entries, Hg, skip);
+  hipLaunchKernelGGL(k_grid<R>, dim3(nchunks), dim3(MF_BLOCK), sh, st, g, vp, rz, chunks, entries,
+                     Hg, skip);
 }
 
 void launch_vis_keys(const double* u, const double* v, int M, int NX, int NY, int preset,
@@ -572,16 +607,16 @@ int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, cons
                                      const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
   template void launch_colfft_fwd<R>(const cplx<R>*, cplx<R>*, int, int, int, int, int, int,     \
                                      const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
-  template void launch_colfft_inv<R>(const cplx<R>*, cplx<R>*, int, int, int, int, int,          \
+  template void launch_colfft_inv<R>(cplx<R>*, cplx<R>*, int, int, int, int, int,                \
                                      const FFTPlan&, const cplx<R>*, hipStream_t, const int*);   \
   template void launch_rowfft_c2r<R>(const cplx<R>*, const R*, const R*, R, R*, int, int, int,   \
                                      int, const FFTPlan&, const cplx<R>*, hipStream_t,           \
                                      const int*);                                                \
   template void launch_degrid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, cplx<R>*, \
-                                 double*, int, cplx<R>*, const int*, const int*, hipStream_t,    \
-                                 const int*);                                                    \
-  template void launch_grid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, const int*, \
-                               const int*, const unsigned*, cplx<R>*, hipStream_t, const int*);
+                                 double*, double*, int, int, cplx<R>*, const int*, const int*,   \
+                                 hipStream_t, const int*);                                       \
+  template void launch_grid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, const int4*, \
+                               int, const unsigned*, cplx<R>*, hipStream_t, const int*);
 
 MF_INST(float)
 MF_INST(double)

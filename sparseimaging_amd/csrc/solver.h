@@ -25,7 +25,8 @@ struct NufftState {
   FFTPlan Px, Py;
   DevMem twxf, twyf, twxd, twyd, e4xf, e4yf, e4xd, e4yd;
   DevMem mxy, af, e1f, visf, wf, ad, e1d, visd, wd, perm, yneg;
-  DevMem keys_a, keys_b, idx_a, vals_a, ent, tstart, tend, sort_tmp, err, ones, zero_vis;
+  DevMem keys_a, keys_b, idx_a, vals_a, ent, tstart, tend, sort_tmp, err, ones, zero_vis, chunks;
+  int nchunks = 0;
   DevMem T, Gf, Hg, U, rw, T64, Gf64, Hg64, U64, rw64;
   VisParams<float> pf() {
     return {(int2*)mxy.p, (float2*)af.p, (float*)e1f.p, (float2*)visf.p, (float*)wf.p};
@@ -43,7 +44,7 @@ struct Context {
   long last_steps = 0;
   NufftState nu;
   // solver workspaces
-  DevMem x, z, xnew, g, x64, part, stat, stat64, chi64, state, cost;
+  DevMem x, z, xnew, g, dstep, x64, part, stat, stat64, chi64, state, cost;
   DevMem fgp;  // TV prox duals
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
@@ -68,10 +69,11 @@ template <typename R>
 void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* model,
                const int* skip);
 template <typename R>
-void nufft_adj(Context& C, const cplx<R>* rw, R* g, const int* skip);
+void nufft_fwd_delta(Context& C, const R* d, double* chi_part, double* lip_part, const int* skip);
+template <typename R>
+void nufft_adj(Context& C, const cplx<R>* rz, R* g, const int* skip);
 int image_stats64(Context& C, const double* d_x64, int NX, int NY, double out[4]);
 int nufft_chi2_64(Context& C, const double* d_x64, double* chi2);
 int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts);
-int solve_nufft_tv(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts);
 
 }  // namespace mf

@@ -6,7 +6,9 @@
 // (reference preNUFFT, c/mfista_nufft_lib.c:50-121) runs on the GPU: cell
 // indices and tap slopes per visibility, a radix sort by gridding tile for
 // locality, and the per-tile visibility lists of the LDS gridding kernel.
+#include <algorithm>
 #include <chrono>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -15,6 +17,7 @@
 #include "../../include/mfista.h"
 #include "mfista_kernels.h"
 #include "nufft.h"
+#include "driver.h"
 #include "solver.h"
 
 namespace mf {
@@ -44,7 +47,7 @@ void DevMem::release() {
   n = 0;
 }
 
-static int upload_twiddles(const FFTPlan& P, DevMem& tf, DevMem& td, hipStream_t st) {
+int upload_twiddles_pub(const FFTPlan& P, DevMem& tf, DevMem& td, hipStream_t st) {
   const int L = P.L;
   std::vector<float2> hf(L);
   std::vector<double2> hd(L);
@@ -107,7 +110,7 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   g.t3y = (pi / Mry) * (pi / Mry) / tauy;
   g.inv_MM = 1.0 / (4.0 * (double)NX * NY);
   if (!make_fft_plan(g.Lx, &N.Px) || !make_fft_plan(g.Ly, &N.Py)) return MFISTA_E_SIZE;
-  if (upload_twiddles(N.Px, N.twxf, N.twxd, st) || upload_twiddles(N.Py, N.twyf, N.twyd, st))
+  if (upload_twiddles_pub(N.Px, N.twxf, N.twxd, st) || upload_twiddles_pub(N.Py, N.twyf, N.twyd, st))
     return MFISTA_E_HIP;
   {  // separable deconvolution E4(i,j) = e4x[i] * e4y[j]
     std::vector<double> ex(NX), ey(NY);
@@ -193,10 +196,24 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   launch_tile_ranges((const int*)N.keys_b.p, (int)nent, ntiles, (int*)N.tstart.p,
                      (int*)N.tend.p, st);
   int herr = 0;
+  std::vector<int> hs(ntiles), he(ntiles);
   MF_CHECK(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, st));
+  MF_CHECK(hipMemcpyAsync(hs.data(), N.tstart.p, (size_t)ntiles * 4, hipMemcpyDeviceToHost, st));
+  MF_CHECK(hipMemcpyAsync(he.data(), N.tend.p, (size_t)ntiles * 4, hipMemcpyDeviceToHost, st));
   MF_CHECK(hipStreamSynchronize(st));
   MF_CHECK(hipGetLastError());
   if (herr) return MFISTA_E_FOV;
+  // gridding work list: (tile, first entry, end entry) chunks of <= kGridChunk entries
+  std::vector<int4> ch;
+  for (int t = 0; t < ntiles; t++)
+    for (int e = hs[t]; e < he[t]; e += kGridChunk)
+      ch.push_back(make_int4(t, e, std::min(he[t], e + kGridChunk), 0));
+  N.nchunks = (int)ch.size();
+  if (N.chunks.ensure(ch.size() * sizeof(int4) + 16)) return MFISTA_E_HIP;
+  if (!ch.empty())
+    MF_CHECK(hipMemcpyAsync(N.chunks.p, ch.data(), ch.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipMemsetAsync(N.Hg.p, 0, nky * g.Lx * sizeof(float2), st));
+  MF_CHECK(hipStreamSynchronize(st));
   N.ready = true;
   return 0;
 }
@@ -212,6 +229,7 @@ int nufft_ensure64(Context& C) {
       N.U64.ensure((size_t)N.NX * nky * sizeof(double2)) ||
       N.rw64.ensure((size_t)N.M * sizeof(double2)))
     return MFISTA_E_HIP;
+  MF_CHECK(hipMemsetAsync(N.Hg64.p, 0, nky * g.Lx * sizeof(double2), C.stream));
   N.have64 = true;
   return 0;
 }
@@ -243,10 +261,10 @@ struct Sel<double> {
   static double2* U(NufftState& N) { return (double2*)N.U64.p; }
 };
 
-// A x: residual rw = w^2 (V' - A x) (optional), chi^2 partials (optional),
+// A x: residual rz = w (V' - A x) (optional), chi^2 partials (optional),
 // model visibilities in input order (optional)
 template <typename R>
-void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* model,
+void nufft_fwd(Context& C, const R* x, cplx<R>* rz, double* chi_part, cplx<R>* model,
                const int* skip) {
   NufftState& N = C.nu;
   const GridGeom& g = N.geom;
@@ -254,17 +272,33 @@ void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* m
                        N.Py, Sel<R>::ty(N), C.stream, skip);
   launch_colfft_fwd<R>(Sel<R>::T(N), Sel<R>::Gf(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.MSP,
                        N.Px, Sel<R>::tx(N), C.stream, skip);
-  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), rw, chi_part, degrid_blocks(N.M, N.MSP),
-                   model, (const int*)N.perm.p, (const int*)N.yneg.p, C.stream, skip);
+  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), rz, chi_part, nullptr, 0,
+                   degrid_blocks(N.M, N.MSP), model, (const int*)N.perm.p, (const int*)N.yneg.p,
+                   C.stream, skip);
 }
 
-// g = Re(A^H r) from the reflected-frame weighted residual rw
+// A d for a step d = xnew - z: partials of |rz - w A d|^2 (chi^2 at xnew) and
+// of |w A d|^2, the term of the backtracking test
 template <typename R>
-void nufft_adj(Context& C, const cplx<R>* rw, R* gout, const int* skip) {
+void nufft_fwd_delta(Context& C, const R* d, double* chi_part, double* lip_part, const int* skip) {
   NufftState& N = C.nu;
   const GridGeom& g = N.geom;
-  launch_grid<R>(g, Sel<R>::vp(N), rw, (const int*)N.tstart.p, (const int*)N.tend.p,
-                 (const unsigned*)N.ent.p, Sel<R>::Hg(N), C.stream, skip);
+  launch_rowfft_fwd<R>(d, Sel<R>::ex(N), Sel<R>::ey(N), Sel<R>::T(N), N.NX, N.NY, g.Ly, N.NY / 2,
+                       N.Py, Sel<R>::ty(N), C.stream, skip);
+  launch_colfft_fwd<R>(Sel<R>::T(N), Sel<R>::Gf(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.MSP,
+                       N.Px, Sel<R>::tx(N), C.stream, skip);
+  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), (cplx<R>*)(std::is_same<R, float>::value ? N.rw.p : N.rw64.p),
+                   chi_part, lip_part, 1, degrid_blocks(N.M, N.MSP), nullptr, nullptr, nullptr,
+                   C.stream, skip);
+}
+
+// g = Re(A^H w r) from the reflected-frame residual rz
+template <typename R>
+void nufft_adj(Context& C, const cplx<R>* rz, R* gout, const int* skip) {
+  NufftState& N = C.nu;
+  const GridGeom& g = N.geom;
+  launch_grid<R>(g, Sel<R>::vp(N), rz, (const int4*)N.chunks.p, N.nchunks, (const unsigned*)N.ent.p,
+                 Sel<R>::Hg(N), C.stream, skip);
   launch_colfft_inv<R>(Sel<R>::Hg(N), Sel<R>::U(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.Px,
                        Sel<R>::tx(N), C.stream, skip);
   launch_rowfft_c2r<R>(Sel<R>::U(N), Sel<R>::ex(N), Sel<R>::ey(N),
@@ -275,6 +309,7 @@ void nufft_adj(Context& C, const cplx<R>* rw, R* gout, const int* skip) {
 template void nufft_fwd<float>(Context&, const float*, float2*, double*, float2*, const int*);
 template void nufft_fwd<double>(Context&, const double*, double2*, double*, double2*, const int*);
 template void nufft_adj<float>(Context&, const float2*, float*, const int*);
+template void nufft_fwd_delta<float>(Context&, const float*, double*, double*, const int*);
 template void nufft_adj<double>(Context&, const double2*, double*, const int*);
 
 // gather y (input order) into the reflected frame, sorted order
@@ -354,94 +389,21 @@ int nufft_chi2_64(Context& C, const double* d_x64, double* chi2) {
 }
 
 // ---------------------------------------------------------------------------
-// MFISTA L1 + TSV with the NUFFT operator, FP32 operators, FP64 control.
-static int solve_nufft_tsv(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
-  NufftState& N = C.nu;
-  hipStream_t st = C.stream;
-  const int NX = A.NX, NY = A.NY, n = NX * NY;
-  const int W = NX, H = NY;  // TSV view of the flat image: NY rows of NX (x[NX*j+i])
-  PartialLayout L;
-  L.n_degrid = degrid_blocks(N.M, N.MSP);
-  L.n_elem = elem_blocks(n);
-  L.chi_z = 0;
-  L.chi_x = L.n_degrid;
-  L.tsv_z = 2 * L.n_degrid;
-  L.prox = L.tsv_z + L.n_elem;
-  L.total = L.prox + 4 * L.n_elem;
-  if (C.x.ensure(n * 4) || C.z.ensure(n * 4) || C.xnew.ensure(n * 4) || C.g.ensure(n * 4) ||
-      C.x64.ensure(n * 8) || C.part.ensure((size_t)L.total * 8) ||
-      C.stat.ensure((size_t)4 * L.n_elem * 8) || C.state.ensure(sizeof(DevState)) ||
-      C.cost.ensure((size_t)(A.maxiter > 0 ? A.maxiter : 1) * 8))
-    return MFISTA_E_HIP;
-  float* x = (float*)C.x.p;
-  float* z = (float*)C.z.p;
-  float* xnew = (float*)C.xnew.p;
-  float* g = (float*)C.g.p;
-  double* part = (double*)C.part.p;
-  DevState* S = (DevState*)C.state.p;
-  float2* rw = (float2*)N.rw.p;
-  launch_cast<double, float>(A.xinit, x, n, st);
-  MF_CHECK(hipMemcpyAsync(z, x, n * 4, hipMemcpyDeviceToDevice, st));
-  DevState h;
-  memset(&h, 0, sizeof(h));
-  h.c = A.cinit;
-  h.lambda_l1 = A.lambda_l1;
-  h.lambda_tsv = A.lambda_tsv;
-  h.lambda_tv = 0.0;
-  h.eps_abs = A.eps_abs;
-  h.rel_tol = opts ? opts->rel_tol : 0.0;
-  h.eta_first = (N.preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
-  h.maxiter = A.maxiter;
-  h.nonneg = A.nonneg;
-  h.tv_mode = 0;
-  MF_CHECK(hipMemcpyAsync(S, &h, sizeof(h), hipMemcpyHostToDevice, st));
-  // initial cost
-  nufft_fwd<float>(C, x, nullptr, part + L.chi_x, nullptr, nullptr);
-  launch_stats<float>(x, W, H, (double*)C.stat.p, st);
-  launch_init_control(S, part, L, (const double*)C.stat.p, L.n_elem, st);
-  const int* gp = &S->gate_prologue;
-  const int* gt = &S->gate_trial;
-  const float* box = A.box_flag ? A.box : nullptr;
-  const int K = 16;
-  int* hflag = C.hflags;
-  int pending = -1;
-  const long cap = (long)A.maxiter * 64 + 100000;
-  long step = 0;
-  for (;; step++) {
-    // iteration prologue: A z, residual, A^H r, TSV gradient
-    nufft_fwd<float>(C, z, rw, part + L.chi_z, nullptr, gp);
-    nufft_adj<float>(C, rw, g, gp);
-    if (A.lambda_tsv > 0) launch_tsv_grad<float>(z, g, W, H, A.lambda_tsv, part + L.tsv_z, st, gp);
-    // one backtracking trial
-    launch_prox<float>(z, g, xnew, box, W, H, S, part + L.prox, st, gt);
-    nufft_fwd<float>(C, xnew, nullptr, part + L.chi_x, nullptr, gt);
-    launch_control(S, (double*)C.cost.p, part, L, st);
-    launch_momentum<float>(x, xnew, z, n, S, st);
-    if ((step + 1) % K == 0) {
-      const int slot = (int)((step / K) & 1);
-      MF_CHECK(hipMemcpyAsync(&hflag[slot], &S->done, 4, hipMemcpyDeviceToHost, st));
-      MF_CHECK(hipEventRecord(C.ev[slot], st));
-      if (pending >= 0) {
-        MF_CHECK(hipEventSynchronize(C.ev[pending]));
-        if (hflag[pending]) break;
-      }
-      pending = slot;
-    }
-    if (step > cap) {
-      fprintf(stderr, "[mfista_amd] solver did not terminate\n");
-      return MFISTA_E_ARG;
-    }
+struct NufftOp : Operator<float> {
+  Context& C;
+  explicit NufftOp(Context& c) : C(c) {}
+  int npart() const override { return degrid_blocks(C.nu.M, C.nu.MSP); }
+  void fwd(const float* x, bool keep, double* part, const int* skip) override {
+    nufft_fwd<float>(C, x, keep ? (float2*)C.nu.rw.p : nullptr, part, nullptr, skip);
   }
-  MF_CHECK(hipStreamSynchronize(st));
-  MF_CHECK(hipGetLastError());
-  MF_CHECK(hipMemcpy(&h, S, sizeof(h), hipMemcpyDeviceToHost));
-  res->ITER = h.ITER;
-  res->Lip_const = h.c;
-  if (opts) opts->max_trials = h.max_trials;
-  C.last_steps = step + 1;
-  launch_cast<float, double>(x, A.xout, n, st);
-  return 0;
-}
+  void adj(float* g, const int* skip) override {
+    nufft_adj<float>(C, (const float2*)C.nu.rw.p, g, skip);
+  }
+  bool has_delta() const override { return true; }
+  void fwd_delta(const float* d, double* chi_part, double* lip_part, const int* skip) override {
+    nufft_fwd_delta<float>(C, d, chi_part, lip_part, skip);
+  }
+};
 
 int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
   const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
@@ -477,17 +439,39 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     for (int i = 0; i < A.M; i++) s += hi[i] * hi[i];
     B.eps_abs = A.eps * s / (double)A.M;
   }
-  int iter_ok = 1;
-  if (tv_mode) {
-    rc = solve_nufft_tv(C, B, res, opts);
-  } else if (A.nonneg != 0 && A.nonneg != 1) {
+  RunStats rs;
+  rs.c = A.cinit;
+  if (!tv_mode && A.nonneg != 0 && A.nonneg != 1) {
     printf("nonneg_flag must be chosen properly.\n");
-    iter_ok = 0;
-    res->ITER = 0;
-    res->Lip_const = A.cinit;
     rc = 0;
   } else {
-    rc = solve_nufft_tsv(C, B, res, opts);
+    NufftOp op(C);
+    RunParams P;
+    P.NX = A.NX;
+    P.NY = A.NY;
+    P.maxiter = A.maxiter;
+    P.nonneg = A.nonneg;
+    P.box_flag = A.box_flag;
+    P.tv_mode = tv_mode;
+    P.fgp_iter = (opts && opts->fgp_iter > 0) ? opts->fgp_iter : FGPITER;
+    P.eps_abs = B.eps_abs;
+    P.rel_tol = opts ? opts->rel_tol : 0.0;
+    P.lambda_l1 = A.lambda_l1;
+    P.lambda_tv = A.lambda_tv;
+    P.lambda_tsv = A.lambda_tsv;
+    P.cinit = A.cinit;
+    P.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
+    P.xinit = A.xinit;
+    P.xout = A.xout;
+    P.box = A.box;
+    rc = mfista_run<float>(C, op, P, &rs);
+    C.last_steps = rs.steps;
+  }
+  res->ITER = rs.ITER;
+  res->Lip_const = rs.c;
+  if (opts) {
+    opts->max_trials = rs.max_trials;
+    opts->total_trials = rs.total_trials;
   }
   if (rc) return rc;
   MF_CHECK(hipStreamSynchronize(C.stream));
@@ -499,7 +483,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
   res->maxiter = A.maxiter;
   // FP64 result pass (calc_result_nufft, c/mfista_nufft_lib.c:791-905)
   double chi2 = 0, stats[4] = {0, 0, 0, 0};
-  const double* xo = iter_ok ? A.xout : A.xout;
+  const double* xo = A.xout;
   if ((rc = nufft_chi2_64(C, xo, &chi2))) return rc;
   if ((rc = image_stats64(C, xo, A.NX, A.NY, stats))) return rc;
   fill_result_common(res, A.NX, A.NY, A.lambda_l1, A.lambda_tv, A.lambda_tsv, chi2, stats);
@@ -743,6 +727,93 @@ void y2x_nufft(double* u_dx, double* v_dy, int M, int Nx, int Ny, double* yin_r,
                double* xout) {
   int rc = mfista_nufft_adjoint(MFISTA_PRESET_CXX12, 1, u_dx, v_dy, M, Nx, Ny, yin_r, yin_i, xout);
   if (rc) fprintf(stderr, "[mfista_amd] y2x_nufft failed with status %d\n", rc);
+}
+
+
+// per-kernel timing of the last NUFFT solve's kernels, one event pair per
+// launch on the library stream (the tracing hook bench.py reads)
+int mfista_profile_stages(int reps, double* ms, int nstage) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C || !C->nu.ready || !C->x.p || reps <= 0 || !ms) return MFISTA_E_ARG;
+  NufftState& N = C->nu;
+  hipStream_t st = C->stream;
+  const int NX = N.NX, NY = N.NY, n = NX * NY;
+  const GridGeom& g = N.geom;
+  float* z = (float*)C->z.p;
+  float* xnew = (float*)C->xnew.p;
+  float* gr = (float*)C->g.p;
+  double* part = (double*)C->part.p;
+  const DevState* S = (const DevState*)C->state.p;
+  std::vector<hipEvent_t> e0(reps), e1(reps);
+  for (int r = 0; r < reps; r++) {
+    MF_CHECK(hipEventCreate(&e0[r]));
+    MF_CHECK(hipEventCreate(&e1[r]));
+  }
+  auto run = [&](int stage) {
+    switch (stage) {
+      case 0:
+        launch_rowfft_fwd<float>(z, (const float*)N.e4xf.p, (const float*)N.e4yf.p, (float2*)N.T.p, NX,
+                                 NY, g.Ly, NY / 2, N.Py, (const float2*)N.twyf.p, st, nullptr);
+        break;
+      case 1:
+        launch_colfft_fwd<float>((const float2*)N.T.p, (float2*)N.Gf.p, NX, NY + 1, g.Lx, NX / 2, 0,
+                                 N.MSP, N.Px, (const float2*)N.twxf.p, st, nullptr);
+        break;
+      case 2:
+        launch_degrid<float>(g, N.pf(), (const float2*)N.Gf.p, (float2*)N.rw.p, part, nullptr, 0,
+                             degrid_blocks(N.M, N.MSP), nullptr, nullptr, nullptr, st, nullptr);
+        break;
+      case 3:
+        launch_grid<float>(g, N.pf(), (const float2*)N.rw.p, (const int4*)N.chunks.p, N.nchunks,
+                           (const unsigned*)N.ent.p, (float2*)N.Hg.p, st, nullptr);
+        break;
+      case 4:
+        launch_colfft_inv<float>((float2*)N.Hg.p, (float2*)N.U.p, NX, NY + 1, g.Lx, NX / 2, 0,
+                                 N.Px, (const float2*)N.twxf.p, st, nullptr);
+        break;
+      case 5:
+        launch_rowfft_c2r<float>((const float2*)N.U.p, (const float*)N.e4xf.p, (const float*)N.e4yf.p,
+                                 1.0f / (8.0f * NX * NY), gr, NX, NY, g.Ly, NY / 2, N.Py,
+                                 (const float2*)N.twyf.p, st, nullptr);
+        break;
+      case 6:
+        launch_tsv_grad<float>(z, gr, NX, NY, 0.0, part, st, nullptr);
+        break;
+      case 7:
+        launch_prox<float>(z, gr, xnew, (float*)C->dstep.p, nullptr, NX, NY, S, part, st, nullptr);
+        break;
+    }
+  };
+  const int ns = nstage < 8 ? nstage : 8;
+  for (int stage = 0; stage < ns; stage++) {
+    run(stage);
+    run(stage);
+    for (int r = 0; r < reps; r++) {
+      MF_CHECK(hipEventRecord(e0[r], st));
+      run(stage);
+      MF_CHECK(hipEventRecord(e1[r], st));
+    }
+    MF_CHECK(hipStreamSynchronize(st));
+    double tot = 0;
+    for (int r = 0; r < reps; r++) {
+      float t = 0;
+      MF_CHECK(hipEventElapsedTime(&t, e0[r], e1[r]));
+      tot += t;
+    }
+    ms[stage] = tot / reps;
+  }
+  for (int r = 0; r < reps; r++) {
+    (void)hipEventDestroy(e0[r]);
+    (void)hipEventDestroy(e1[r]);
+  }
+  MF_CHECK(hipGetLastError());
+  return 0;
+}
+
+long mfista_last_steps(void) {
+  Context* C = get_context();
+  return C ? C->last_steps : -1;
 }
 
 }  // extern "C"

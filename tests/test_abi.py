@@ -1,0 +1,61 @@
+"""The C ABI (CPU only, no compute): the library loads, exports every function
+include/mfista.h declares, and struct RESULT has the reference layout
+(c/mfista.h:45-66: 8 ints, 10 doubles, a pointer and a double; 128 bytes)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import sparseimaging_amd as sa
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mfista.h")
+
+
+def _declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char|long)\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = sa.lib()
+    names = _declared_functions()
+    assert "mfista_imaging_core_nufft" in names and "y2x_nufft" in names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(sa.EXPORTED) == names
+
+
+def test_reference_symbols_present():
+    for n in ("mfista_imaging_core_nufft", "mfista_imaging_core_fft", "mfista_imaging_core_dft",
+              "x2y_nufft", "y2x_nufft"):
+        assert hasattr(sa.lib(), n)
+
+
+def test_result_layout_matches_reference(tmp_path):
+    prog = tmp_path / "layout.c"
+    prog.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "mfista.h"\n'
+        "int main(void){struct RESULT r; (void)r;\n"
+        'printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(struct RESULT), offsetof(struct RESULT, nonneg),'
+        " offsetof(struct RESULT, lambda_l1), offsetof(struct RESULT, comp_time),"
+        " offsetof(struct RESULT, residual), offsetof(struct RESULT, Lip_const)); return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(v) for v in out] == [128, 28, 32, 104, 112, 120]
+    assert ctypes.sizeof(sa.RESULT) == 128
+    assert sa.RESULT.residual.offset == 112 and sa.RESULT.Lip_const.offset == 120
+
+
+def test_product_does_not_import_the_oracle():
+    pkg = os.path.join(ROOT, "sparseimaging_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "liboracle" not in text, f

@@ -1,0 +1,109 @@
+"""GPU FFT mode, DFT mode, L1+TV (FGP) and the reference's own data files,
+each against the FP64 oracle on the same inputs.
+
+Bars: FP64 modes (FFT, DFT) image relative L2 <= 1e-6 and objective <= 1e-9
+relative (same arithmetic, only summation order differs); FP32 NUFFT modes
+image relative L2 <= 1e-3 and objective <= 1e-6 relative."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import sparseimaging_amd as sa
+from sparseimaging_amd import synth
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _cmp(tag, xg, rg, o, img_tol, cost_tol):
+    xo, ro = o["x"], o["result"]
+    l2 = np.linalg.norm(xg - xo) / max(np.linalg.norm(xo), 1e-300)
+    dc = abs(rg["finalcost"] - ro["finalcost"]) / abs(ro["finalcost"])
+    print(f"  {tag}: ITER gpu={rg['ITER']} oracle={ro['ITER']} image relL2={l2:.3e} cost rel={dc:.3e}", flush=True)
+    assert l2 <= img_tol, (tag, l2)
+    assert dc <= cost_tol, (tag, dc)
+
+
+def test_fft_mode_cfg1_64():
+    P = synth.make_fft_problem(N=64, ncell=1500, seed=1001)
+    kw = dict(maxiter=1000, eps=1e-5, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=10.0, cinit=1e4)
+    print("case fft-mode 64x64 M=3000 L1+TSV", flush=True)
+    xg, rg, _ = sa.mfista_imaging_core_fft(P["u_idx"], P["v_idx"], P["y_r"], P["y_i"], P["noise"], 64, 64,
+                                           nonneg_flag=1, **kw)
+    o = oracle.imaging_core_fft(P["u_idx"], P["v_idx"], P["y_r"], P["y_i"], P["noise"], 64, 64, kw["maxiter"],
+                                kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"], nonneg=1)
+    _cmp("fft cfg1", xg, rg, o, 1e-6, 1e-9)
+    assert rg["M"] == P["M"] // 2
+
+
+def test_fft_mode_reference_data_tv():
+    d = np.load(os.path.join(G, "ref_fft_data.npz"))
+    N = int(d["NX"])
+    print(f"case fft-mode reference fft_data.txt {N}x{N} L1+TV", flush=True)
+    kw = dict(maxiter=60, eps=1e-5, lambda_l1=10.0, lambda_tv=50.0, lambda_tsv=0.0, cinit=1e4)
+    xg, rg, _ = sa.mfista_imaging_core_fft(d["u"], d["v"], d["y_r"], d["y_i"], d["noise"], N, N, nonneg_flag=1,
+                                           fgp_iter=30, **kw)
+    o = oracle.imaging_core_fft(d["u"], d["v"], d["y_r"], d["y_i"], d["noise"], N, N, kw["maxiter"], kw["eps"],
+                                kw["lambda_l1"], kw["lambda_tv"], 0.0, kw["cinit"], nonneg=1, fgp_iter=30)
+    _cmp("fft ref-data TV", xg, rg, o, 1e-6, 1e-9)
+
+
+@pytest.mark.parametrize("tv", [False, True])
+def test_dft_mode(tv):
+    rng = np.random.default_rng(5)
+    NX = NY = 16
+    N, M = NX * NY, 400
+    A = rng.standard_normal((M, N)) / np.sqrt(M)
+    y = A @ np.maximum(rng.standard_normal(N), 0) + 0.05 * rng.standard_normal(M)
+    kw = dict(maxiter=400, eps=1e-9, lambda_l1=0.05, lambda_tv=0.1 if tv else 0.0,
+              lambda_tsv=0.0 if tv else 0.2, cinit=1.0)
+    print(f"case dft-mode M={M} N={N} tv={tv}", flush=True)
+    xg, rg, _, resid = sa.mfista_imaging_core_dft(y, A, NX, NY, nonneg_flag=1, fgp_iter=40, **kw)
+    o = oracle.imaging_core_dft(y, A, NX, NY, kw["maxiter"], kw["eps"], kw["lambda_l1"], kw["lambda_tv"],
+                                kw["lambda_tsv"], kw["cinit"], nonneg=1, fgp_iter=40)
+    _cmp(f"dft tv={tv}", xg, rg, o, 1e-6, 1e-9)
+    assert np.abs(resid - o["residual"]).max() <= 1e-6 * np.abs(o["residual"]).max()
+    assert rg["M"] == M and rg["N"] == N
+
+
+def test_nufft_tv_c24():
+    P = synth.make_nufft_problem(N=64, M=2500, seed=1003, amax=0.35, kind="tv", coverage="disc")
+    kw = dict(maxiter=3000, eps=0.0, lambda_l1=0.5, lambda_tv=5.0, lambda_tsv=0.0, cinit=1e4, rel_tol=1e-8)
+    print("case nufft L1+TV C-24 64x64 M=2500 fgp=50", flush=True)
+    xg, rg, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                             nonneg_flag=1, preset=sa.PRESET_C24, fgp_iter=50, **kw)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], kw["lambda_tv"], 0.0, kw["cinit"], nonneg=1,
+                                  preset=oracle.PRESET_C24, fgp_iter=50, rel_tol=kw["rel_tol"])
+    _cmp("nufft TV", xg, rg, o, 1e-3, 1e-6)
+
+
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_nufft_reference_data(preset):
+    """c/nufft_data.txt (200x200, M=7648) with the author's run parameters
+    (matlab/testdata4nufft.mat: lambda_l1=100, lambda_tsv=100, cinit=5e8, nonneg)."""
+    d = np.load(os.path.join(G, "ref_nufft_data.npz"))
+    N = int(d["NX"])
+    kw = dict(maxiter=2000, eps=0.0, lambda_l1=100.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=5e8, rel_tol=1e-8)
+    print(f"case reference nufft_data.txt {N}x{N} M={d['u'].size} preset={preset}", flush=True)
+    xg, rg, _ = sa.mfista_imaging_core_nufft(d["u"], d["v"], d["vis_r"], d["vis_i"], d["std"], N, N,
+                                             nonneg_flag=1, preset=preset, **kw)
+    o = oracle.imaging_core_nufft(d["u"], d["v"], d["vis_r"], d["vis_i"], d["std"], N, N, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"], nonneg=1,
+                                  preset=preset, rel_tol=kw["rel_tol"])
+    _cmp("nufft ref-data", xg, rg, o, 1e-3, 1e-6)
+
+
+def test_clean_box_and_nonneg_flag():
+    P = synth.make_nufft_problem(N=64, M=1500, seed=8, amax=0.35)
+    box = np.ones((64, 64), np.float32)
+    box[:, :20] = 0
+    kw = dict(maxiter=400, eps=1e-5, lambda_l1=1.0, lambda_tsv=10.0, cinit=1e4)
+    xg, rg, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                             nonneg_flag=0, cl_box=box, preset=sa.PRESET_CXX12, **kw)
+    assert np.all(xg[:, :20] == 0)
+    x2, r2, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                             nonneg_flag=2, preset=sa.PRESET_CXX12, **kw)
+    assert r2["ITER"] == 0  # c/mfista_nufft_lib.c:634-637
