@@ -56,7 +56,7 @@ void launch_control(DevState* S, double* cost, const double* part, PartialLayout
 void launch_init_control(DevState* S, const double* part, PartialLayout L, const double* stats,
                          int nstat_blocks, hipStream_t st);
 template <typename Rs, typename Rd>
-void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st);
+void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st, const int* skip = nullptr);
 int elem_blocks(int n);
 
 }  // namespace mf

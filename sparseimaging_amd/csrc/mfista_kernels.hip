@@ -150,7 +150,8 @@ k_momentum(R* __restrict__ x, const R* __restrict__ xnew, R* __restrict__ z, int
 }
 
 template <typename Rs, typename Rd>
-__global__ void k_cast(const Rs* __restrict__ a, Rd* __restrict__ b, int n) {
+__global__ void k_cast(const Rs* __restrict__ a, Rd* __restrict__ b, int n, const int* skip) {
+  if (skip && *skip) return;
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f < n) b[f] = (Rd)a[f];
 }
@@ -333,8 +334,8 @@ void launch_init_control(DevState* S, const double* part, PartialLayout L, const
   hipLaunchKernelGGL(k_init_control, dim3(1), dim3(MF_BLOCK), 0, st, S, part, L, stats, nsb);
 }
 template <typename Rs, typename Rd>
-void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st) {
-  hipLaunchKernelGGL((k_cast<Rs, Rd>), dim3(elem_blocks(n)), dim3(MF_BLOCK), 0, st, a, b, n);
+void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st, const int* skip) {
+  hipLaunchKernelGGL((k_cast<Rs, Rd>), dim3(elem_blocks(n)), dim3(MF_BLOCK), 0, st, a, b, n, skip);
 }
 
 #define MF_INST2(R)                                                                          \
@@ -346,9 +347,9 @@ void launch_cast(const Rs* a, Rd* b, int n, hipStream_t st) {
   template void launch_momentum<R>(R*, const R*, R*, int, const DevState*, hipStream_t);
 MF_INST2(float)
 MF_INST2(double)
-template void launch_cast<float, double>(const float*, double*, int, hipStream_t);
-template void launch_cast<double, float>(const double*, float*, int, hipStream_t);
-template void launch_cast<double, double>(const double*, double*, int, hipStream_t);
-template void launch_cast<float, float>(const float*, float*, int, hipStream_t);
+template void launch_cast<float, double>(const float*, double*, int, hipStream_t, const int*);
+template void launch_cast<double, float>(const double*, float*, int, hipStream_t, const int*);
+template void launch_cast<double, double>(const double*, double*, int, hipStream_t, const int*);
+template void launch_cast<float, float>(const float*, float*, int, hipStream_t, const int*);
 
 }  // namespace mf

@@ -57,8 +57,9 @@ void launch_rowfft_c2r(const cplx<R>* U, const R* e4x, const R* e4y, R scale, R*
                        hipStream_t st, const int* skip);
 template <typename R>
 void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rz,
-                   double* chi_part, double* lip_part, int mode, int nblk, cplx<R>* model,
-                   const int* perm, const int* yneg_sorted, hipStream_t st, const int* skip);
+                   float2* rz32, const double2* rz64, double* chi_part, double* lip_part, int mode,
+                   int nblk, cplx<R>* model, const int* perm, const int* yneg_sorted,
+                   hipStream_t st, const int* skip);
 template <typename R>
 void launch_grid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* rz, const int4* chunks,
                  int nchunks, const unsigned* entries, cplx<R>* Hg, hipStream_t st,

@@ -197,18 +197,20 @@ k_rowfft_c2r(const cplx<R>* __restrict__ U, const R* __restrict__ e4x, const R* 
 
 // --------------------------------------------------------------------------
 // degridding: G lanes per visibility; lane l owns tap column kx = mx + l.
-// mode 0: r = (V' - A x) w, stored in rz, partial sum_k |r_k|^2
-// mode 1: x is a step d; t = (A d) w, r_new = rz - t; partials sum |r_new|^2 and
-//         sum |t|^2 (the Lipschitz-test term), rz left unchanged
+// mode 0: r = (V' - A x) w in R precision; stored in rz (R) and/or rz32
+//         (FP32 copy for the gridder); partial sum_k |r_k|^2
+// mode 1: x is a step d; t = (A d) w, r_new = rz64 - t with rz64 the FP64
+//         residual at the point the step starts from; partials sum |r_new|^2
+//         and sum |t|^2 (the Lipschitz-test term)
 template <typename R, int G>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rz,
-         double* __restrict__ chi_part, double* __restrict__ lip_part, int mode,
-         cplx<R>* __restrict__ model, const int* __restrict__ perm,
-         const int* __restrict__ yneg, const int* skip) {
+         float2* __restrict__ rz32, const double2* __restrict__ rz64, double* __restrict__ chi_part,
+         double* __restrict__ lip_part, int mode, cplx<R>* __restrict__ model,
+         const int* __restrict__ perm, const int* __restrict__ yneg, const int* skip) {
   if (skip && *skip) return;
   using C = cplx<R>;
-  __shared__ double red[16];
+  __shared__ double red[32];
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = gid / G, lane = threadIdx.x & (G - 1);
   const int S = 2 * g.MSP;
@@ -245,12 +247,13 @@ k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* _
       const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
       v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
       if (rz) rz[s] = r;
+      if (rz32) rz32[s] = make_float2((float)r.x, (float)r.y);
     } else {
-      const C t = mk<R>(F.x * w, F.y * w);
-      const C r0 = rz[s];
-      const C rn = mk<R>(r0.x - t.x, r0.y - t.y);
-      v[0] = (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
-      v[1] = (double)t.x * (double)t.x + (double)t.y * (double)t.y;
+      const double tx = (double)F.x * (double)w, ty = (double)F.y * (double)w;
+      const double2 r0 = rz64[s];
+      const double rx = r0.x - tx, ry = r0.y - ty;
+      v[0] = rx * rx + ry * ry;
+      v[1] = tx * tx + ty * ty;
     }
     if (model) model[perm[s]] = yneg[s] ? cconj(F) : F;
   }
@@ -541,14 +544,15 @@ int degrid_blocks(int M, int MSP) {
 
 template <typename R>
 void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf, cplx<R>* rz,
-                   double* chi_part, double* lip_part, int mode, int nblk, cplx<R>* model,
-                   const int* perm, const int* yneg_sorted, hipStream_t st, const int* skip) {
+                   float2* rz32, const double2* rz64, double* chi_part, double* lip_part, int mode,
+                   int nblk, cplx<R>* model, const int* perm, const int* yneg_sorted,
+                   hipStream_t st, const int* skip) {
   if (degrid_lanes(g.MSP) == 16)
-    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, chi_part,
-                       lip_part, mode, model, perm, yneg_sorted, skip);
+    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
+                       chi_part, lip_part, mode, model, perm, yneg_sorted, skip);
   else
-    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, chi_part,
-                       lip_part, mode, model, perm, yneg_sorted, skip);
+    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
+                       chi_part, lip_part, mode, model, perm, yneg_sorted, skip);
 }
 
 template <typename R>
@@ -613,8 +617,8 @@ int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, cons
                                      int, const FFTPlan&, const cplx<R>*, hipStream_t,           \
                                      const int*);                                                \
   template void launch_degrid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, cplx<R>*, \
-                                 double*, double*, int, int, cplx<R>*, const int*, const int*,   \
-                                 hipStream_t, const int*);                                       \
+                                 float2*, const double2*, double*, double*, int, int, cplx<R>*,  \
+                                 const int*, const int*, hipStream_t, const int*);               \
   template void launch_grid<R>(const GridGeom&, const VisParams<R>&, const cplx<R>*, const int4*, \
                                int, const unsigned*, cplx<R>*, hipStream_t, const int*);
 

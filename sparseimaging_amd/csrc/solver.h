@@ -66,8 +66,8 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
                 const double* d_vi, const double* d_sd, int M, int NX, int NY);
 int nufft_ensure64(Context& C);
 template <typename R>
-void nufft_fwd(Context& C, const R* x, cplx<R>* rw, double* chi_part, cplx<R>* model,
-               const int* skip);
+void nufft_fwd(Context& C, const R* x, cplx<R>* rz, double* chi_part, cplx<R>* model,
+               const int* skip, float2* rz32 = nullptr);
 template <typename R>
 void nufft_fwd_delta(Context& C, const R* d, double* chi_part, double* lip_part, const int* skip);
 template <typename R>

@@ -261,24 +261,25 @@ struct Sel<double> {
   static double2* U(NufftState& N) { return (double2*)N.U64.p; }
 };
 
-// A x: residual rz = w (V' - A x) (optional), chi^2 partials (optional),
-// model visibilities in input order (optional)
+// A x: residual rz = w (V' - A x) in R precision and/or its FP32 copy rz32
+// (all optional), chi^2 partials (optional), model visibilities in input order
+// (optional)
 template <typename R>
 void nufft_fwd(Context& C, const R* x, cplx<R>* rz, double* chi_part, cplx<R>* model,
-               const int* skip) {
+               const int* skip, float2* rz32) {
   NufftState& N = C.nu;
   const GridGeom& g = N.geom;
   launch_rowfft_fwd<R>(x, Sel<R>::ex(N), Sel<R>::ey(N), Sel<R>::T(N), N.NX, N.NY, g.Ly, N.NY / 2,
                        N.Py, Sel<R>::ty(N), C.stream, skip);
   launch_colfft_fwd<R>(Sel<R>::T(N), Sel<R>::Gf(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.MSP,
                        N.Px, Sel<R>::tx(N), C.stream, skip);
-  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), rz, chi_part, nullptr, 0,
+  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), rz, rz32, nullptr, chi_part, nullptr, 0,
                    degrid_blocks(N.M, N.MSP), model, (const int*)N.perm.p, (const int*)N.yneg.p,
                    C.stream, skip);
 }
 
-// A d for a step d = xnew - z: partials of |rz - w A d|^2 (chi^2 at xnew) and
-// of |w A d|^2, the term of the backtracking test
+// A d for a step d = xnew - z (FP32): partials of |rz64 - w A d|^2 (chi^2 at
+// xnew, with rz64 the FP64 residual at z) and of |w A d|^2
 template <typename R>
 void nufft_fwd_delta(Context& C, const R* d, double* chi_part, double* lip_part, const int* skip) {
   NufftState& N = C.nu;
@@ -287,7 +288,7 @@ void nufft_fwd_delta(Context& C, const R* d, double* chi_part, double* lip_part,
                        N.Py, Sel<R>::ty(N), C.stream, skip);
   launch_colfft_fwd<R>(Sel<R>::T(N), Sel<R>::Gf(N), N.NX, N.NY + 1, g.Lx, N.NX / 2, 0, N.MSP,
                        N.Px, Sel<R>::tx(N), C.stream, skip);
-  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), (cplx<R>*)(std::is_same<R, float>::value ? N.rw.p : N.rw64.p),
+  launch_degrid<R>(g, Sel<R>::vp(N), Sel<R>::Gf(N), nullptr, nullptr, (const double2*)N.rw64.p,
                    chi_part, lip_part, 1, degrid_blocks(N.M, N.MSP), nullptr, nullptr, nullptr,
                    C.stream, skip);
 }
@@ -306,8 +307,8 @@ void nufft_adj(Context& C, const cplx<R>* rz, R* gout, const int* skip) {
                        N.Py, Sel<R>::ty(N), C.stream, skip);
 }
 
-template void nufft_fwd<float>(Context&, const float*, float2*, double*, float2*, const int*);
-template void nufft_fwd<double>(Context&, const double*, double2*, double*, double2*, const int*);
+template void nufft_fwd<float>(Context&, const float*, float2*, double*, float2*, const int*, float2*);
+template void nufft_fwd<double>(Context&, const double*, double2*, double*, double2*, const int*, float2*);
 template void nufft_adj<float>(Context&, const float2*, float*, const int*);
 template void nufft_fwd_delta<float>(Context&, const float*, double*, double*, const int*);
 template void nufft_adj<double>(Context&, const double2*, double*, const int*);
@@ -389,12 +390,18 @@ int nufft_chi2_64(Context& C, const double* d_x64, double* chi2) {
 }
 
 // ---------------------------------------------------------------------------
+// The momentum-point residual (the objective of the backtracking test and the
+// input of the gradient) is evaluated with an FP64 forward NUFFT; the step
+// transform, the gradient and the prox stay FP32.
 struct NufftOp : Operator<float> {
   Context& C;
   explicit NufftOp(Context& c) : C(c) {}
   int npart() const override { return degrid_blocks(C.nu.M, C.nu.MSP); }
   void fwd(const float* x, bool keep, double* part, const int* skip) override {
-    nufft_fwd<float>(C, x, keep ? (float2*)C.nu.rw.p : nullptr, part, nullptr, skip);
+    double* x64 = (double*)C.x64.p;
+    launch_cast<float, double>(x, x64, C.nu.NX * C.nu.NY, C.stream, skip);
+    nufft_fwd<double>(C, x64, keep ? (double2*)C.nu.rw64.p : nullptr, part, nullptr, skip,
+                      keep ? (float2*)C.nu.rw.p : nullptr);
   }
   void adj(float* g, const int* skip) override {
     nufft_adj<float>(C, (const float2*)C.nu.rw.p, g, skip);
@@ -445,6 +452,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     printf("nonneg_flag must be chosen properly.\n");
     rc = 0;
   } else {
+    if (nufft_ensure64(C) || C.x64.ensure((size_t)A.NX * A.NY * 8)) return MFISTA_E_HIP;
     NufftOp op(C);
     RunParams P;
     P.NX = A.NX;
@@ -761,8 +769,9 @@ int mfista_profile_stages(int reps, double* ms, int nstage) {
                                  N.MSP, N.Px, (const float2*)N.twxf.p, st, nullptr);
         break;
       case 2:
-        launch_degrid<float>(g, N.pf(), (const float2*)N.Gf.p, (float2*)N.rw.p, part, nullptr, 0,
-                             degrid_blocks(N.M, N.MSP), nullptr, nullptr, nullptr, st, nullptr);
+        launch_degrid<float>(g, N.pf(), (const float2*)N.Gf.p, nullptr, (float2*)N.rw.p, nullptr, part,
+                             nullptr, 0, degrid_blocks(N.M, N.MSP), nullptr, nullptr, nullptr, st,
+                             nullptr);
         break;
       case 3:
         launch_grid<float>(g, N.pf(), (const float2*)N.rw.p, (const int4*)N.chunks.p, N.nchunks,
