@@ -36,7 +36,8 @@ def log(*a):
 
 
 def stage_bytes(N, M, MSP, n_entries):
-    """Algorithmic (compulsory) HBM bytes per launch of each kernel, FP32 data.
+    """Algorithmic (compulsory) HBM bytes per launch of each stream-engine
+    kernel, FP32 data.
 
     H = 2N(N+1) complex cells of the half-plane grid, Hf = 2N(N+MSP) cells of
     the forward spectrum with its mirrored rows, P = N^2 pixels.  Per
@@ -60,6 +61,31 @@ def stage_bytes(N, M, MSP, n_entries):
     }
 
 
+def persistent_bytes(N, M, MSP, n_entries, iters, trials):
+    """Algorithmic bytes of one persistent-engine solve (one launch).
+
+    Per iteration (prologue at the momentum point z, FP64 forward):
+      momentum + FP64 row pass   read x, xnew, write z (12 B/px), write T64 16 B/cell
+      FP64 column pass           read T64 16 B/cell, write Gf64 16 B/cell
+      FP64 degrid                read Gf64 16 B/cell, 56 B/vis params, write rz64 16 + rz32 8 B/vis
+      FP32 grid                  read rz32 8 + 28 B/vis params + 4 B/entry, write Hg 8 B/cell
+      FP32 inverse column pass   read Hg, write U, clear Hg (8 B/cell each)
+      c2r + TSV gradient         read U 8 B/cell, z 4 B/px, write g 4 B/px
+    Per trial:
+      prox + FP32 row pass       read g, z, write xnew (12 B/px), write T 8 B/cell
+      FP32 column pass           read T, write Gf (8 B/cell each)
+      FP32 degrid (mode 1)       read Gf 8 B/cell, 32 B/vis params, rz64 16 B/vis
+    """
+    P = N * N
+    T = N * (N + 1)
+    H = 2 * N * (N + 1)
+    Hf = 2 * N * (N + MSP)
+    per_iter = ((12 * P + 16 * T) + (16 * T + 16 * Hf) + (16 * Hf + 56 * M + 24 * M)
+                + (36 * M + 4 * n_entries + 8 * H) + (8 * H + 8 * T + 8 * H) + (8 * T + 8 * P))
+    per_trial = (12 * P + 8 * T) + (8 * T + 8 * Hf) + (8 * Hf + 48 * M)
+    return per_iter * iters + per_trial * trials, per_iter, per_trial
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +94,9 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=150,
                     help="oracle iterations timed for the CPU baseline (0 = skip)")
     ap.add_argument("--profile-reps", type=int, default=50)
+    ap.add_argument("--engine", choices=["persistent", "stream"], default="persistent")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -83,13 +112,22 @@ def main():
 
     N, M, seed = 256, 20000, 1002
     cfg = dict(maxiter=50000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=1e4,
-               nonneg_flag=1, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+               nonneg_flag=1, preset=sa.PRESET_CXX12, rel_tol=1e-6,
+               engine=sa.ENGINE_PERSISTENT if args.engine == "persistent" else sa.ENGINE_STREAM)
     P = synth.make_nufft_problem(N=N, M=M, seed=seed)
     dev = torch.device("cuda:0")
     t = {k: torch.tensor(P[k], dtype=torch.float64, device=dev) for k in ("u", "v", "vis_r", "vis_i", "std")}
     x0 = torch.zeros(N * N, dtype=torch.float64, device=dev)
     xo = torch.zeros(N * N, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    if args.copy_calib_mib > 0:
+        nel = args.copy_calib_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.copy_calib_mib} MiB done")
+        del src, dst
 
     def solve():
         return sa.solve_nufft_device(t["u"], t["v"], t["vis_r"], t["vis_i"], t["std"], N, N, x0, xo, **cfg)
@@ -109,22 +147,33 @@ def main():
     log(f"bench: ITER={ITER} trials={trials} finalcost={res['finalcost']:.10e} "
         f"ms/solve={ms_per_step:.2f} iters/s={ITER / (ms_per_step / 1e3):.1f}")
 
-    # per-kernel durations and the dominant kernel
+    # per-kernel durations of the stream engine's kernels (same problem, HIP events)
     prof = sa.profile_stages(args.profile_reps)
     MSP = 6
     n_entries = int(1.6 * M)  # gridding tile-list entries (about 1.6 per visibility here)
     nbytes = stage_bytes(N, M, MSP, n_entries)
     launches = {"rowfft_fwd": ITER + trials, "colfft_fwd": ITER + trials, "degrid": ITER + trials,
                 "grid": ITER, "colfft_inv": ITER, "rowfft_c2r": ITER, "tsv_grad": ITER, "prox": trials}
-    share = {k: prof[k] * launches[k] for k in prof}
-    dom = max(share, key=share.get)
-    ach = nbytes[dom] / (prof[dom] * 1e-3) / 1e9
     kern = {k: {"ms": round(prof[k], 5), "launches": launches[k], "GBps": round(nbytes[k] / (prof[k] * 1e-3) / 1e9, 1)}
             for k in prof}
-    log("bench: kernels " + json.dumps(kern))
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": nbytes[dom], "launch_ms": round(prof[dom], 5)}
+    log("bench: stream-engine kernels " + json.dumps(kern))
+    if args.engine == "persistent":
+        # the dominant (only) kernel is the persistent solver: one launch per solve
+        kms = info["kernel_ms"]
+        tot, per_it, per_tr = persistent_bytes(N, M, MSP, n_entries, ITER, trials)
+        ach = tot / (kms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": "k_mfista_persistent", "achieved": round(ach, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": None, "bytes_per_launch": tot, "bytes_per_iteration": per_it,
+                    "bytes_per_trial": per_tr, "launch_ms": round(kms, 3)}
+    else:
+        share = {k: prof[k] * launches[k] for k in prof}
+        dom = max(share, key=share.get)
+        ach = nbytes[dom] / (prof[dom] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_launch": nbytes[dom], "launch_ms": round(prof[dom], 5)}
+    log("bench: roofline " + json.dumps(roofline))
 
     cpu = None
     if args.cpu_iters > 0:
@@ -159,7 +208,7 @@ def main():
         "data": "synthetic (seeded ring + 3 Gaussians, 12 elliptical uv arcs)",
         "config": {"workload": "nufft_l1tsv_256_m20000_12tap", "N": N, "M": M, "taps": 12,
                    "lambda_l1": 1.0, "lambda_tsv": 100.0, "rel_tol": 1e-6, "start": "cold",
-                   "iterations": ITER, "trials": trials},
+                   "iterations": ITER, "trials": trials, "engine": args.engine},
         "iterations_per_s": round(ITER / (ms_per_step / 1e3), 2),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -96,6 +96,9 @@ void y2x_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *yin_r,
 #define MFISTA_PRESET_C24 0   /* c/mfista_nufft_lib.c: MSP=12, round, tau=12/N^2 */
 #define MFISTA_PRESET_CXX12 1 /* c++/mfista_nufft_lib.cpp: MSP=6, floor, tau=6/N^2, eta0=10 */
 
+#define MFISTA_ENGINE_PERSISTENT 0
+#define MFISTA_ENGINE_STREAM 1
+
 #define MFISTA_OK 0
 #define MFISTA_E_BOTH_REG -1   /* lambda_tv and lambda_tsv both positive */
 #define MFISTA_E_TV_PRESET -2  /* TV with the C++ preset (not in that library) */
@@ -114,6 +117,9 @@ typedef struct mfista_opts {
   int max_trials;   /* out: largest number of backtracking trials in one iteration */
   double solve_ms;  /* out: GPU wall time of the solve (excluding the FP64 RESULT pass) */
   int total_trials; /* out: backtracking trials over the whole solve (forward NUFFTs of xnew) */
+  int engine;       /* NUFFT solves: MFISTA_ENGINE_PERSISTENT (default, one cooperative
+                       launch per solve) or MFISTA_ENGINE_STREAM (one launch per kernel) */
+  double kernel_ms; /* out: persistent engine, duration of the solver launch (HIP events) */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);

@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmfista_amd.so")
 
 PRESET_C24 = 0
 PRESET_CXX12 = 1
+ENGINE_PERSISTENT = 0  # one cooperative launch per NUFFT solve
+ENGINE_STREAM = 1      # one launch per kernel, device-side gating
 MAXITER, MINITER, FGPITER, TD, ETA, EPS = 50000, 100, 100, 50, 1.1, 1.0e-5
 
 STATUS = {
@@ -61,6 +63,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("preset", C.c_int), ("fgp_iter", C.c_int), ("rel_tol", C.c_double), ("verbose", C.c_int),
         ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
+        ("engine", C.c_int), ("kernel_ms", C.c_double),
     ]
 
 
@@ -147,22 +150,23 @@ def _box(cl_box, n):
     return 1, b, b.ctypes.data_as(_fp)
 
 
-def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0):
+def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT):
     o = Opts()
     lib().mfista_default_opts(C.byref(o))
-    o.preset, o.fgp_iter, o.rel_tol, o.verbose = preset, fgp_iter, rel_tol, verbose
+    o.preset, o.fgp_iter, o.rel_tol, o.verbose, o.engine = preset, fgp_iter, rel_tol, verbose, engine
     return o
 
 
 def _info(o):
-    return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials}
+    return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials,
+            "kernel_ms": o.kernel_ms}
 
 
 # ---------------------------------------------------------------- solvers
 def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
                               nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
-                              rel_tol=0.0):
+                              rel_tol=0.0, engine=ENGINE_PERSISTENT):
     """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts)."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
@@ -175,7 +179,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     xout = np.zeros(NX * NY)
     bflag, b, pb = _box(cl_box, NX * NY)
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine)
     rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
                                             lambda_tv, lambda_tsv, cinit, px0,
                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
@@ -236,7 +240,8 @@ def mfista_imaging_core_dft(y, A, NX, NY, maxiter=MAXITER, eps=EPS, lambda_l1=1.
 
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
                        lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
-                       t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None):
+                       t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None,
+                       engine=ENGINE_PERSISTENT):
     """HBM-resident solve: every array argument is a device pointer (int) or a
     torch CUDA tensor (float64; cl_box float32).  Returns (RESULT dict, opts dict)."""
 
@@ -249,7 +254,7 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
     if M is None:
         raise ValueError("pass torch tensors (M is taken from t_u)")
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine)
     rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
                                          maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
                                          ptr(t_xinit), ptr(t_xout), nonneg_flag,

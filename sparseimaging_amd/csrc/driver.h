@@ -33,11 +33,15 @@ struct RunParams {
 struct RunStats {
   int ITER = 0, max_trials = 0, total_trials = 0, status = 0;
   double c = 0;
+  double kernel_ms = 0;  // persistent engine: duration of the solver launch
   long steps = 0;
 };
 
 template <typename R>
 int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
+
+// the whole NUFFT solve in one cooperative launch (persistent.hip)
+int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
 
 // FGP TV prox kernels (reference c/mfista_TV_lib.c:189-384)
 template <typename R>

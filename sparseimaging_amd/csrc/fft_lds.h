@@ -29,14 +29,14 @@ inline bool make_fft_plan(int L, FFTPlan* P) {
   return m == 1 && P->nstage <= 24;
 }
 
-template <typename R>
-__device__ inline void bfly(int rad, cplx<R>* v, int sign) {
+template <typename R, int RAD>
+__device__ inline void bfly(cplx<R> (&v)[RAD], int sign) {
   using C = cplx<R>;
-  if (rad == 2) {
+  if constexpr (RAD == 2) {
     C a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
-  } else if (rad == 4) {
+  } else if constexpr (RAD == 4) {
     C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
     C s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
     // w = exp(sign*2*pi*i/4) = sign*i ; w*d = sign*(-d.y, d.x)
@@ -45,7 +45,7 @@ __device__ inline void bfly(int rad, cplx<R>* v, int sign) {
     v[2] = csub(s02, s13);
     v[1] = cadd(d02, wd);
     v[3] = csub(d02, wd);
-  } else if (rad == 3) {
+  } else if constexpr (RAD == 3) {
     const R h = (R)0.86602540378443864676;  // sqrt(3)/2
     C t1 = cadd(v[1], v[2]);
     C t2 = mk<R>(v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y);
@@ -54,7 +54,7 @@ __device__ inline void bfly(int rad, cplx<R>* v, int sign) {
     v[0] = cadd(v[0], t1);
     v[1] = cadd(t2, t3);
     v[2] = csub(t2, t3);
-  } else {  // rad == 5
+  } else {  // RAD == 5
     const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
     const R s1 = (R)0.95105651629515357212, s2 = (R)0.58778525229247312917;
     C a = v[0];
@@ -76,6 +76,38 @@ __device__ inline void bfly(int rad, cplx<R>* v, int sign) {
   }
 }
 
+// one Stockham stage of radix RAD (compile time, so the butterfly stays in
+// registers): src -> dst for nseq sequences of length L, Ns = product of the
+// radices of the earlier stages
+template <typename R, int RAD>
+__device__ inline void fft_stage(const cplx<R>* src, cplx<R>* dst, int nseq, int L, int Ns,
+                                 const cplx<R>* __restrict__ tw, int sign) {
+  using C = cplx<R>;
+  const int nbf = L / RAD;
+  const int span = Ns * RAD;
+  const int tscale = L / span;
+  const int total = nbf * nseq;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int q = t / nbf;
+    const int j = t - q * nbf;
+    const C* in = src + q * L;
+    C* out = dst + q * L;
+    const int k = j % Ns;
+    C v[RAD];
+    v[0] = in[j];
+#pragma unroll
+    for (int r = 1; r < RAD; r++) {
+      C w = tw[r * k * tscale];
+      if (sign > 0) w.y = -w.y;
+      v[r] = cmul(in[j + r * nbf], w);
+    }
+    bfly<R, RAD>(v, sign);
+    const int base = (j / Ns) * span + k;
+#pragma unroll
+    for (int r = 0; r < RAD; r++) out[base + r * Ns] = v[r];
+  }
+}
+
 // Transforms nseq sequences held in buf0 (length L each, stride L).  buf1 is
 // scratch of the same size.  Returns the buffer holding the result.  Must be
 // called by every thread of the workgroup; ends with a barrier.
@@ -89,33 +121,17 @@ __device__ cplx<R>* lds_fft(cplx<R>* buf0, cplx<R>* buf1, int nseq, const FFTPla
   int Ns = 1;
   for (int s = 0; s < P.nstage; s++) {
     const int rad = P.radix[s];
-    const int nbf = L / rad;
-    const int span = Ns * rad;
-    const int tscale = L / span;
-    const int total = nbf * nseq;
-    for (int t = threadIdx.x; t < total; t += blockDim.x) {
-      const int q = t / nbf;
-      const int j = t - q * nbf;
-      const C* in = src + q * L;
-      C* out = dst + q * L;
-      const int k = j % Ns;
-      C v[5];
-      v[0] = in[j];
-      for (int r = 1; r < rad; r++) {
-        C a = in[j + r * nbf];
-        C w = tw[r * k * tscale];
-        if (sign > 0) w.y = -w.y;
-        v[r] = cmul(a, w);
-      }
-      bfly<R>(rad, v, sign);
-      const int base = (j / Ns) * span + k;
-      for (int r = 0; r < rad; r++) out[base + r * Ns] = v[r];
+    switch (rad) {
+      case 4: fft_stage<R, 4>(src, dst, nseq, L, Ns, tw, sign); break;
+      case 2: fft_stage<R, 2>(src, dst, nseq, L, Ns, tw, sign); break;
+      case 3: fft_stage<R, 3>(src, dst, nseq, L, Ns, tw, sign); break;
+      default: fft_stage<R, 5>(src, dst, nseq, L, Ns, tw, sign); break;
     }
     __syncthreads();
     C* tmp = src;
     src = dst;
     dst = tmp;
-    Ns = span;
+    Ns *= rad;
   }
   return src;
 }

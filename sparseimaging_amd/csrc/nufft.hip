@@ -19,189 +19,65 @@
 #include <hipcub/hipcub.hpp>
 
 #include "nufft.h"
+#include "stages.h"
 
 namespace mf {
 
 // --------------------------------------------------------------------------
-// forward row pass: real rows -> half spectra along y
+// stand-alone kernels: one stage task per workgroup (stages.h)
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_rowfft_fwd(const R* __restrict__ x, const R* __restrict__ e4x, const R* __restrict__ e4y,
              cplx<R>* __restrict__ T, int NX, int NY, int Ly, int shift, FFTPlan P,
              const cplx<R>* __restrict__ tw, int nseq, const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
-  C* b0 = reinterpret_cast<C*>(smem_d4);
-  C* b1 = b0 + nseq * Ly;
-  const int row0 = blockIdx.x * 2 * nseq;
-  for (int t = threadIdx.x; t < nseq * Ly; t += blockDim.x) {
-    const int q = t / Ly, b = t - q * Ly;
-    int j = b + shift;
-    if (j >= Ly) j -= Ly;
-    const int i0 = row0 + 2 * q, i1 = i0 + 1;
-    R re = 0, im = 0;
-    if (j < NY) {
-      const R ey = e4y ? e4y[j] : (R)1;
-      if (i0 < NX) re = x[(size_t)i0 * NY + j] * (e4x ? e4x[i0] : (R)1) * ey;
-      if (i1 < NX) im = x[(size_t)i1 * NY + j] * (e4x ? e4x[i1] : (R)1) * ey;
-    }
-    b0[t] = mk<R>(re, im);
-  }
-  __syncthreads();
-  C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
-  const int Lh = Ly / 2 + 1, nr = 2 * nseq;
-  for (int t = threadIdx.x; t < Lh * nr; t += blockDim.x) {
-    const int k = t / nr, r = t - k * nr;
-    const int i = row0 + r;
-    if (i >= NX) continue;
-    const int q = r >> 1;
-    const C z = res[q * Ly + k], zm = res[q * Ly + (k == 0 ? 0 : Ly - k)];
-    C o;
-    if (!(r & 1))
-      o = mk<R>((R)0.5 * (z.x + zm.x), (R)0.5 * (z.y - zm.y));
-    else
-      o = mk<R>((R)0.5 * (z.y + zm.y), (R)-0.5 * (z.x - zm.x));
-    T[(size_t)k * NX + i] = o;
-  }
+  auto load = [&](int i, int j) -> R {
+    return x[(size_t)i * NY + j] * (e4x ? e4x[i] : (R)1) * (e4y ? e4y[j] : (R)1);
+  };
+  st_rowfft_fwd<R>(blockIdx.x, nseq, NX, NY, Ly, shift, P, tw, T,
+                   reinterpret_cast<cplx<R>*>(smem_d4), load);
 }
 
-// forward column pass: c2c along x for each ky row of T
-// mode 0: NUFFT layout Gf[ky + MSP - 1][(kx + NX) mod Lx] with mirrored rows
-// mode 1: plain layout out[ky][kx]
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_colfft_fwd(const cplx<R>* __restrict__ T, cplx<R>* __restrict__ out, int NX, int nky, int Lx,
              int shift, int mode, int MSP, FFTPlan P, const cplx<R>* __restrict__ tw, int nseq,
              const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
-  C* b0 = reinterpret_cast<C*>(smem_d4);
-  C* b1 = b0 + nseq * Lx;
-  const int ky0 = blockIdx.x * nseq;
-  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
-    const int q = t / Lx, a = t - q * Lx;
-    int i = a + shift;
-    if (i >= Lx) i -= Lx;
-    const int ky = ky0 + q;
-    b0[t] = (ky < nky && i < NX) ? T[(size_t)ky * NX + i] : mk<R>(0, 0);
-  }
-  __syncthreads();
-  C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
-  const int NXh = Lx / 2;
-  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
-    const int q = t / Lx, k = t - q * Lx;
-    const int ky = ky0 + q;
-    if (ky >= nky) continue;
-    const C v = res[t];
-    if (mode == 0) {
-      int col = k + NXh;
-      if (col >= Lx) col -= Lx;
-      out[(size_t)(ky + MSP - 1) * Lx + col] = v;
-      if (ky >= 1 && ky <= MSP - 1) {
-        int colm = NXh - k;
-        if (colm < 0) colm += Lx;
-        out[(size_t)(MSP - 1 - ky) * Lx + colm] = cconj(v);
-      }
-    } else {
-      out[(size_t)ky * Lx + k] = v;
-    }
-  }
+  st_colfft_fwd<R>(blockIdx.x, nseq, T, out, NX, nky, Lx, shift, mode, MSP, P, tw,
+                   reinterpret_cast<cplx<R>*>(smem_d4));
 }
 
-// adjoint column pass: c2c backward along x for each ky, transposed output U[i][ky]
-// mode 0: input in NUFFT layout Hg[ky][(kx + NX) mod Lx]; mode 1: plain in[ky][kx]
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_colfft_inv(cplx<R>* __restrict__ in, cplx<R>* __restrict__ U, int NX, int nky, int Lx,
              int shift, int mode, FFTPlan P, const cplx<R>* __restrict__ tw, int nseq,
              const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
-  C* b0 = reinterpret_cast<C*>(smem_d4);
-  C* b1 = b0 + nseq * Lx;
-  const int ky0 = blockIdx.x * nseq;
-  const int NXh = Lx / 2;
-  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
-    const int q = t / Lx, k = t - q * Lx;
-    const int ky = ky0 + q;
-    int col = k;
-    if (mode == 0) {
-      col = k + NXh;
-      if (col >= Lx) col -= Lx;
-    }
-    C v = mk<R>(0, 0);
-    if (ky < nky) {
-      v = in[(size_t)ky * Lx + col];
-      if (mode == 0) in[(size_t)ky * Lx + col] = mk<R>(0, 0);  // the gridder accumulates into zeros
-    }
-    b0[t] = v;
-  }
-  __syncthreads();
-  C* res = lds_fft<R>(b0, b1, nseq, P, tw, +1);
-  for (int t = threadIdx.x; t < NX * nseq; t += blockDim.x) {
-    const int i = t / nseq, q = t - i * nseq;
-    const int ky = ky0 + q;
-    if (ky >= nky) continue;
-    int a = i - shift;
-    if (a < 0) a += Lx;
-    U[(size_t)i * nky + ky] = res[q * Lx + a];
-  }
+  st_colfft_inv<R>(blockIdx.x, nseq, in, U, NX, nky, Lx, shift, mode, P, tw,
+                   reinterpret_cast<cplx<R>*>(smem_d4));
 }
 
-// adjoint row pass: 1-D c2r along y for two image rows per complex FFT.  Only
-// the real parts of the DC and Nyquist entries are used (FFTW c2r semantics).
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_rowfft_c2r(const cplx<R>* __restrict__ U, const R* __restrict__ e4x, const R* __restrict__ e4y,
              R scale, R* __restrict__ g, int NX, int NY, int Ly, int shift, FFTPlan P,
              const cplx<R>* __restrict__ tw, int nseq, const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
-  C* b0 = reinterpret_cast<C*>(smem_d4);
-  C* b1 = b0 + nseq * Ly;
-  const int row0 = blockIdx.x * 2 * nseq;
-  const int Lh = Ly / 2, nky = Ly / 2 + 1;
-  for (int t = threadIdx.x; t < nseq * Ly; t += blockDim.x) {
-    const int q = t / Ly, b = t - q * Ly;
-    const int i0 = row0 + 2 * q, i1 = i0 + 1;
-    const int k = (b <= Lh) ? b : Ly - b;
-    C A = (i0 < NX) ? U[(size_t)i0 * nky + k] : mk<R>(0, 0);
-    C B = (i1 < NX) ? U[(size_t)i1 * nky + k] : mk<R>(0, 0);
-    if (k == 0 || k == Lh) {
-      A.y = 0;
-      B.y = 0;
-    }
-    b0[t] = (b <= Lh) ? mk<R>(A.x - B.y, A.y + B.x) : mk<R>(A.x + B.y, -A.y + B.x);
-  }
-  __syncthreads();
-  C* res = lds_fft<R>(b0, b1, nseq, P, tw, +1);
-  const int nr = 2 * nseq;
-  for (int t = threadIdx.x; t < nr * NY; t += blockDim.x) {
-    const int r = t / NY, j = t - r * NY;
-    const int i = row0 + r;
-    if (i >= NX) continue;
-    int b = j - shift;
-    if (b < 0) b += Ly;
-    const C z = res[(r >> 1) * Ly + b];
-    R val = (r & 1) ? z.y : z.x;
-    val *= scale;
-    if (e4x) val *= e4x[i];
-    if (e4y) val *= e4y[j];
-    g[(size_t)i * NY + j] = val;
-  }
+  auto store = [&](int i, int j, R v) {
+    v *= scale;
+    if (e4x) v *= e4x[i];
+    if (e4y) v *= e4y[j];
+    g[(size_t)i * NY + j] = v;
+  };
+  st_rowfft_c2r<R>(blockIdx.x, nseq, U, NX, NY, Ly, shift, P, tw,
+                   reinterpret_cast<cplx<R>*>(smem_d4), store);
 }
 
-// --------------------------------------------------------------------------
-// degridding: G lanes per visibility; lane l owns tap column kx = mx + l.
-// mode 0: r = (V' - A x) w in R precision; stored in rz (R) and/or rz32
-//         (FP32 copy for the gridder); partial sum_k |r_k|^2
-// mode 1: x is a step d; t = (A d) w, r_new = rz64 - t with rz64 the FP64
-//         residual at the point the step starts from; partials sum |r_new|^2
-//         and sum |t|^2 (the Lipschitz-test term)
 template <typename R, int G>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rz,
@@ -209,54 +85,9 @@ k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* _
          double* __restrict__ lip_part, int mode, cplx<R>* __restrict__ model,
          const int* __restrict__ perm, const int* __restrict__ yneg, const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   __shared__ double red[32];
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = gid / G, lane = threadIdx.x & (G - 1);
-  const int S = 2 * g.MSP;
-  const R t3x = (R)g.t3x, t3y = (R)g.t3y;
-  C acc = mk<R>(0, 0);
-  if (s < g.M && lane < S) {
-    const int2 m = vp.mxy[s];
-    const C a = vp.a[s];
-    const R l = (R)(lane - g.MSP + 1);
-    const R gx = dev_exp(l * a.x - l * l * t3x);
-    const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - g.MSP + 1);
-    for (int b = 0; b < S; b++) {
-      const R lp = (R)(b - g.MSP + 1);
-      const R gy = dev_exp(lp * a.y - lp * lp * t3y);
-      const C gv = col[(size_t)b * g.Lx];
-      acc.x += gy * gv.x;
-      acc.y += gy * gv.y;
-    }
-    acc.x *= gx;
-    acc.y *= gx;
-  }
-#pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    acc.x += __shfl_xor(acc.x, off, 64);
-    acc.y += __shfl_xor(acc.y, off, 64);
-  }
-  double v[2] = {0.0, 0.0};
-  if (s < g.M && lane == 0) {
-    const R e1s = vp.e1[s] * (R)g.inv_MM;
-    const C F = mk<R>(e1s * acc.x, -e1s * acc.y);
-    const R w = vp.w[s];
-    if (mode == 0) {
-      const C V = vp.vis[s];
-      const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
-      v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-      if (rz) rz[s] = r;
-      if (rz32) rz32[s] = make_float2((float)r.x, (float)r.y);
-    } else {
-      const double tx = (double)F.x * (double)w, ty = (double)F.y * (double)w;
-      const double2 r0 = rz64[s];
-      const double rx = r0.x - tx, ry = r0.y - ty;
-      v[0] = rx * rx + ry * ry;
-      v[1] = tx * tx + ty * ty;
-    }
-    if (model) model[perm[s]] = yneg[s] ? cconj(F) : F;
-  }
+  double v[2];
+  st_degrid<R, G>(blockIdx.x, g, vp, Gf, rz, rz32, rz64, mode, model, perm, yneg, v);
   if (chi_part) {
     block_sum<2>(v, red);
     if (threadIdx.x == 0) {
@@ -266,89 +97,14 @@ k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* _
   }
 }
 
-// --------------------------------------------------------------------------
-// gridding: one workgroup (4 wavefronts) per chunk of at most kGridChunk
-// entries of one TY x TX tile of Hg.  Each wavefront accumulates into its own
-// LDS copy (its lanes own distinct taps, so no LDS atomics); the copies are
-// summed and the nonzero cells added to Hg with one global atomic each.  Heavy
-// tiles (dense uv tracks) are spread over many workgroups.  Hg is zeroed by
-// the adjoint column pass after it has read the grid.
 template <typename R>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_grid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ rz, const int4* __restrict__ chunks,
        const unsigned* __restrict__ ent, cplx<R>* __restrict__ Hg, const int* skip) {
   if (skip && *skip) return;
-  using C = cplx<R>;
   extern __shared__ double4 smem_d4[];
-  C* acc = reinterpret_cast<C*>(smem_d4);
-  const int TX = g.TX, TY = g.TY, TT = TX * TY;
-  const int4 ch = chunks[blockIdx.x];
-  const int tile = ch.x;
-  const int tr = tile / g.ntx, tc = tile - tr * g.ntx;
-  const int rb = tr * TY, cb = tc * TX;
-  for (int t = threadIdx.x; t < 4 * TT; t += blockDim.x) acc[t] = mk<R>(0, 0);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  C* my = acc + wave * TT;
-  const int MSP = g.MSP, NX = g.NX;
-  const R t3x = (R)g.t3x, t3y = (R)g.t3y;
-  for (int e = ch.y + wave; e < ch.z; e += 4) {
-    const unsigned en = ent[e];
-    const int s = (int)(en & 0x7fffffffu);
-    const bool mir = (en >> 31) != 0;
-    const int2 m = vp.mxy[s];
-    const C a = vp.a[s];
-    const R ew = vp.e1[s] * vp.w[s];
-    const C r = rz[s];
-    C d = mk<R>(ew * r.x, -ew * r.y);
-    int lp0, lp1, l0, l1;
-    if (!mir) {
-      lp0 = max(-MSP + 1, rb - m.y);
-      lp1 = min(MSP + 1, rb + TY - m.y);
-      l0 = max(-MSP + 1, cb - m.x - NX);
-      l1 = min(MSP + 1, cb + TX - m.x - NX);
-    } else {
-      d.y = -d.y;
-      lp0 = max(-MSP + 1, -m.y - rb - TY + 1);
-      lp1 = min(MSP + 1, -m.y - rb + 1);
-      l0 = max(-MSP + 1, NX - m.x - cb - TX + 1);
-      l1 = min(MSP + 1, NX - m.x - cb + 1);
-    }
-    const int nl = l1 - l0, nlp = lp1 - lp0;
-    if (nl <= 0 || nlp <= 0) continue;
-    const int ntap = nl * nlp;
-    for (int t = lane; t < ntap; t += 64) {
-      const int ia = t / nl, ib = t - ia * nl;
-      const int lp = lp0 + ia, l = l0 + ib;
-      const R fl = (R)l, flp = (R)lp;
-      const R wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
-      int row, col;
-      if (!mir) {
-        row = m.y + lp - rb;
-        col = m.x + NX + l - cb;
-      } else {
-        row = -(m.y + lp) - rb;
-        col = NX - m.x - l - cb;
-      }
-      C& c = my[row * TX + col];
-      c.x += wgt * d.x;
-      c.y += wgt * d.y;
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < TT; t += blockDim.x) {
-    const int rr = t / TX, cc = t - rr * TX;
-    const int row = rb + rr, col = cb + cc;
-    C sum = acc[t];
-    sum = cadd(sum, acc[TT + t]);
-    sum = cadd(sum, acc[2 * TT + t]);
-    sum = cadd(sum, acc[3 * TT + t]);
-    if (row <= g.NY && col < g.Lx && (sum.x != (R)0 || sum.y != (R)0)) {
-      R* dst = reinterpret_cast<R*>(Hg + (size_t)row * g.Lx + col);
-      unsafeAtomicAdd(dst, sum.x);
-      unsafeAtomicAdd(dst + 1, sum.y);
-    }
-  }
+  st_grid<R>(blockIdx.x, g, vp, [&](int s) { return rz[s]; }, chunks, ent, Hg,
+             reinterpret_cast<cplx<R>*>(smem_d4));
 }
 
 // --------------------------------------------------------------------------

@@ -45,7 +45,14 @@ struct Context {
   NufftState nu;
   // solver workspaces
   DevMem x, z, xnew, g, dstep, x64, part, stat, stat64, chi64, state, cost;
-  DevMem fgp;  // TV prox duals
+  DevMem fgp;   // TV prox duals
+  DevMem fgpc;  // FGP momentum coefficients
+  DevMem bar;   // grid barrier of the persistent solver
+  DevMem px, pxn, pz;  // persistent solver: FP64 iterates x, xnew, z
+  DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
+  DevMem vmod;  // persistent solver: three FP64 weighted model vectors
+  int persist_blocks_per_cu = 1;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
+  int persist_blocks = 0;  // workgroups of the last persistent launch
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

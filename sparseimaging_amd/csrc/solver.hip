@@ -10,6 +10,7 @@
 #include <chrono>
 #include <type_traits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -311,6 +312,7 @@ template void nufft_fwd<float>(Context&, const float*, float2*, double*, float2*
 template void nufft_fwd<double>(Context&, const double*, double2*, double*, double2*, const int*, float2*);
 template void nufft_adj<float>(Context&, const float2*, float*, const int*);
 template void nufft_fwd_delta<float>(Context&, const float*, double*, double*, const int*);
+template void nufft_fwd_delta<double>(Context&, const double*, double*, double*, const int*);
 template void nufft_adj<double>(Context&, const double2*, double*, const int*);
 
 // gather y (input order) into the reflected frame, sorted order
@@ -393,22 +395,27 @@ int nufft_chi2_64(Context& C, const double* d_x64, double* chi2) {
 // The momentum-point residual (the objective of the backtracking test and the
 // input of the gradient) is evaluated with an FP64 forward NUFFT; the step
 // transform, the gradient and the prox stay FP32.
-struct NufftOp : Operator<float> {
+// The stream engine's NUFFT operator.  Iterates are FP64 (FP32 iterates
+// stagnate near a tight tolerance: the prox step falls below their rounding);
+// the residual at the momentum point comes from an FP64 forward transform,
+// the gradient from the FP32 adjoint of its FP32 copy, and the trial's A d
+// from an FP64 forward transform of the step.
+struct NufftOp : Operator<double> {
   Context& C;
   explicit NufftOp(Context& c) : C(c) {}
   int npart() const override { return degrid_blocks(C.nu.M, C.nu.MSP); }
-  void fwd(const float* x, bool keep, double* part, const int* skip) override {
-    double* x64 = (double*)C.x64.p;
-    launch_cast<float, double>(x, x64, C.nu.NX * C.nu.NY, C.stream, skip);
-    nufft_fwd<double>(C, x64, keep ? (double2*)C.nu.rw64.p : nullptr, part, nullptr, skip,
+  void fwd(const double* x, bool keep, double* part, const int* skip) override {
+    nufft_fwd<double>(C, x, keep ? (double2*)C.nu.rw64.p : nullptr, part, nullptr, skip,
                       keep ? (float2*)C.nu.rw.p : nullptr);
   }
-  void adj(float* g, const int* skip) override {
-    nufft_adj<float>(C, (const float2*)C.nu.rw.p, g, skip);
+  void adj(double* g, const int* skip) override {
+    float* g32 = (float*)C.g32.p;
+    nufft_adj<float>(C, (const float2*)C.nu.rw.p, g32, skip);
+    launch_cast<float, double>(g32, g, C.nu.NX * C.nu.NY, C.stream, skip);
   }
   bool has_delta() const override { return true; }
-  void fwd_delta(const float* d, double* chi_part, double* lip_part, const int* skip) override {
-    nufft_fwd_delta<float>(C, d, chi_part, lip_part, skip);
+  void fwd_delta(const double* d, double* chi_part, double* lip_part, const int* skip) override {
+    nufft_fwd_delta<double>(C, d, chi_part, lip_part, skip);
   }
 };
 
@@ -452,7 +459,9 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     printf("nonneg_flag must be chosen properly.\n");
     rc = 0;
   } else {
-    if (nufft_ensure64(C) || C.x64.ensure((size_t)A.NX * A.NY * 8)) return MFISTA_E_HIP;
+    if (nufft_ensure64(C) || C.x64.ensure((size_t)A.NX * A.NY * 8) ||
+        C.g32.ensure((size_t)A.NX * A.NY * 4))
+      return MFISTA_E_HIP;
     NufftOp op(C);
     RunParams P;
     P.NX = A.NX;
@@ -472,7 +481,14 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     P.xinit = A.xinit;
     P.xout = A.xout;
     P.box = A.box;
-    rc = mfista_run<float>(C, op, P, &rs);
+    int coop = 0, dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
+    const int engine = opts ? opts->engine : MFISTA_ENGINE_PERSISTENT;
+    if (engine == MFISTA_ENGINE_PERSISTENT && coop)
+      rc = solve_nufft_persistent(C, P, &rs);
+    else
+      rc = mfista_run<double>(C, op, P, &rs);
     C.last_steps = rs.steps;
   }
   res->ITER = rs.ITER;
@@ -480,6 +496,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
   if (opts) {
     opts->max_trials = rs.max_trials;
     opts->total_trials = rs.total_trials;
+    opts->kernel_ms = rs.kernel_ms;
   }
   if (rc) return rc;
   MF_CHECK(hipStreamSynchronize(C.stream));
@@ -521,6 +538,11 @@ Context* get_context() {
     }
     (void)hipEventCreateWithFlags(&C->ev[0], hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&C->ev[1], hipEventDisableTiming);
+    // tuning knob: workgroups per CU of the persistent solver (1 to 4, capped by occupancy)
+    if (const char* e = getenv("MFISTA_PERSISTENT_BLOCKS_PER_CU")) {
+      const int v = atoi(e);
+      if (v >= 1 && v <= 4) C->persist_blocks_per_cu = v;
+    }
     g_ctx[dev] = C;
   }
   return g_ctx[dev];
@@ -548,6 +570,7 @@ void mfista_default_opts(mfista_opts* o) {
   o->preset = MFISTA_PRESET_C24;
   o->fgp_iter = FGPITER;
   o->rel_tol = 0.0;
+  o->engine = MFISTA_ENGINE_PERSISTENT;
 }
 
 int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double* d_vis_r,
@@ -743,11 +766,28 @@ void y2x_nufft(double* u_dx, double* v_dy, int M, int Nx, int Ny, double* yin_r,
 int mfista_profile_stages(int reps, double* ms, int nstage) {
   std::lock_guard<std::mutex> lock(global_mutex());
   Context* C = get_context();
-  if (!C || !C->nu.ready || !C->x.p || reps <= 0 || !ms) return MFISTA_E_ARG;
+  if (!C || !C->nu.ready || reps <= 0 || !ms) return MFISTA_E_ARG;
   NufftState& N = C->nu;
   hipStream_t st = C->stream;
   const int NX = N.NX, NY = N.NY, n = NX * NY;
   const GridGeom& g = N.geom;
+  // the stream-engine buffers these kernels use, allocated here: the last
+  // solve may have run on the persistent engine, which has its own
+  {
+    const size_t rb = (size_t)n * 4;
+    const size_t np = (size_t)degrid_blocks(N.M, N.MSP) + (size_t)(kProxW + 1) * elem_blocks(n);
+    if (C->z.ensure(rb) || C->xnew.ensure(rb) || C->g.ensure(rb) || C->dstep.ensure(rb) ||
+        C->part.ensure(np * 8) || C->state.ensure(sizeof(DevState)))
+      return MFISTA_E_HIP;
+    MF_CHECK(hipMemsetAsync(C->z.p, 0, rb, st));
+    MF_CHECK(hipMemsetAsync(C->g.p, 0, rb, st));
+    DevState h;
+    memset(&h, 0, sizeof(h));
+    h.c = 1.0;
+    h.lambda_l1 = 1.0;
+    h.nonneg = 1;
+    MF_CHECK(hipMemcpy(C->state.p, &h, sizeof(h), hipMemcpyHostToDevice));
+  }
   float* z = (float*)C->z.p;
   float* xnew = (float*)C->xnew.p;
   float* gr = (float*)C->g.p;

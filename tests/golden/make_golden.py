@@ -126,7 +126,37 @@ def reference_data():
                             v=v.astype(np.int32), y_r=yr, y_i=yi, noise=sd, NX=hdr["NX"], NY=hdr["NY"])
 
 
+def oracle_refdata_solutions():
+    """Converged FP64 oracle solutions on the reference's nufft_data.txt with the
+    author's run parameters (matlab/testdata4nufft.mat: lambda_l1 = 100,
+    lambda_tsv = 100, cinit = 5e8, nonneg), run to a 1e-8 relative objective
+    tolerance.  The oracle is pinned separately (closed forms, independent
+    optimiser); this is the regression target of the GPU reference-data test.
+    Slow (minutes per preset)."""
+    import sys
+
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    import oracle
+
+    d = np.load(os.path.join(HERE, "ref_nufft_data.npz"))
+    N = int(d["NX"])
+    out = {}
+    for preset in (oracle.PRESET_CXX12, oracle.PRESET_C24):
+        o = oracle.imaging_core_nufft(d["u"], d["v"], d["vis_r"], d["vis_i"], d["std"], N, N, 20000, 0.0,
+                                      100.0, 0.0, 100.0, 5e8, nonneg=1, preset=preset, rel_tol=1e-8)
+        out[f"x{preset}"] = o["x"].astype(np.float32)
+        out[f"finalcost{preset}"] = o["result"]["finalcost"]
+        out[f"iter{preset}"] = o["result"]["ITER"]
+        print("preset", preset, o["result"]["ITER"], o["result"]["finalcost"], flush=True)
+    np.savez_compressed(os.path.join(HERE, "ref_nufft_solution.npz"), **out)
+
+
 if __name__ == "__main__":
+    import sys as _sys
+
+    if len(_sys.argv) > 1 and _sys.argv[1] == "solutions":
+        oracle_refdata_solutions()
+        raise SystemExit(0)
     nufft_golden(32, 40, 3201, 1.5)
     nufft_golden(16, 25, 1601, 0.5)
     fftmode_golden()
