@@ -78,7 +78,7 @@ k_rowfft_c2r(const cplx<R>* __restrict__ U, const R* __restrict__ e4x, const R* 
                    reinterpret_cast<cplx<R>*>(smem_d4), store);
 }
 
-template <typename R, int G>
+template <typename R, int MSPC>
 __global__ void __launch_bounds__(MF_BLOCK)
 k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rz,
          float2* __restrict__ rz32, const double2* __restrict__ rz64, double* __restrict__ chi_part,
@@ -87,7 +87,7 @@ k_degrid(GridGeom g, VisParams<R> vp, const cplx<R>* __restrict__ Gf, cplx<R>* _
   if (skip && *skip) return;
   __shared__ double red[32];
   double v[2];
-  st_degrid<R, G>(blockIdx.x, g, vp, Gf, rz, rz32, rz64, mode, model, perm, yneg, v);
+  st_degrid<R, MSPC>(blockIdx.x, g, vp, Gf, rz, rz32, rz64, mode, model, perm, yneg, v);
   if (chi_part) {
     block_sum<2>(v, red);
     if (threadIdx.x == 0) {
@@ -290,7 +290,8 @@ void launch_rowfft_c2r(const cplx<R>* U, const R* e4x, const R* e4y, R scale, R*
                      NX, NY, Ly, shift, P, tw, nseq, skip);
 }
 
-static inline int degrid_lanes(int MSP) { return (2 * MSP <= 16) ? 16 : 32; }
+// the presets use MSP 6 (CXX-12) or 12 (C-24); the degridder is compiled for both
+static inline int degrid_lanes(int MSP) { return (MSP == 6) ? DegridShape<6>::G : DegridShape<12>::G; }
 
 int degrid_blocks(int M, int MSP) {
   const int G = degrid_lanes(MSP);
@@ -303,11 +304,11 @@ void launch_degrid(const GridGeom& g, const VisParams<R>& vp, const cplx<R>* Gf,
                    float2* rz32, const double2* rz64, double* chi_part, double* lip_part, int mode,
                    int nblk, cplx<R>* model, const int* perm, const int* yneg_sorted,
                    hipStream_t st, const int* skip) {
-  if (degrid_lanes(g.MSP) == 16)
-    hipLaunchKernelGGL((k_degrid<R, 16>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
+  if (g.MSP == 6)
+    hipLaunchKernelGGL((k_degrid<R, 6>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
                        chi_part, lip_part, mode, model, perm, yneg_sorted, skip);
   else
-    hipLaunchKernelGGL((k_degrid<R, 32>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
+    hipLaunchKernelGGL((k_degrid<R, 12>), dim3(nblk), dim3(MF_BLOCK), 0, st, g, vp, Gf, rz, rz32, rz64,
                        chi_part, lip_part, mode, model, perm, yneg_sorted, skip);
 }
 

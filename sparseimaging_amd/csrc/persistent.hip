@@ -43,6 +43,7 @@ struct PArgs {
   double2* vm[3];  // weighted model vectors W A (.), roles rotate on acceptance
   const int4* chunks;
   int nchunks;
+  const int* granges;  // chunk range [granges[b], granges[b+1]) of workgroup b
   const unsigned* ent;
   double *x, *xnew, *z;  // iterates in FP64 (FP32 iterates stagnate near a tight tolerance)
   float* gr;
@@ -51,51 +52,177 @@ struct PArgs {
   const double* fgp_c;  // c1[k] = fgp_c[2k], c2[k] = fgp_c[2k+1]
   int fgp_iter, tv_mode, maxiter, nonneg;
   double lambda_l1, lambda_tsv, lambda_tv, eps_abs, rel_tol, eta_first, cinit, adj_scale;
-  double* part;
+  double* part;  // kPW doubles per workgroup (see kP* below)
   int n_deg, n_row, n_ky, n_elem;
+  int lds_tables;  // twiddle and E4 tables staged in LDS
   double* cost;
   DevState* S;
-  unsigned long long* bar;  // [0] arrival counter, [1] error flag
+  unsigned long long* bar;  // barrier words, see BarCtx
+  int hier_barrier;         // XCD-hierarchical grid barrier
 };
 
-// Grid-wide barrier for a cooperative launch.  bar[0] is a monotonic 64-bit
-// arrival counter: the k-th barrier is passed once it reaches k * nblk, so no
-// reset (and no ordering of a reset) is needed.  Each workgroup pays exactly
-// one agent-scope release fence (L2 write-back) before arriving and one
-// acquire fence (L2 invalidate) after the wait; the wait itself polls with
-// relaxed atomic loads, which leave the caches alone.  A bounded spin sets
-// the error flag bar[1] instead of waiting for ever.
-__device__ inline bool grid_sync(unsigned long long* bar, unsigned nblk, unsigned long long& epoch) {
+// Grid-wide barriers for a cooperative launch.  bar[] layout (64-bit words):
+//   [0] arrivals of the flat barrier, or of the XCD leaders
+//   [1] error flag (a bounded wait that runs out sets it; every wait checks it)
+// Every signal word is written with an atomic read-modify-write: those are
+// performed at the device coherence point, where the polling loads (relaxed,
+// agent scope) read; a plain store would stay in the writer's L2.
+//   [2] arrivals of the set-up barrier, [3] XCD id out of range
+//   [4..7] the first failed wait: word, target, value seen, workgroup | polls << 20
+//   XCD x: [kXBase + kXStride x + 0] local arrivals, [+8] members, [+16] release epoch
+// (each on its own 64-byte line).
+constexpr int kXBase = 32, kXStride = 32;
+constexpr size_t kBarBytes = (size_t)(kXBase + 8 * kXStride) * 8;
+constexpr long kSpinLimit = 1L << 26;
+
+struct BarCtx {
+  unsigned long long* bar;
+  unsigned nblk;   // workgroups in the launch
+  unsigned nx;     // workgroups on this workgroup's XCD
+  unsigned ngrp;   // XCDs with workgroups
+  int xcc;         // this workgroup's XCD
+  int leader;      // first workgroup of its XCD
+  int hier;        // 1: XCD-hierarchical barrier, 0: flat
+};
+
+// bounded wait until bar[i] >= target; false (and the error flag set) on
+// time-out.  RMW = true polls with atomic read-modify-writes (fetch_add 0),
+// which are performed at the coherence point: measured on gfx950, relaxed
+// atomic loads of a word that one workgroup increments are seen by the other
+// workgroups of its XCD only after ~20 us, RMW polls within ~1 us.
+template <bool RMW>
+__device__ inline bool wait_ge(unsigned long long* bar, int i, unsigned long long target) {
+  long spins = 0;
+  while (true) {
+    const unsigned long long v =
+        RMW ? __hip_atomic_fetch_add(&bar[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+            : __hip_atomic_load(&bar[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v >= target) return true;
+    __builtin_amdgcn_s_sleep(1);
+    if ((++spins & 1023) == 0 &&
+        (__hip_atomic_fetch_add(&bar[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || spins > kSpinLimit)) {
+      // the first workgroup to give up records what it waited for
+      if (__hip_atomic_fetch_or(&bar[1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        __hip_atomic_fetch_or(&bar[4], (unsigned long long)i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(&bar[5], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(&bar[6], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(&bar[7], (unsigned long long)blockIdx.x | ((unsigned long long)spins << 20),
+                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return false;
+    }
+  }
+}
+
+// Flat barrier on counter bar[ci]: one agent-scope release fence (L2
+// write-back) before arriving and one acquire fence (L2 invalidate) after the
+// wait, per workgroup.  The k-th barrier is passed once the monotonic counter
+// reaches k * nblk, so no reset is needed.
+__device__ inline bool flat_sync(unsigned long long* bar, int ci, unsigned nblk, unsigned long long epoch) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __hip_atomic_fetch_add(&bar[ci], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const bool ok = wait_ge<false>(bar, ci, epoch * (unsigned long long)nblk);
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return ok;
+}
+
+// XCD-hierarchical barrier.  The L2 is per XCD and the L1 is write-through,
+// so once every wave of the XCD's workgroups has its stores acknowledged
+// (s_waitcnt vmcnt(0)) the data sits in that XCD's L2.  Only the XCD leader
+// then writes the L2 back (release fence) and arrives at the global counter;
+// after all leaders have arrived it releases its XCD, and every workgroup
+// takes an acquire fence.  One L2 write-back per XCD instead of one per
+// workgroup, and 8-way instead of nblk-way contention on the global counter.
+__device__ inline bool xcd_sync(const BarCtx& B, unsigned long long epoch) {
+  unsigned long long* bar = B.bar;
+  const int xb = kXBase + kXStride * B.xcc;
+  __hip_atomic_fetch_add(&bar[xb], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (B.leader) {
+    if (!wait_ge<true>(bar, xb, epoch * (unsigned long long)B.nx)) return false;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&bar[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!wait_ge<true>(bar, 0, epoch * (unsigned long long)B.ngrp)) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __hip_atomic_fetch_add(&bar[xb + 16], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // = epoch
+    return true;
+  }
+  if (!wait_ge<true>(bar, xb + 16, epoch)) return false;
+  // agent-scope acquire: a work-group-scope invalidate does not reach the L1
+  // of a non-split work-group, which could then serve stale lines
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+__device__ inline bool grid_sync(const BarCtx& B, unsigned long long& epoch) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   __shared__ int fail;
   epoch += 1;
+  if (threadIdx.x == 0) fail = (B.hier ? xcd_sync(B, epoch) : flat_sync(B.bar, 0, B.nblk, epoch)) ? 0 : 1;
+  __syncthreads();
+  return fail == 0;
+}
+
+// Set-up: every workgroup learns its XCD (hardware register XCC_ID), the XCD
+// member counts and its leader; one flat barrier publishes the counts.  The
+// hierarchical barrier is used only if every XCD id is in range.
+__device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, int want_hier) {
   if (threadIdx.x == 0) {
-    fail = 0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(&bar[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long target = epoch * (unsigned long long)nblk;
-    long spins = 0;
-    while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(1);
-      if ((++spins & 1023) == 0 &&
-          (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
-           spins > (1L << 26))) {
-        __hip_atomic_store(&bar[1], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        fail = 1;
-        break;
-      }
+    unsigned v;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    B.bar = bar;
+    B.nblk = gridDim.x;
+    B.xcc = (int)(v & 0xf);
+    B.leader = 0;
+    B.hier = 0;
+    if (B.xcc < 8) {
+      const unsigned long long idx = __hip_atomic_fetch_add(&bar[kXBase + kXStride * B.xcc + 8], 1ull,
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      B.leader = (idx == 0);
+    } else {
+      __hip_atomic_fetch_or(&bar[3], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // out of range
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+  __shared__ int fail;
+  if (threadIdx.x == 0) {
+    fail = flat_sync(bar, 2, gridDim.x, 1) ? 0 : 1;
+    if (!fail) {
+      unsigned ng = 0;
+      for (int x = 0; x < 8; x++)
+        ng += __hip_atomic_fetch_add(&bar[kXBase + kXStride * x + 8], 0ull, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) ? 1u : 0u;
+      B.ngrp = ng;
+      B.nx = (B.xcc < 8) ? (unsigned)__hip_atomic_fetch_add(&bar[kXBase + kXStride * B.xcc + 8], 0ull,
+                                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : 0u;
+      const bool bad = __hip_atomic_fetch_add(&bar[3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      B.hier = (want_hier && !bad) ? 1 : 0;
+    }
   }
   __syncthreads();
   return fail == 0;
 }
 
-__device__ inline double blk_reduce(const double* p, int cnt, int stride, double* red) {
-  double v[1] = {0.0};
-  for (int i = threadIdx.x; i < cnt; i += blockDim.x) v[0] += p[(size_t)i * stride];
-  block_sum<1>(v, red);
-  return v[0];
+// Per-workgroup partial sums: one row of kPW doubles per workgroup, written
+// by the stage that produces them and reduced (in workgroup order, the same
+// in every workgroup) by the control step.
+constexpr int kPW = 8;
+enum { kPD2 = 0, kPL1 = 1, kPReg = 2, kPTsvd = 3, kPChi = 4, kPLip = 5 };
+// init-phase slots reuse the same row: |x|_1, TSV(x), TV(x), chi^2(x)
+enum { kPIl1 = 0, kPItsv = 1, kPItv = 2, kPIchi = 4 };
+
+template <int NV>
+__device__ inline void part_reduce(const double* part, int nb, const int (&slot)[NV], double (&out)[NV],
+                                   double* red) {
+#pragma unroll
+  for (int q = 0; q < NV; q++) out[q] = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    const double* row = part + (size_t)i * kPW;
+#pragma unroll
+    for (int q = 0; q < NV; q++) out[q] += row[slot[q]];
+  }
+  block_sum<NV>(out, red);
 }
 
 struct BlockState {
@@ -172,82 +299,272 @@ __device__ inline float lpq_f(const float* p, const float* q, int i, int j, int 
   return v;
 }
 
-constexpr int kT1W = 2;  // per-row-task trial partials: |d|^2, |xnew|_1
+// Gridding of this workgroup's range [cb, ce) of the tile-ordered chunk list
+// (ranges balanced by entry count on the host).  The entries of up to kGB
+// chunks are loaded at once, one thread per entry, into LDS (cell, tap
+// slopes, deposit value E1 w conj(r)), so the dependent global loads of a
+// whole batch overlap.  Consecutive chunks of one tile accumulate into the
+// same LDS tile (LDS float atomics); the tile is flushed when the tile
+// changes: with plain stores when this workgroup processed all of the tile's
+// chunks (Hg is zero there), otherwise with one global atomic per touched
+// cell.  Deposits as st_grid (stages.h).
+constexpr int kGB = MF_BLOCK / kGridChunk;
 
-template <int G>
+template <typename Resid>
+__device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp, Resid resid,
+                                   const int4* __restrict__ chunks, int cb, int ce,
+                                   const unsigned* __restrict__ ent, float2* __restrict__ Hg, char* lds) {
+  const int TX = g.TX, TY = g.TY, TT = TX * TY;
+  float2* tile = reinterpret_cast<float2*>(lds);
+  int2* em = reinterpret_cast<int2*>(tile + TT);
+  float2* ea = reinterpret_cast<float2*>(em + MF_BLOCK);
+  float2* ed = ea + MF_BLOCK;
+  int* eflag = reinterpret_cast<int*>(ed + MF_BLOCK);  // -1 empty, 0 direct, 1 mirror
+  int4* cmeta = reinterpret_cast<int4*>(eflag + MF_BLOCK);  // kGB chunk records
+  const int MSP = g.MSP, NX = g.NX;
+  const float t3x = (float)g.t3x, t3y = (float)g.t3y;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int cur = -1;        // tile in the LDS accumulator
+  int cur_first = 0;   // its first chunk was processed here
+  auto flush = [&](int tid, bool owned) {
+    const int tr = tid / g.ntx, tc = tid - tr * g.ntx;
+    const int rb = tr * TY, cbx = tc * TX;
+    for (int t = threadIdx.x; t < TT; t += blockDim.x) {
+      const int rr = t / TX, cc = t - rr * TX;
+      const int row = rb + rr, col = cbx + cc;
+      const float2 v = tile[t];
+      if (row <= g.NY && col < g.Lx && (v.x != 0.f || v.y != 0.f)) {
+        float2* dst = Hg + (size_t)row * g.Lx + col;
+        if (owned) {
+          *dst = v;
+        } else {
+          unsafeAtomicAdd(&dst->x, v.x);
+          unsafeAtomicAdd(&dst->y, v.y);
+        }
+      }
+    }
+  };
+  for (int c0 = cb; c0 < ce; c0 += kGB) {
+    const int nc = min(kGB, ce - c0);
+    __syncthreads();  // the previous batch's entries are consumed
+    {
+      const int q = threadIdx.x, j = q / kGridChunk;
+      int flag = -1;
+      if (j < nc) {
+        const int4 ch = chunks[c0 + j];
+        if ((q % kGridChunk) == 0) cmeta[j] = ch;
+        const int e = ch.y + (q % kGridChunk);
+        if (e < ch.z) {
+          const unsigned en = ent[e];
+          const int sv = (int)(en & 0x7fffffffu);
+          flag = (int)(en >> 31);
+          em[q] = vp.mxy[sv];
+          ea[q] = vp.a[sv];
+          const float ew = vp.e1[sv] * vp.w[sv];
+          const float2 r = resid(sv);
+          ed[q] = make_float2(ew * r.x, flag ? ew * r.y : -ew * r.y);
+        }
+      }
+      eflag[q] = flag;
+    }
+    __syncthreads();
+    for (int j = 0; j < nc; j++) {
+      const int4 ch = cmeta[j];
+      if (ch.x != cur) {
+        if (cur >= 0) {
+          __syncthreads();
+          flush(cur, false);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < TT; t += blockDim.x) tile[t] = make_float2(0.f, 0.f);
+        __syncthreads();
+        cur = ch.x;
+        cur_first = ch.w & 1;
+      }
+      const int tr = ch.x / g.ntx, tc = ch.x - tr * g.ntx;
+      const int rb = tr * TY, cbx = tc * TX;
+      for (int q = j * kGridChunk + wave; q < (j + 1) * kGridChunk; q += MF_BLOCK / 64) {
+        const int flag = eflag[q];
+        if (flag < 0) continue;
+        const int2 m = em[q];
+        const float2 a = ea[q];
+        const float2 d = ed[q];
+        int lp0, lp1, l0, l1;
+        if (!flag) {
+          lp0 = max(-MSP + 1, rb - m.y);
+          lp1 = min(MSP + 1, rb + TY - m.y);
+          l0 = max(-MSP + 1, cbx - m.x - NX);
+          l1 = min(MSP + 1, cbx + TX - m.x - NX);
+        } else {
+          lp0 = max(-MSP + 1, -m.y - rb - TY + 1);
+          lp1 = min(MSP + 1, -m.y - rb + 1);
+          l0 = max(-MSP + 1, NX - m.x - cbx - TX + 1);
+          l1 = min(MSP + 1, NX - m.x - cbx + 1);
+        }
+        const int nl = l1 - l0, nlp = lp1 - lp0;
+        if (nl <= 0 || nlp <= 0) continue;
+        const int ntap = nl * nlp;
+        for (int t = lane; t < ntap; t += 64) {
+          const int ia = t / nl, ib = t - ia * nl;
+          const int lp = lp0 + ia, l = l0 + ib;
+          const float fl = (float)l, flp = (float)lp;
+          const float wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
+          int row, col;
+          if (!flag) {
+            row = m.y + lp - rb;
+            col = m.x + NX + l - cbx;
+          } else {
+            row = -(m.y + lp) - rb;
+            col = NX - m.x - l - cbx;
+          }
+          float* cell = reinterpret_cast<float*>(tile + row * TX + col);
+          atomicAdd(cell, wgt * d.x);
+          atomicAdd(cell + 1, wgt * d.y);
+        }
+      }
+      if (ch.w & 2) {  // last chunk of the tile: flush now
+        __syncthreads();
+        flush(cur, cur_first != 0);
+        cur = -1;
+      }
+    }
+  }
+  if (cur >= 0) {
+    __syncthreads();
+    flush(cur, false);
+  }
+  __syncthreads();
+}
+
+// LDS bytes of the batched gridder's tile and entry arrays
+__host__ __device__ inline size_t grid_lds_bytes(int TX, int TY) {
+  return (size_t)TX * TY * 8 + (size_t)MF_BLOCK * (8 + 8 + 8 + 4) + (size_t)kGB * 16;
+}
+
+// LDS bytes of the staged twiddle (FP32 and FP64, both axes) and E4 tables
+__host__ __device__ inline size_t table_lds_bytes(int Lx, int Ly, int NX, int NY) {
+  return (size_t)(Lx + Ly) * (16 + 8) + (size_t)(NX + NY) * (8 + 4);
+}
+
+template <int MSPC>
 __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
   extern __shared__ double4 smem_d4[];
-  __shared__ double red[16 * kProxW];
+  __shared__ double red[16 * kPW];
   __shared__ BlockState bs;
   __shared__ double ring[kTD + 1];
-  float2* sm32 = reinterpret_cast<float2*>(smem_d4);
-  double2* sm64 = reinterpret_cast<double2*>(smem_d4);
+  __shared__ BarCtx B;
   const GridGeom& g = A.g;
   const unsigned nb = gridDim.x;
+  const int lb = blockIdx.x;
   const int NX = g.NX, NY = g.NY, n = NX * NY, W = NX, H = NY;
   const bool tsv = A.lambda_tsv > 0.0;
   const bool tv = A.tv_mode != 0;
   unsigned long long epoch = 0;  // barriers passed by this workgroup
-  // partial-sum regions
-  double* P_chix = A.part;
-  double* P_lip = P_chix + A.n_deg;
-  double* P_t1 = P_lip + A.n_deg;                 // kT1W per row task
-  double* P_reg = P_t1 + kT1W * A.n_row;          // 2 per element task
-  double* P_init = P_reg + 2 * A.n_elem;          // 4 per row task
+  double* prow = A.part + (size_t)lb * kPW;
+
+  if (!bar_setup(B, A.bar, A.hier_barrier)) return;
+
+  // LDS: [tables][stage scratch]
+  char* lds = reinterpret_cast<char*>(smem_d4);
+  const double2 *twxd = A.twxd, *twyd = A.twyd;
+  const float2 *twxf = A.twxf, *twyf = A.twyf;
+  const double *e4xd = A.e4xd, *e4yd = A.e4yd;
+  const float *e4xf = A.e4xf, *e4yf = A.e4yf;
+  if (A.lds_tables) {
+    double2* t64x = reinterpret_cast<double2*>(lds);
+    double2* t64y = t64x + g.Lx;
+    float2* t32x = reinterpret_cast<float2*>(t64y + g.Ly);
+    float2* t32y = t32x + g.Lx;
+    double* q64x = reinterpret_cast<double*>(t32y + g.Ly);
+    double* q64y = q64x + NX;
+    float* q32x = reinterpret_cast<float*>(q64y + NY);
+    float* q32y = q32x + NX;
+    for (int i = threadIdx.x; i < g.Lx; i += blockDim.x) {
+      t64x[i] = A.twxd[i];
+      t32x[i] = A.twxf[i];
+    }
+    for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) {
+      t64y[i] = A.twyd[i];
+      t32y[i] = A.twyf[i];
+    }
+    for (int i = threadIdx.x; i < NX; i += blockDim.x) {
+      q64x[i] = A.e4xd[i];
+      q32x[i] = A.e4xf[i];
+    }
+    for (int i = threadIdx.x; i < NY; i += blockDim.x) {
+      q64y[i] = A.e4yd[i];
+      q32y[i] = A.e4yf[i];
+    }
+    twxd = t64x;
+    twyd = t64y;
+    twxf = t32x;
+    twyf = t32y;
+    e4xd = q64x;
+    e4yd = q64y;
+    e4xf = q32x;
+    e4yf = q32y;
+    lds += (table_lds_bytes(g.Lx, g.Ly, NX, NY) + 255) & ~(size_t)255;
+    __syncthreads();
+  }
+  float2* sm32 = reinterpret_cast<float2*>(lds);
+  double2* sm64 = reinterpret_cast<double2*>(lds);
 
   // ---------------- initial cost and models: W A x0 -> vm[0] (x), vm[2] (z)
-  for (int t = blockIdx.x; t < A.n_row; t += nb) {
-    double st[4] = {0, 0, 0, 0};
-    auto load = [&](int i, int j) -> double {
-      const int f = i * NY + j;
-      const double xc = A.x[f];
-      A.z[f] = xc;
-      st[0] += fabs(xc);
-      st[3] += (xc != 0.0) ? 1.0 : 0.0;
-      const int c = f % W, r = f / W;
-      double dh = 0, dv = 0;
-      if (c < W - 1) dh = xc - A.x[f + 1];
-      if (r < H - 1) dv = xc - A.x[f + W];
-      st[1] += dh * dh + dv * dv;
-      if (c < W - 1 && r < H - 1)
-        st[2] += sqrt(dh * dh + dv * dv);
-      else if (c < W - 1)
-        st[2] += fabs(dh);
-      else if (r < H - 1)
-        st[2] += fabs(dv);
-      return xc * A.e4xd[i] * A.e4yd[j];
-    };
-    st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, A.twyd, A.T64, sm64, load);
-    block_sum<4>(st, red);
-    if (threadIdx.x == 0)
-      for (int q = 0; q < 4; q++) P_init[4 * t + q] = st[q];
-  }
-  if (!grid_sync(A.bar, nb, epoch)) return;
-  for (int t = blockIdx.x; t < A.n_ky; t += nb)
-    st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, A.twxd, sm64);
-  if (!grid_sync(A.bar, nb, epoch)) return;
-  for (int t = blockIdx.x; t < A.n_deg; t += nb) {
-    double v[2];
-    st_degrid_model<G>(t, g, A.pd, A.Gf64, nullptr, A.vm[0], A.vm[2], v);
-    block_sum<2>(v, red);
-    if (threadIdx.x == 0) P_chix[t] = v[0];
-  }
-  if (!grid_sync(A.bar, nb, epoch)) return;
   {
-    const double chi = blk_reduce(P_chix, A.n_deg, 1, red);
-    const double l1 = blk_reduce(P_init + 0, A.n_row, 4, red);
-    const double ts = blk_reduce(P_init + 1, A.n_row, 4, red);
-    const double tvs = blk_reduce(P_init + 2, A.n_row, 4, red);
+    double st[4] = {0, 0, 0, 0};
+    for (int t = lb; t < A.n_row; t += nb) {
+      auto load = [&](int i, int j) -> double {
+        const int f = i * NY + j;
+        const double xc = A.x[f];
+        A.z[f] = xc;
+        st[0] += fabs(xc);
+        const int c = f % W, r = f / W;
+        double dh = 0, dv = 0;
+        if (c < W - 1) dh = xc - A.x[f + 1];
+        if (r < H - 1) dv = xc - A.x[f + W];
+        st[1] += dh * dh + dv * dv;
+        if (c < W - 1 && r < H - 1)
+          st[2] += sqrt(dh * dh + dv * dv);
+        else if (c < W - 1)
+          st[2] += fabs(dh);
+        else if (r < H - 1)
+          st[2] += fabs(dv);
+        return xc * e4xd[i] * e4yd[j];
+      };
+      st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, sm64, load);
+    }
+    block_sum<4>(st, red);
     if (threadIdx.x == 0) {
-      double c0 = chi / 2 + A.lambda_l1 * l1;
+      prow[kPIl1] = st[0];
+      prow[kPItsv] = st[1];
+      prow[kPItv] = st[2];
+    }
+  }
+  if (!grid_sync(B, epoch)) return;
+  for (int t = lb; t < A.n_ky; t += nb)
+    st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
+  if (!grid_sync(B, epoch)) return;
+  {
+    double v[2] = {0, 0};
+    for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, nullptr, A.vm[0], A.vm[2], v);
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) prow[kPIchi] = v[0];
+  }
+  if (!grid_sync(B, epoch)) return;
+  {
+    const int slot[4] = {kPIchi, kPIl1, kPItsv, kPItv};
+    double sum[4];
+    part_reduce<4>(A.part, nb, slot, sum, red);
+    if (threadIdx.x == 0) {
+      double c0 = sum[0] / 2 + A.lambda_l1 * sum[1];
       if (tv)
-        c0 += A.lambda_tv * tvs;
+        c0 += A.lambda_tv * sum[3];
       else if (tsv)
-        c0 += A.lambda_tsv * ts;
+        c0 += A.lambda_tsv * sum[2];
       bs.c = A.cinit;
       bs.mu = 1.0;
       bs.costtmp = c0;
-      bs.l1_x = l1;
+      bs.l1_x = sum[1];
       bs.iter = 0;
       bs.trial = 0;
       bs.accept = 0;
@@ -277,15 +594,12 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
       const double2* mn = A.vm[bs.in];
       double2* mz = A.vm[bs.iz];
       if (mom) {
-        for (int t = blockIdx.x; t < A.n_elem; t += nb) {
-          const int f = t * blockDim.x + threadIdx.x;
-          if (f < n) {
-            const double xf = A.x[f], xn = A.xnew[f];
-            A.z[f] = ma * xf + mb * xn;
-            if (acc) A.x[f] = xn;
-          }
+        for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+          const double xf = A.x[f], xn = A.xnew[f];
+          A.z[f] = ma * xf + mb * xn;
+          if (acc) A.x[f] = xn;
         }
-        for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < g.M; s += nb * blockDim.x) {
+        for (int s = lb * blockDim.x + threadIdx.x; s < g.M; s += nb * blockDim.x) {
           const double2 a = mx[s], b = mn[s];
           mz[s] = make_double2(ma * a.x + mb * b.x, ma * a.y + mb * b.y);
         }
@@ -302,18 +616,17 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
         }
         return make_float2((float)(V.x * w - m.x), (float)(V.y * w - m.y));
       };
-      for (int t = blockIdx.x; t < A.nchunks; t += nb)
-        st_grid<float>(t, g, A.pf, resid, A.chunks, A.ent, A.Hg, sm32);
-      if (!grid_sync(A.bar, nb, epoch)) return;
+      grid_chunks(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent, A.Hg, lds);
+      if (!grid_sync(B, epoch)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles
         const int tmp = bs.ix;
         bs.ix = bs.in;
         bs.in = tmp;
       }
       // P2: inverse column pass (clears Hg behind it)
-      for (int t = blockIdx.x; t < A.n_ky; t += nb)
-        st_colfft_inv<float>(t, 1, A.Hg, A.U, NX, NY + 1, g.Lx, NX / 2, 0, A.Px, A.twxf, sm32);
-      if (!grid_sync(A.bar, nb, epoch)) return;
+      for (int t = lb; t < A.n_ky; t += nb)
+        st_colfft_inv<float>(t, 1, A.Hg, A.U, NX, NY + 1, g.Lx, NX / 2, 0, A.Px, twxf, sm32);
+      if (!grid_sync(B, epoch)) return;
     }
     const int trial_first = prologue;
     const double c = bs.c;
@@ -325,7 +638,7 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
     // minus lambda_tsv grad TSV(z)
     auto gstore = [&](int i, int j, float v) {
       const int f = i * NY + j;
-      float gv = v * sc * A.e4xf[i] * A.e4yf[j];
+      float gv = v * sc * e4xf[i] * e4yf[j];
       if (tsv) {
         const int cc = f % W, r = f / W;
         const double zc = A.z[f];
@@ -340,9 +653,9 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
     };
     if (tv) {
       if (trial_first) {
-        for (int t = blockIdx.x; t < A.n_row; t += nb)
-          st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, A.twyf, sm32, gstore);
-        if (!grid_sync(A.bar, nb, epoch)) return;
+        for (int t = lb; t < A.n_row; t += nb)
+          st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+        if (!grid_sync(B, epoch)) return;
       }
       // FGP iterations (c/mfista_TV_lib.c:189-384), b = z + g/c (nonneg: -l1/c + g/c + z)
       const float bshift = (A.nonneg == 1) ? (float)(-A.lambda_l1 / c) : 0.f;
@@ -351,9 +664,7 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
       for (int k = 0; k < A.fgp_iter; k++) {
         const float c1 = (float)A.fgp_c[2 * k], c2 = (float)A.fgp_c[2 * k + 1];
         const int first = (k == 0);
-        for (int t = blockIdx.x; t < A.n_elem; t += nb) {
-          const int f = t * blockDim.x + threadIdx.x;
-          if (f >= n) continue;
+        for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
           const int i = f % W, j = f / W;
           auto xat = [&](int ii, int jj) -> float {
             const int ff = jj * W + ii;
@@ -394,7 +705,7 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
             A.qq[kk] = nq_;
           }
         }
-        if (!grid_sync(A.bar, nb, epoch)) return;
+        if (!grid_sync(B, epoch)) return;
         float* tmp = ri;
         ri = ro;
         ro = tmp;
@@ -405,80 +716,81 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
     }
     // T1: [c2r + gradient on the first trial] -> xnew (prox or final FGP step),
     // d = xnew - z (FP64), partials, FP64 forward row pass of d
-    for (int t = blockIdx.x; t < A.n_row; t += nb) {
-      if (!tv && trial_first)
-        st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, A.twyf, sm32, gstore);
-      double v[kT1W] = {0, 0};
+    {
+      double v[2] = {0, 0};
       const double cinv_d = 1.0 / c, eta_d = A.lambda_l1 / c, lam_d = A.lambda_tv / c;
       const double bshift = (tv && A.nonneg == 1) ? -eta_d : 0.0;
       const bool have_dual = tv && A.fgp_iter > 0;
-      auto load = [&](int i, int j) -> double {
-        const int f = i * NY + j;
-        const double zf = A.z[f];
-        const bool boxed = A.box && A.box[f] == 0.0f;
-        double xn;
-        if (!tv) {
-          xn = prox_p<double>((double)A.gr[f] * cinv_d + zf, eta_d, A.nonneg, boxed);
-        } else {
-          const int ii = f % W, jj = f / W;
-          const double bb = bshift + cinv_d * (double)A.gr[f] + zf;
-          const double L = have_dual ? (double)lpq_f(A.pp, A.qq, ii, jj, W, H) : 0.0;
-          xn = fgp_proj_d(bb - lam_d * L, eta_d, A.nonneg == 1, boxed);
-        }
-        A.xnew[f] = xn;
-        const double d = xn - zf;
-        v[0] += d * d;
-        v[1] += fabs(xn);
-        return d * A.e4xd[i] * A.e4yd[j];
-      };
-      st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, A.twyd, A.T64, sm64, load);
-      block_sum<kT1W>(v, red);
-      if (threadIdx.x == 0)
-        for (int q = 0; q < kT1W; q++) P_t1[kT1W * t + q] = v[q];
-    }
-    if (!grid_sync(A.bar, nb, epoch)) return;
-    // T2: FP64 column pass of d; regulariser partials of xnew and d
-    for (int t = blockIdx.x; t < A.n_ky; t += nb)
-      st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, A.twxd, sm64);
-    for (int t = blockIdx.x; t < A.n_elem; t += nb) {
-      double r[2] = {0, 0};
-      const int f = t * blockDim.x + threadIdx.x;
-      if (f < n) reg_terms(A.xnew, A.z, f, W, H, tv, r);
-      block_sum<2>(r, red);
+      for (int t = lb; t < A.n_row; t += nb) {
+        if (!tv && trial_first)
+          st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+        auto load = [&](int i, int j) -> double {
+          const int f = i * NY + j;
+          const double zf = A.z[f];
+          const bool boxed = A.box && A.box[f] == 0.0f;
+          double xn;
+          if (!tv) {
+            xn = prox_p<double>((double)A.gr[f] * cinv_d + zf, eta_d, A.nonneg, boxed);
+          } else {
+            const int ii = f % W, jj = f / W;
+            const double bb = bshift + cinv_d * (double)A.gr[f] + zf;
+            const double L = have_dual ? (double)lpq_f(A.pp, A.qq, ii, jj, W, H) : 0.0;
+            xn = fgp_proj_d(bb - lam_d * L, eta_d, A.nonneg == 1, boxed);
+          }
+          A.xnew[f] = xn;
+          const double d = xn - zf;
+          v[0] += d * d;
+          v[1] += fabs(xn);
+          return d * e4xd[i] * e4yd[j];
+        };
+        st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, sm64, load);
+      }
+      block_sum<2>(v, red);
       if (threadIdx.x == 0) {
-        P_reg[2 * t] = r[0];
-        P_reg[2 * t + 1] = r[1];
+        prow[kPD2] = v[0];
+        prow[kPL1] = v[1];
       }
     }
-    if (!grid_sync(A.bar, nb, epoch)) return;
+    if (!grid_sync(B, epoch)) return;
+    // T2: FP64 column pass of d; regulariser partials of xnew and d
+    for (int t = lb; t < A.n_ky; t += nb)
+      st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
+    {
+      double r[2] = {0, 0};
+      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x)
+        reg_terms(A.xnew, A.z, f, W, H, tv, r);
+      block_sum<2>(r, red);
+      if (threadIdx.x == 0) {
+        prow[kPReg] = r[0];
+        prow[kPTsvd] = r[1];
+      }
+    }
+    if (!grid_sync(B, epoch)) return;
     // T3: FP64 degrid of d: W A xnew = W A z + W A d, chi^2(xnew), |W A d|^2
     {
       const double2* mz = A.vm[bs.iz];
       double2* mn = A.vm[bs.in];
-      for (int t = blockIdx.x; t < A.n_deg; t += nb) {
-        double v[2];
-        st_degrid_model<G>(t, g, A.pd, A.Gf64, mz, mn, nullptr, v);
-        block_sum<2>(v, red);
-        if (threadIdx.x == 0) {
-          P_chix[t] = v[0];
-          P_lip[t] = v[1];
-        }
+      double v[2] = {0, 0};
+      for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, mz, mn, nullptr, v);
+      block_sum<2>(v, red);
+      if (threadIdx.x == 0) {
+        prow[kPChi] = v[0];
+        prow[kPLip] = v[1];
       }
     }
-    if (!grid_sync(A.bar, nb, epoch)) return;
+    if (!grid_sync(B, epoch)) return;
 
     // ---------------- control (every workgroup, identical arithmetic)
-    const double chi_x = blk_reduce(P_chix, A.n_deg, 1, red);
-    const double lip = blk_reduce(P_lip, A.n_deg, 1, red);
-    const double s_d2 = blk_reduce(P_t1 + 0, A.n_row, kT1W, red);
-    const double l1n = blk_reduce(P_t1 + 1, A.n_row, kT1W, red);
-    const double regn = blk_reduce(P_reg + 0, A.n_elem, 2, red);
-    const double tsvd = blk_reduce(P_reg + 1, A.n_elem, 2, red);
+    const int slot[6] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd};
+    double sum[6];
+    part_reduce<6>(A.part, nb, slot, sum, red);
     if (threadIdx.x == 0) {
+      const double chi_x = sum[0], lip = sum[1], s_d2 = sum[2], l1n = sum[3], regn = sum[4],
+                   tsvd = sum[5];
       const int it = bs.iter;
       if (prologue) {
         ring[it % (kTD + 1)] = bs.costtmp;
-        if (blockIdx.x == 0) A.cost[it] = bs.costtmp;
+        if (lb == 0) A.cost[it] = bs.costtmp;
         bs.prologue = 0;
       }
       double F = chi_x / 2 + ((!tv && tsv) ? A.lambda_tsv * regn : 0.0);
@@ -551,13 +863,9 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
   }
 
   // the accepted image of the last iteration (the reference's x = xnew)
-  if (bs.mom_pending && bs.accept) {
-    for (int t = blockIdx.x; t < A.n_elem; t += nb) {
-      const int f = t * blockDim.x + threadIdx.x;
-      if (f < n) A.x[f] = A.xnew[f];
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bs.mom_pending && bs.accept)
+    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) A.x[f] = A.xnew[f];
+  if (lb == 0 && threadIdx.x == 0) {
     DevState* S = A.S;
     S->c = bs.c;
     S->mu = bs.mu;
@@ -578,12 +886,33 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   const int NX = P.NX, NY = P.NY, n = NX * NY;
   const int n_row = (NX + 1) / 2, n_ky = NY + 1, n_elem = elem_blocks(n);
   const int n_deg = degrid_blocks(N.M, N.MSP);
-  const size_t npart = 2 * (size_t)n_deg + (kT1W + 4) * (size_t)n_row + 2 * (size_t)n_elem;
+  if (N.MSP != 6 && N.MSP != 12) return MFISTA_E_ARG;
+  const void* fn = (N.MSP == 6) ? (const void*)k_mfista_persistent<6>
+                                : (const void*)k_mfista_persistent<12>;
+  // dynamic LDS: [twiddle + E4 tables, when they fit][stage scratch: FP64
+  // row/column FFT or the batched gridder]
+  const int L = g.Lx > g.Ly ? g.Lx : g.Ly;
+  size_t sh = 2 * (size_t)L * sizeof(double2);
+  if (grid_lds_bytes(g.TX, g.TY) > sh) sh = grid_lds_bytes(g.TX, g.TY);
+  int dev = 0, ncu = 0, lds_max = 0;
+  MF_CHECK(hipGetDevice(&dev));
+  MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  MF_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + 64;
+  const size_t tables = (table_lds_bytes(g.Lx, g.Ly, NX, NY) + 255) & ~(size_t)255;
+  const int lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
+  if (lds_tables) sh += tables;
+  int per_cu = 0;
+  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MF_BLOCK, sh));
+  if (per_cu < 1) return MFISTA_E_HIP;
+  const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 1;
+  const int nblk = ncu * (per_cu < want ? per_cu : want);
+  const size_t npart = (size_t)nblk * kPW;
   const size_t rb = (size_t)n * 4;
   if (C.px.ensure(2 * rb) || C.pz.ensure(2 * rb) || C.pxn.ensure(2 * rb) || C.g.ensure(rb) ||
       C.vmod.ensure(3 * (size_t)N.M * sizeof(double2)) ||
       C.part.ensure(npart * 8) || C.state.ensure(sizeof(DevState)) ||
-      C.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || C.bar.ensure(64) ||
+      C.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || C.bar.ensure(kBarBytes) ||
       C.fgpc.ensure((size_t)2 * (P.fgp_iter > 0 ? P.fgp_iter : 1) * 8))
     return MFISTA_E_HIP;
   if (P.tv_mode && C.fgp.ensure(7 * rb)) return MFISTA_E_HIP;
@@ -600,7 +929,7 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
     MF_CHECK(hipMemcpyAsync(C.fgpc.p, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, st));
   }
   MF_CHECK(hipMemcpyAsync(C.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-  MF_CHECK(hipMemsetAsync(C.bar.p, 0, 64, st));
+  MF_CHECK(hipMemsetAsync(C.bar.p, 0, kBarBytes, st));
   PArgs A;
   memset(&A, 0, sizeof(A));
   A.g = g;
@@ -659,20 +988,25 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   A.cost = (double*)C.cost.p;
   A.S = (DevState*)C.state.p;
   A.bar = (unsigned long long*)C.bar.p;
-  // dynamic LDS: the largest stage (FP64 row/column FFT or the gridding tile copies)
-  const int L = g.Lx > g.Ly ? g.Lx : g.Ly;
-  size_t sh = 2 * (size_t)L * sizeof(double2);
-  const size_t shg = 4 * (size_t)g.TX * g.TY * sizeof(float2);
-  if (shg > sh) sh = shg;
-  const void* fn = (N.MSP * 2 <= 16) ? (const void*)k_mfista_persistent<16>
-                                      : (const void*)k_mfista_persistent<32>;
-  int per_cu = 0, ncu = 0, dev = 0;
-  MF_CHECK(hipGetDevice(&dev));
-  MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MF_BLOCK, sh));
-  if (per_cu < 1) return MFISTA_E_HIP;
-  const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 1;
-  const int nblk = ncu * (per_cu < want ? per_cu : want);
+  A.lds_tables = lds_tables;
+  {  // contiguous chunk ranges with balanced entry counts
+    std::vector<int> rg(nblk + 1, 0);
+    const std::vector<int4>& ch = N.h_chunks;
+    long tot = 0;
+    for (const int4& c4 : ch) tot += c4.z - c4.y;
+    long acc = 0;
+    int b = 1;
+    for (int c = 0; c < (int)ch.size() && b < nblk; c++) {
+      acc += ch[c].z - ch[c].y;
+      while (b < nblk && acc * nblk >= tot * (long)b) rg[b++] = c + 1;
+    }
+    for (; b <= nblk; b++) rg[b] = (int)ch.size();
+    if (C.granges.ensure((size_t)(nblk + 1) * 4)) return MFISTA_E_HIP;
+    MF_CHECK(hipMemcpyAsync(C.granges.p, rg.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipStreamSynchronize(st));
+    A.granges = (const int*)C.granges.p;
+  }
+  A.hier_barrier = C.persist_hier_barrier;
   void* args[] = {&A};
   hipEvent_t e0, e1;
   MF_CHECK(hipEventCreate(&e0));
@@ -688,10 +1022,13 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   out->kernel_ms = kms;
   C.persist_blocks = nblk;
   MF_CHECK(hipGetLastError());
-  unsigned long long hb[2];
+  unsigned long long hb[8];
   MF_CHECK(hipMemcpy(hb, C.bar.p, sizeof(hb), hipMemcpyDeviceToHost));
   if (hb[1]) {
-    fprintf(stderr, "[mfista_amd] persistent solver: grid barrier timed out\n");
+    fprintf(stderr,
+            "[mfista_amd] persistent solver: grid barrier timed out (%s barrier; workgroup %llu "
+            "waited on word %llu for %llu, saw %llu, after %llu polls)\n",
+            A.hier_barrier ? "XCD-hierarchical" : "flat", hb[7] & 0xfffff, hb[4], hb[5], hb[6], hb[7] >> 20);
     return MFISTA_E_HIP;
   }
   DevState h;

@@ -1,6 +1,8 @@
 // solver.h -- host-side context and workspaces shared by the solver files.
 #pragma once
 
+#include <vector>
+
 #include <mutex>
 
 #include "../../include/mfista.h"
@@ -27,6 +29,7 @@ struct NufftState {
   DevMem mxy, af, e1f, visf, wf, ad, e1d, visd, wd, perm, yneg;
   DevMem keys_a, keys_b, idx_a, vals_a, ent, tstart, tend, sort_tmp, err, ones, zero_vis, chunks;
   int nchunks = 0;
+  std::vector<int4> h_chunks;  // host copy of the chunk list
   DevMem T, Gf, Hg, U, rw, T64, Gf64, Hg64, U64, rw64;
   VisParams<float> pf() {
     return {(int2*)mxy.p, (float2*)af.p, (float*)e1f.p, (float2*)visf.p, (float*)wf.p};
@@ -51,8 +54,10 @@ struct Context {
   DevMem px, pxn, pz;  // persistent solver: FP64 iterates x, xnew, z
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
   DevMem vmod;  // persistent solver: three FP64 weighted model vectors
+  DevMem granges;  // persistent solver: first gridding chunk of each workgroup (nblk + 1)
   int persist_blocks_per_cu = 1;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
+  int persist_hier_barrier = 0;  // MFISTA_PERSISTENT_FLAT_BARRIER=0 selects the XCD-hierarchical barrier
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

@@ -205,10 +205,14 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   MF_CHECK(hipGetLastError());
   if (herr) return MFISTA_E_FOV;
   // gridding work list: (tile, first entry, end entry) chunks of <= kGridChunk entries
-  std::vector<int4> ch;
+  // (w: bit 0 = first chunk of its tile, bit 1 = last chunk of its tile)
+  std::vector<int4>& ch = N.h_chunks;
+  ch.clear();
   for (int t = 0; t < ntiles; t++)
-    for (int e = hs[t]; e < he[t]; e += kGridChunk)
-      ch.push_back(make_int4(t, e, std::min(he[t], e + kGridChunk), 0));
+    for (int e = hs[t]; e < he[t]; e += kGridChunk) {
+      const int end = std::min(he[t], e + kGridChunk);
+      ch.push_back(make_int4(t, e, end, (e == hs[t] ? 1 : 0) | (end == he[t] ? 2 : 0)));
+    }
   N.nchunks = (int)ch.size();
   if (N.chunks.ensure(ch.size() * sizeof(int4) + 16)) return MFISTA_E_HIP;
   if (!ch.empty())
@@ -543,6 +547,7 @@ Context* get_context() {
       const int v = atoi(e);
       if (v >= 1 && v <= 4) C->persist_blocks_per_cu = v;
     }
+    if (const char* e = getenv("MFISTA_PERSISTENT_FLAT_BARRIER")) C->persist_hier_barrier = atoi(e) ? 0 : 1;
     g_ctx[dev] = C;
   }
   return g_ctx[dev];

@@ -181,32 +181,43 @@ __device__ inline void st_rowfft_c2r(int task, int nseq, const cplx<R>* __restri
   __syncthreads();
 }
 
-// Tap gather of the visibilities of one task (blockDim / G of them), G lanes
-// per visibility; lane l owns tap column kx = mx + l.  Returns true on the
-// group leader of a valid visibility s, with F = (A x)_s in the reflected
-// frame (the NU_SIGN conjugation and 1/(4 NX NY) applied).
-template <typename R, int G>
+// Tap gather of the visibilities of one task, G = 16 (MSP 6) or 32 (MSP 12)
+// lanes per visibility and blockDim / G visibilities per task; lane l owns tap
+// column kx = mx + l.  The tap count 2 MSP is a compile-time constant so the
+// 2 MSP gathers of a lane are issued together.  Returns true on the group
+// leader of a valid visibility s, with F = (A x)_s in the reflected frame
+// (the NU_SIGN conjugation and 1/(4 NX NY) applied).
+template <int MSPC>
+struct DegridShape {
+  static constexpr int S = 2 * MSPC;
+  static constexpr int G = (S <= 16) ? 16 : 32;
+};
+
+template <typename R, int MSPC>
 __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<R>& vp,
                                    const cplx<R>* __restrict__ Gf, int& s, cplx<R>& F) {
   using C = cplx<R>;
+  constexpr int S = DegridShape<MSPC>::S, G = DegridShape<MSPC>::G;
   const int gid = task * blockDim.x + threadIdx.x;
   s = gid / G;
   const int lane = threadIdx.x & (G - 1);
-  const int S = 2 * g.MSP;
   const R t3x = (R)g.t3x, t3y = (R)g.t3y;
   C acc = mk<R>(0, 0);
   if (s < g.M && lane < S) {
     const int2 m = vp.mxy[s];
     const C a = vp.a[s];
-    const R l = (R)(lane - g.MSP + 1);
+    const R l = (R)(lane - MSPC + 1);
     const R gx = dev_exp(l * a.x - l * l * t3x);
-    const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - g.MSP + 1);
+    const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
+    C gv[S];
+#pragma unroll
+    for (int b = 0; b < S; b++) gv[b] = col[(size_t)b * g.Lx];
+#pragma unroll
     for (int b = 0; b < S; b++) {
-      const R lp = (R)(b - g.MSP + 1);
+      const R lp = (R)(b - MSPC + 1);
       const R gy = dev_exp(lp * a.y - lp * lp * t3y);
-      const C gv = col[(size_t)b * g.Lx];
-      acc.x += gy * gv.x;
-      acc.y += gy * gv.y;
+      acc.x += gy * gv[b].x;
+      acc.y += gy * gv[b].y;
     }
     acc.x *= gx;
     acc.y *= gx;
@@ -229,7 +240,7 @@ __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<
 //   mode 1: x is a step d: t = (A d) w, r_new = rz64 - t; v[0] = |r_new|^2,
 //           v[1] = |t|^2
 // v[] holds this thread's contribution (nonzero on the group leaders only).
-template <typename R, int G>
+template <typename R, int MSPC>
 __device__ inline void st_degrid(int task, const GridGeom& g, const VisParams<R>& vp,
                                  const cplx<R>* __restrict__ Gf, cplx<R>* __restrict__ rz,
                                  float2* __restrict__ rz32, const double2* __restrict__ rz64,
@@ -239,7 +250,7 @@ __device__ inline void st_degrid(int task, const GridGeom& g, const VisParams<R>
   using C = cplx<R>;
   int s;
   C F;
-  const bool lead = degrid_core<R, G>(task, g, vp, Gf, s, F);
+  const bool lead = degrid_core<R, MSPC>(task, g, vp, Gf, s, F);
   v[0] = 0.0;
   v[1] = 0.0;
   if (lead) {
@@ -262,10 +273,10 @@ __device__ inline void st_degrid(int task, const GridGeom& g, const VisParams<R>
 }
 
 // FP64 degridding into weighted model vectors m = W A x (reflected frame).
-//   init (mz_in == nullptr): t = W A x; out0 = out1 = t; v[0] = |W V' - t|^2
+//   init (mz_in == nullptr): t = W A x; out0 = out1 = t; v[0] += |W V' - t|^2
 //   step (mz_in != nullptr): t = W A d; out0 = mz + t (the model of z + d);
-//        v[0] = |W V' - mz - t|^2, v[1] = |t|^2
-template <int G>
+//        v[0] += |W V' - mz - t|^2, v[1] += |t|^2
+template <int MSPC>
 __device__ inline void st_degrid_model(int task, const GridGeom& g, const VisParams<double>& vp,
                                        const double2* __restrict__ Gf,
                                        const double2* __restrict__ mz_in,
@@ -273,23 +284,21 @@ __device__ inline void st_degrid_model(int task, const GridGeom& g, const VisPar
                                        double v[2]) {
   int s;
   double2 F;
-  const bool lead = degrid_core<double, G>(task, g, vp, Gf, s, F);
-  v[0] = 0.0;
-  v[1] = 0.0;
+  const bool lead = degrid_core<double, MSPC>(task, g, vp, Gf, s, F);
   if (lead) {
     const double w = vp.w[s];
     const double2 V = vp.vis[s];
     const double tx = F.x * w, ty = F.y * w;
     if (!mz_in) {
       const double rx = V.x * w - tx, ry = V.y * w - ty;
-      v[0] = rx * rx + ry * ry;
+      v[0] += rx * rx + ry * ry;
       out0[s] = make_double2(tx, ty);
       out1[s] = make_double2(tx, ty);
     } else {
       const double2 mz = mz_in[s];
       const double rx = V.x * w - mz.x - tx, ry = V.y * w - mz.y - ty;
-      v[0] = rx * rx + ry * ry;
-      v[1] = tx * tx + ty * ty;
+      v[0] += rx * rx + ry * ry;
+      v[1] += tx * tx + ty * ty;
       out0[s] = make_double2(mz.x + tx, mz.y + ty);
     }
   }
