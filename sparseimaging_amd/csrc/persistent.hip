@@ -300,40 +300,49 @@ __device__ inline float lpq_f(const float* p, const float* q, int i, int j, int 
 }
 
 // Gridding of this workgroup's range [cb, ce) of the tile-ordered chunk list
-// (ranges balanced by entry count on the host).  The entries of up to kGB
-// chunks are loaded at once, one thread per entry, into LDS (cell, tap
-// slopes, deposit value E1 w conj(r)), so the dependent global loads of a
-// whole batch overlap.  Consecutive chunks of one tile accumulate into the
-// same LDS tile (LDS float atomics); the tile is flushed when the tile
-// changes: with plain stores when this workgroup processed all of the tile's
-// chunks (Hg is zero there), otherwise with one global atomic per touched
-// cell.  Deposits as st_grid (stages.h).
+// (ranges balanced by entry count on the host), owner-computes: thread t owns
+// cells t, t + 256, ... of the current tile and gathers the deposits of every
+// entry of the chunk onto them, so there are no LDS or intra-tile atomics and
+// the sums stay in registers across consecutive chunks of one tile.  The
+// entries of up to kGB chunks are loaded at once, one thread per entry, into
+// LDS (cell, tap slopes, deposit value E1 w conj(r)); each is then read by all
+// lanes as a broadcast.  A tile is flushed when it changes: with plain stores
+// when this workgroup processed all of the tile's chunks (Hg is zero there),
+// otherwise with one global atomic per touched cell.  Deposits as st_grid
+// (stages.h): visibility s at taps (l, l') with weight
+// exp(l a_x - l^2 t3x + l' a_y - l'^2 t3y), conjugated at -k for mirror entries.
 constexpr int kGB = MF_BLOCK / kGridChunk;
 
-template <typename Resid>
+template <int CPT, typename Resid>
 __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp, Resid resid,
                                    const int4* __restrict__ chunks, int cb, int ce,
                                    const unsigned* __restrict__ ent, float2* __restrict__ Hg, char* lds) {
-  const int TX = g.TX, TY = g.TY, TT = TX * TY;
-  float2* tile = reinterpret_cast<float2*>(lds);
-  int2* em = reinterpret_cast<int2*>(tile + TT);
+  const int TX = g.TX, TY = g.TY;
+  int2* em = reinterpret_cast<int2*>(lds);
   float2* ea = reinterpret_cast<float2*>(em + MF_BLOCK);
   float2* ed = ea + MF_BLOCK;
   int* eflag = reinterpret_cast<int*>(ed + MF_BLOCK);  // -1 empty, 0 direct, 1 mirror
   int4* cmeta = reinterpret_cast<int4*>(eflag + MF_BLOCK);  // kGB chunk records
   const int MSP = g.MSP, NX = g.NX;
   const float t3x = (float)g.t3x, t3y = (float)g.t3y;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  int cur = -1;        // tile in the LDS accumulator
-  int cur_first = 0;   // its first chunk was processed here
-  auto flush = [&](int tid, bool owned) {
-    const int tr = tid / g.ntx, tc = tid - tr * g.ntx;
-    const int rb = tr * TY, cbx = tc * TX;
-    for (int t = threadIdx.x; t < TT; t += blockDim.x) {
-      const int rr = t / TX, cc = t - rr * TX;
-      const int row = rb + rr, col = cbx + cc;
-      const float2 v = tile[t];
-      if (row <= g.NY && col < g.Lx && (v.x != 0.f || v.y != 0.f)) {
+  // the cells this thread owns: (rr[c], cc[c]) within the tile
+  int rr[CPT], cc[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) {
+    const int cell = threadIdx.x + c * MF_BLOCK;
+    rr[c] = cell / TX;
+    cc[c] = cell - rr[c] * TX;
+  }
+  float2 acc[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) acc[c] = make_float2(0.f, 0.f);
+  int cur = -1, cur_first = 0, rb = 0, cbx = 0;
+  auto flush = [&](bool owned) {
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      const int row = rb + rr[c], col = cbx + cc[c];
+      const float2 v = acc[c];
+      if (rr[c] < TY && row <= g.NY && col < g.Lx && (v.x != 0.f || v.y != 0.f)) {
         float2* dst = Hg + (size_t)row * g.Lx + col;
         if (owned) {
           *dst = v;
@@ -342,6 +351,7 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
           unsafeAtomicAdd(&dst->y, v.y);
         }
       }
+      acc[c] = make_float2(0.f, 0.f);
     }
   };
   for (int c0 = cb; c0 < ce; c0 += kGB) {
@@ -371,74 +381,51 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
     for (int j = 0; j < nc; j++) {
       const int4 ch = cmeta[j];
       if (ch.x != cur) {
-        if (cur >= 0) {
-          __syncthreads();
-          flush(cur, false);
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < TT; t += blockDim.x) tile[t] = make_float2(0.f, 0.f);
-        __syncthreads();
+        if (cur >= 0) flush(false);
         cur = ch.x;
         cur_first = ch.w & 1;
+        const int tr = cur / g.ntx, tc = cur - tr * g.ntx;
+        rb = tr * TY;
+        cbx = tc * TX;
       }
-      const int tr = ch.x / g.ntx, tc = ch.x - tr * g.ntx;
-      const int rb = tr * TY, cbx = tc * TX;
-      for (int q = j * kGridChunk + wave; q < (j + 1) * kGridChunk; q += MF_BLOCK / 64) {
+      for (int q = j * kGridChunk; q < (j + 1) * kGridChunk; q++) {
         const int flag = eflag[q];
-        if (flag < 0) continue;
+        if (flag < 0) break;  // a chunk's entries are contiguous from its start
         const int2 m = em[q];
         const float2 a = ea[q];
         const float2 d = ed[q];
-        int lp0, lp1, l0, l1;
-        if (!flag) {
-          lp0 = max(-MSP + 1, rb - m.y);
-          lp1 = min(MSP + 1, rb + TY - m.y);
-          l0 = max(-MSP + 1, cbx - m.x - NX);
-          l1 = min(MSP + 1, cbx + TX - m.x - NX);
-        } else {
-          lp0 = max(-MSP + 1, -m.y - rb - TY + 1);
-          lp1 = min(MSP + 1, -m.y - rb + 1);
-          l0 = max(-MSP + 1, NX - m.x - cbx - TX + 1);
-          l1 = min(MSP + 1, NX - m.x - cbx + 1);
-        }
-        const int nl = l1 - l0, nlp = lp1 - lp0;
-        if (nl <= 0 || nlp <= 0) continue;
-        const int ntap = nl * nlp;
-        for (int t = lane; t < ntap; t += 64) {
-          const int ia = t / nl, ib = t - ia * nl;
-          const int lp = lp0 + ia, l = l0 + ib;
-          const float fl = (float)l, flp = (float)lp;
-          const float wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
-          int row, col;
-          if (!flag) {
-            row = m.y + lp - rb;
-            col = m.x + NX + l - cbx;
-          } else {
-            row = -(m.y + lp) - rb;
-            col = NX - m.x - l - cbx;
+        // tap offsets of cell (row, col): direct l' = row - my, l = col - mx - NX;
+        // mirror l' = -row - my, l = NX - mx - col
+        const int oy = flag ? -(rb + m.y) : rb - m.y;
+        const int ox = flag ? NX - m.x - cbx : cbx - m.x - NX;
+        const int sg = flag ? -1 : 1;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+          const int lp = oy + sg * rr[c];
+          const int l = ox + sg * cc[c];
+          if (lp > -MSP && lp <= MSP && l > -MSP && l <= MSP) {
+            const float fl = (float)l, flp = (float)lp;
+            const float wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
+            acc[c].x += wgt * d.x;
+            acc[c].y += wgt * d.y;
           }
-          float* cell = reinterpret_cast<float*>(tile + row * TX + col);
-          atomicAdd(cell, wgt * d.x);
-          atomicAdd(cell + 1, wgt * d.y);
         }
       }
-      if (ch.w & 2) {  // last chunk of the tile: flush now
-        __syncthreads();
-        flush(cur, cur_first != 0);
+      if (ch.w & 2) {  // last chunk of the tile
+        flush(cur_first != 0);
         cur = -1;
       }
     }
   }
-  if (cur >= 0) {
-    __syncthreads();
-    flush(cur, false);
-  }
+  if (cur >= 0) flush(false);
   __syncthreads();
 }
 
-// LDS bytes of the batched gridder's tile and entry arrays
+// LDS bytes of the batched gridder's entry arrays
 __host__ __device__ inline size_t grid_lds_bytes(int TX, int TY) {
-  return (size_t)TX * TY * 8 + (size_t)MF_BLOCK * (8 + 8 + 8 + 4) + (size_t)kGB * 16;
+  (void)TX;
+  (void)TY;
+  return (size_t)MF_BLOCK * (8 + 8 + 8 + 4) + (size_t)kGB * 16;
 }
 
 // LDS bytes of the staged twiddle (FP32 and FP64, both axes) and E4 tables
@@ -616,7 +603,8 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
         }
         return make_float2((float)(V.x * w - m.x), (float)(V.y * w - m.y));
       };
-      grid_chunks(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent, A.Hg, lds);
+      grid_chunks<(MSPC <= 8) ? 2 : 4>(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent,
+                                         A.Hg, lds);
       if (!grid_sync(B, epoch)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles
         const int tmp = bs.ix;
@@ -905,7 +893,7 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   int per_cu = 0;
   MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MF_BLOCK, sh));
   if (per_cu < 1) return MFISTA_E_HIP;
-  const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 1;
+  const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 2;
   const int nblk = ncu * (per_cu < want ? per_cu : want);
   const size_t npart = (size_t)nblk * kPW;
   const size_t rb = (size_t)n * 4;

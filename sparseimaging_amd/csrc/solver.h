@@ -55,9 +55,9 @@ struct Context {
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
   DevMem vmod;  // persistent solver: three FP64 weighted model vectors
   DevMem granges;  // persistent solver: first gridding chunk of each workgroup (nblk + 1)
-  int persist_blocks_per_cu = 1;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
+  int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
-  int persist_hier_barrier = 0;  // MFISTA_PERSISTENT_FLAT_BARRIER=0 selects the XCD-hierarchical barrier
+  int persist_hier_barrier = 1;  // MFISTA_PERSISTENT_FLAT_BARRIER=1 selects the flat barrier
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };
