@@ -30,6 +30,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM traffic of one launch of the bench workload, measured with rocprofv3
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a 256 MiB copy and
+# fill; profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md), for a
+# solve of 579 iterations and 1166 trials.
+MEASURED_TRAFFIC = {"bytes": 16854340953, "iterations": 579, "trials": 1166}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -62,28 +68,87 @@ def stage_bytes(N, M, MSP, n_entries):
 
 
 def persistent_bytes(N, M, MSP, n_entries, iters, trials):
-    """Algorithmic bytes of one persistent-engine solve (one launch).
+    """Algorithmic (compulsory) bytes of one persistent-engine solve (one launch).
 
-    Per iteration (prologue at the momentum point z, FP64 forward):
-      momentum + FP64 row pass   read x, xnew, write z (12 B/px), write T64 16 B/cell
-      FP64 column pass           read T64 16 B/cell, write Gf64 16 B/cell
-      FP64 degrid                read Gf64 16 B/cell, 56 B/vis params, write rz64 16 + rz32 8 B/vis
-      FP32 grid                  read rz32 8 + 28 B/vis params + 4 B/entry, write Hg 8 B/cell
-      FP32 inverse column pass   read Hg, write U, clear Hg (8 B/cell each)
-      c2r + TSV gradient         read U 8 B/cell, z 4 B/px, write g 4 B/px
+    P = N^2 pixels, T = N(N+1) row-pass cells, H = 2N(N+1) half-plane grid
+    cells, Hf = 2N(N+MSP) forward-spectrum cells; iterates and models FP64,
+    adjoint grid FP32.
+
+    Per iteration (prologue and the first trial's c2r):
+      P1 momentum + grid   read x, xnew, write z, x (32 B/px); read W A x, W A xnew,
+                           write W A z (48 B/vis); visibility parameters and FP64
+                           weighted data (48 B/vis); tile-list entries (4 B each);
+                           write Hg (8 B/cell)
+      P2 inverse columns   read and clear Hg (16 B/cell), write U (8 B/cell)
+      c2r (first trial)    read U (8 B/cell)
     Per trial:
-      prox + FP32 row pass       read g, z, write xnew (12 B/px), write T 8 B/cell
-      FP32 column pass           read T, write Gf (8 B/cell each)
-      FP32 degrid (mode 1)       read Gf 8 B/cell, 32 B/vis params, rz64 16 B/vis
+      T1 prox + FP64 rows  read g (4 B/px), z (8 B/px), write xnew (8 B/px), T64 (16 B/cell)
+      T2 FP64 columns      read T64 (16 B/cell), write Gf64 (16 B/cell); read xnew, z (16 B/px)
+      T3 FP64 degrid       read Gf64 (16 B/cell), parameters (56 B/vis), W A z (16 B/vis),
+                           write W A xnew (16 B/vis)
     """
     P = N * N
     T = N * (N + 1)
     H = 2 * N * (N + 1)
     Hf = 2 * N * (N + MSP)
-    per_iter = ((12 * P + 16 * T) + (16 * T + 16 * Hf) + (16 * Hf + 56 * M + 24 * M)
-                + (36 * M + 4 * n_entries + 8 * H) + (8 * H + 8 * T + 8 * H) + (8 * T + 8 * P))
-    per_trial = (12 * P + 8 * T) + (8 * T + 8 * Hf) + (8 * Hf + 48 * M)
+    per_iter = (32 * P + 96 * M + 4 * n_entries + 8 * H) + (16 * H + 8 * T) + 8 * T
+    per_trial = (20 * P + 16 * T) + (16 * T + 16 * Hf + 16 * P) + (16 * Hf + 88 * M)
     return per_iter * iters + per_trial * trials, per_iter, per_trial
+
+
+def _batch_vs_single(sa, probs, preset, rel_tol):
+    """Solves `probs` once as one batch and once one by one (persistent
+    engine); returns solves/s of both and the worst objective difference."""
+    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol)  # warm-up
+    t0 = time.perf_counter()
+    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol)
+    tb = time.perf_counter() - t0
+    keys = ("maxiter", "eps", "lambda_l1", "lambda_tv", "lambda_tsv", "cinit", "nonneg_flag")
+    t0 = time.perf_counter()
+    worst = 0.0
+    iters = 0
+    for p, rb in zip(probs, rs):
+        _, r1, _ = sa.mfista_imaging_core_nufft(p["u_dx"], p["v_dy"], p["vis_r"], p["vis_i"], p["vis_std"],
+                                                p["NX"], p["NY"], preset=preset, rel_tol=rel_tol,
+                                                **{k: p[k] for k in keys if k in p})
+        worst = max(worst, abs(rb["finalcost"] - r1["finalcost"]) / abs(r1["finalcost"]))
+        iters += rb["ITER"]
+    ts = time.perf_counter() - t0
+    n = len(probs)
+    return {"solves": n, "batch_solves_per_s": round(n / tb, 3), "single_solves_per_s": round(n / ts, 3),
+            "speedup": round(ts / tb, 2), "batch_ms": round(tb * 1e3, 2), "kernel_ms": round(info["kernel_ms"], 2),
+            "iterations": iters, "max_objective_rel_diff": worst}
+
+
+def bench_lambda_grid(sa, synth):
+    """Batched workload (a): 6 x 6 grid of log10 lambda_l1 in [-2, 3] and
+    log10 lambda_tsv in [0, 5] over one 128^2 visibility set (M = 8000 arcs,
+    seed 1004), L1+TSV, 12-tap preset, 1e-6 relative tolerance."""
+    import numpy as np
+
+    P = synth.make_nufft_problem(N=128, M=8000, seed=1004)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=128, NY=128,
+                maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0)
+    probs = [dict(base, lambda_l1=10.0 ** a, lambda_tsv=10.0 ** b)
+             for a in np.linspace(-2, 3, 6) for b in np.linspace(0, 5, 6)]
+    out = _batch_vs_single(sa, probs, sa.PRESET_CXX12, 1e-6)
+    out["workload"] = "lambda grid 6x6, 128^2, M=8000, 12 taps"
+    log("bench: lambda grid " + json.dumps(out))
+    return out
+
+
+def bench_movie(sa, synth):
+    """Batched workload (b): 32 frames of a drifting ring, 128^2, M per frame
+    uniform in 5000..40000 (seed 1004), lambda_l1 = 1, lambda_tsv = 100,
+    12-tap preset, 1e-6 relative tolerance."""
+    frames = synth.make_movie_frames(nframes=32, N=128, seed=1004)
+    probs = [dict(u_dx=f["u"], v_dy=f["v"], vis_r=f["vis_r"], vis_i=f["vis_i"], vis_std=f["std"], NX=128,
+                  NY=128, maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_l1=1.0, lambda_tsv=100.0,
+                  lambda_tv=0.0) for f in frames]
+    out = _batch_vs_single(sa, probs, sa.PRESET_CXX12, 1e-6)
+    out["workload"] = "movie 32 frames, 128^2, M in [5000, 40000], 12 taps"
+    log("bench: movie " + json.dumps(out))
+    return out
 
 
 def main():
@@ -97,6 +162,9 @@ def main():
     ap.add_argument("--engine", choices=["persistent", "stream"], default="persistent")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
+                    help="also time the batched workloads (6x6 lambda grid; 32-frame movie) "
+                         "against the same problems solved one by one")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -164,7 +232,8 @@ def main():
         ach = tot / (kms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": "k_mfista_persistent", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": None, "bytes_per_launch": tot, "bytes_per_iteration": per_it,
+                    "traffic": MEASURED_TRAFFIC["bytes"], "traffic_solve": MEASURED_TRAFFIC,
+                    "bytes_per_launch": tot, "bytes_per_iteration": per_it,
                     "bytes_per_trial": per_tr, "launch_ms": round(kms, 3)}
     else:
         share = {k: prof[k] * launches[k] for k in prof}
@@ -174,6 +243,12 @@ def main():
                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                     "bytes_per_launch": nbytes[dom], "launch_ms": round(prof[dom], 5)}
     log("bench: roofline " + json.dumps(roofline))
+
+    batch = {}
+    if args.batch in ("lambda", "all"):
+        batch["lambda_grid"] = bench_lambda_grid(sa, synth)
+    if args.batch in ("movie", "all"):
+        batch["movie"] = bench_movie(sa, synth)
 
     cpu = None
     if args.cpu_iters > 0:
@@ -212,6 +287,7 @@ def main():
         "iterations_per_s": round(ITER / (ms_per_step / 1e3), 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batch or None,
         "kernels": kern,
     }
     print(json.dumps(out), flush=True)

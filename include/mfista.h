@@ -145,6 +145,32 @@ int mfista_nufft_solve_device(const double *d_u, const double *d_v, const double
                               int box_flag, const float *d_cl_box,
                               struct RESULT *mfista_result, mfista_opts *opts, void *stream);
 
+/* One problem of a batch: the arguments of mfista_imaging_core_nufft (host
+ * pointers).  No counterpart in the reference, which solves one problem per
+ * call; a batch is a sequence of c/mfista_nufft_lib.c:909 calls. */
+typedef struct {
+  const double *u_dx, *v_dy, *vis_r, *vis_i, *vis_std;
+  int M, NX, NY, maxiter;
+  double eps, lambda_l1, lambda_tv, lambda_tsv, cinit;
+  const double *xinit;
+  double *xout;
+  int nonneg_flag, box_flag;
+  const float *cl_box;
+} mfista_nufft_problem;
+
+/* Solves nprob independent NUFFT problems in one persistent launch (more
+ * launches if there are more problems than workgroups); results[i] receives
+ * problem i's RESULT exactly as mfista_imaging_core_nufft_ex would.  Problems
+ * whose (u_dx, v_dy, vis_r, vis_i, vis_std, M, NX, NY) are the same pointers
+ * and sizes share one NUFFT set-up (a lambda grid over one visibility set).
+ * All problems use opts->preset and opts->fgp_iter, opts->rel_tol.  Returns 0
+ * or the status of the first invalid problem (then nothing is solved); a
+ * problem with nonneg_flag outside {0, 1} gets ITER = 0, as in the reference
+ * (c/mfista_nufft_lib.c:634-637).  opts->solve_ms and opts->kernel_ms report
+ * the whole batch. */
+int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem *probs,
+                                    struct RESULT *results, mfista_opts *opts);
+
 /* same arguments as mfista_imaging_core_fft (without fftw_plan_flag) plus options */
 int mfista_imaging_core_fft_ex(const int *u_idx, const int *v_idx, const double *y_r,
                                const double *y_i, const double *noise_stdev, int M, int NX, int NY,

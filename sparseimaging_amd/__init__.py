@@ -39,9 +39,19 @@ EXPORTED = [
     "x2y_nufft", "y2x_nufft", "mfista_default_opts", "mfista_imaging_core_nufft_ex",
     "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
     "mfista_amd_version", "mfista_profile_stages", "mfista_last_steps",
-    "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex",
+    "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
+
+
+class NufftProblem(C.Structure):
+    """mfista_nufft_problem (include/mfista.h): one problem of a batch, host pointers."""
+
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("u_dx", "v_dy", "vis_r", "vis_i", "vis_std")] + [
+        (n, C.c_int) for n in ("M", "NX", "NY", "maxiter")] + [
+        (n, C.c_double) for n in ("eps", "lambda_l1", "lambda_tv", "lambda_tsv", "cinit")] + [
+        ("xinit", C.POINTER(C.c_double)), ("xout", C.POINTER(C.c_double)), ("nonneg_flag", C.c_int),
+        ("box_flag", C.c_int), ("cl_box", C.POINTER(C.c_float))]
 
 
 class RESULT(C.Structure):
@@ -110,6 +120,8 @@ def _declare(L):
     L.mfista_imaging_core_nufft_ex.restype = i
     L.mfista_imaging_core_nufft_ex.argtypes = [_dp, _dp, _dp, _dp, _dp, i, i, i, i, d, d, d, d, d, _dp,
                                                _dp, i, i, _fp, C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_imaging_core_nufft_batch.restype = i
+    L.mfista_imaging_core_nufft_batch.argtypes = [i, C.POINTER(NufftProblem), C.POINTER(RESULT), C.POINTER(Opts)]
     L.mfista_nufft_solve_device.restype = i
     L.mfista_nufft_solve_device.argtypes = [C.c_void_p] * 5 + [i, i, i, i, d, d, d, d, d] + [
         C.c_void_p, C.c_void_p, i, i, C.c_void_p, C.POINTER(RESULT), C.POINTER(Opts), C.c_void_p]
@@ -187,6 +199,61 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     if rc != 0:
         raise MfistaError(rc)
     return xout.reshape(NX, NY), res.as_dict(), _info(o)
+
+
+def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0):
+    """Independent NUFFT solves in one persistent launch (no reference counterpart:
+    a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
+    the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
+    vis_std, NX, NY and optionally maxiter, eps, lambda_l1, lambda_tv, lambda_tsv,
+    cinit, xinit, nonneg_flag, cl_box).  Problems given the same numpy arrays for
+    u_dx, v_dy, vis_r, vis_i, vis_std share one NUFFT set-up.  Returns
+    (list of x[NX,NY], list of RESULT dicts, opts dict for the whole batch)."""
+    n = len(problems)
+    arr = (NufftProblem * n)()
+    keep = []
+    cache = {}
+
+    def dptr(a):
+        key = id(a)
+        if key not in cache:
+            b = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            cache[key] = b
+            keep.append(a)  # keeps id(a) unique while cached
+        return cache[key]
+
+    outs = []
+    for k, p in enumerate(problems):
+        NX, NY = int(p["NX"]), int(p["NY"])
+        q = arr[k]
+        arrays = [dptr(p[key]) for key in ("u_dx", "v_dy", "vis_r", "vis_i", "vis_std")]
+        q.u_dx, q.v_dy, q.vis_r, q.vis_i, q.vis_std = [a.ctypes.data_as(_dp) for a in arrays]
+        q.M = arrays[0].size
+        q.NX, q.NY = NX, NY
+        q.maxiter = int(p.get("maxiter", MAXITER))
+        q.eps = float(p.get("eps", EPS))
+        q.lambda_l1 = float(p.get("lambda_l1", 1.0))
+        q.lambda_tv = float(p.get("lambda_tv", 0.0))
+        q.lambda_tsv = float(p.get("lambda_tsv", 0.0))
+        q.cinit = float(p.get("cinit", 1e4))
+        x0 = p.get("xinit")
+        x0 = np.zeros(NX * NY) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).ravel()
+        xo = np.zeros(NX * NY)
+        keep += [x0, xo]
+        outs.append(xo)
+        q.xinit = x0.ctypes.data_as(_dp)
+        q.xout = xo.ctypes.data_as(_dp)
+        q.nonneg_flag = int(p.get("nonneg_flag", 1))
+        bflag, b, pb = _box(p.get("cl_box"), NX * NY)
+        keep.append(b)
+        q.box_flag, q.cl_box = bflag, pb
+    res = (RESULT * n)()
+    o = make_opts(preset, fgp_iter, rel_tol)
+    rc = lib().mfista_imaging_core_nufft_batch(n, arr, res, C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    xs = [xo.reshape(int(p["NX"]), int(p["NY"])) for xo, p in zip(outs, problems)]
+    return xs, [r.as_dict() for r in res], _info(o)
 
 
 def mfista_imaging_core_fft(u_idx, v_idx, y_r, y_i, noise_stdev, NX, NY, maxiter=MAXITER, eps=EPS,

@@ -42,6 +42,8 @@ int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
 
 // the whole NUFFT solve in one cooperative launch (persistent.hip)
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
+int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                      std::vector<RunStats>& out, float* kernel_ms);
 
 // FGP TV prox kernels (reference c/mfista_TV_lib.c:189-384)
 template <typename R>

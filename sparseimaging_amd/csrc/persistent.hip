@@ -20,6 +20,7 @@
 // row pass of d | FP64 column pass | FP64 degrid, then 3 stages per further
 // trial: 5 + 3(b-1) grid barriers.  L1+TV keeps the c2r stage separate (the
 // FGP iterations need the whole gradient) and adds one barrier per FGP step.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -55,6 +56,7 @@ struct PArgs {
   double* part;  // kPW doubles per workgroup (see kP* below)
   int n_deg, n_row, n_ky, n_elem;
   int lds_tables;  // twiddle and E4 tables staged in LDS
+  int nblk;        // workgroups solving this problem
   double* cost;
   DevState* S;
   unsigned long long* bar;  // barrier words, see BarCtx
@@ -166,12 +168,12 @@ __device__ inline bool grid_sync(const BarCtx& B, unsigned long long& epoch) {
 // Set-up: every workgroup learns its XCD (hardware register XCC_ID), the XCD
 // member counts and its leader; one flat barrier publishes the counts.  The
 // hierarchical barrier is used only if every XCD id is in range.
-__device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, int want_hier) {
+__device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, unsigned nblk, int want_hier) {
   if (threadIdx.x == 0) {
     unsigned v;
     asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
     B.bar = bar;
-    B.nblk = gridDim.x;
+    B.nblk = nblk;
     B.xcc = (int)(v & 0xf);
     B.leader = 0;
     B.hier = 0;
@@ -186,7 +188,7 @@ __device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, int want_hi
   __syncthreads();
   __shared__ int fail;
   if (threadIdx.x == 0) {
-    fail = flat_sync(bar, 2, gridDim.x, 1) ? 0 : 1;
+    fail = flat_sync(bar, 2, nblk, 1) ? 0 : 1;
     if (!fail) {
       unsigned ng = 0;
       for (int x = 0; x < 8; x++)
@@ -433,23 +435,23 @@ __host__ __device__ inline size_t table_lds_bytes(int Lx, int Ly, int NX, int NY
   return (size_t)(Lx + Ly) * (16 + 8) + (size_t)(NX + NY) * (8 + 4);
 }
 
+// The solve of one problem by workgroups lb = 0 .. nb-1 of a cooperative
+// launch (all of the launch for a single solve, a share of it in a batch).
 template <int MSPC>
-__global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
+__device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const unsigned nb) {
   extern __shared__ double4 smem_d4[];
   __shared__ double red[16 * kPW];
   __shared__ BlockState bs;
   __shared__ double ring[kTD + 1];
   __shared__ BarCtx B;
   const GridGeom& g = A.g;
-  const unsigned nb = gridDim.x;
-  const int lb = blockIdx.x;
   const int NX = g.NX, NY = g.NY, n = NX * NY, W = NX, H = NY;
   const bool tsv = A.lambda_tsv > 0.0;
   const bool tv = A.tv_mode != 0;
   unsigned long long epoch = 0;  // barriers passed by this workgroup
   double* prow = A.part + (size_t)lb * kPW;
 
-  if (!bar_setup(B, A.bar, A.hier_barrier)) return;
+  if (!bar_setup(B, A.bar, nb, A.hier_barrier)) return;
 
   // LDS: [tables][stage scratch]
   char* lds = reinterpret_cast<char*>(smem_d4);
@@ -867,44 +869,88 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
   }
 }
 
-int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
-  NufftState& N = C.nu;
-  hipStream_t st = C.stream;
-  const GridGeom& g = N.geom;
-  const int NX = P.NX, NY = P.NY, n = NX * NY;
-  const int n_row = (NX + 1) / 2, n_ky = NY + 1, n_elem = elem_blocks(n);
-  const int n_deg = degrid_blocks(N.M, N.MSP);
-  if (N.MSP != 6 && N.MSP != 12) return MFISTA_E_ARG;
-  const void* fn = (N.MSP == 6) ? (const void*)k_mfista_persistent<6>
-                                : (const void*)k_mfista_persistent<12>;
+template <int MSPC>
+__global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
+  mfista_body<MSPC>(A, blockIdx.x, gridDim.x);
+}
+
+// A batch of independent solves in one cooperative launch: blk[b] =
+// (problem, workgroup index within the problem) for workgroup b.
+template <int MSPC>
+__global__ void __launch_bounds__(MF_BLOCK) k_mfista_batch(const PArgs* __restrict__ args,
+                                                           const int2* __restrict__ blk) {
+  const int2 pb = blk[blockIdx.x];
+  if (pb.x < 0) return;  // a workgroup slot no problem was dealt
+  const PArgs& A = args[pb.x];
+  mfista_body<MSPC>(A, pb.y, (unsigned)A.nblk);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+// Launch configuration shared by the problems of one launch.
+struct PLaunch {
+  const void* fn = nullptr;
+  size_t sh = 0;       // dynamic LDS bytes
+  int lds_tables = 0;  // tables staged in LDS
+  int nblk = 0;        // workgroups of the launch
+};
+
+static int persistent_config(Context& C, int MSP, const std::vector<const NufftState*>& sets, bool batch,
+                             PLaunch* L) {
+  if (MSP != 6 && MSP != 12) return MFISTA_E_ARG;
+  if (batch)
+    L->fn = (MSP == 6) ? (const void*)k_mfista_batch<6> : (const void*)k_mfista_batch<12>;
+  else
+    L->fn = (MSP == 6) ? (const void*)k_mfista_persistent<6> : (const void*)k_mfista_persistent<12>;
   // dynamic LDS: [twiddle + E4 tables, when they fit][stage scratch: FP64
-  // row/column FFT or the batched gridder]
-  const int L = g.Lx > g.Ly ? g.Lx : g.Ly;
-  size_t sh = 2 * (size_t)L * sizeof(double2);
-  if (grid_lds_bytes(g.TX, g.TY) > sh) sh = grid_lds_bytes(g.TX, g.TY);
+  // row/column FFT or the batched gridder's entries]
+  size_t sh = 0, tables = 0;
+  for (const NufftState* N : sets) {
+    const GridGeom& g = N->geom;
+    const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
+    size_t s1 = 2 * (size_t)Lm * sizeof(double2);
+    if (grid_lds_bytes(g.TX, g.TY) > s1) s1 = grid_lds_bytes(g.TX, g.TY);
+    if (s1 > sh) sh = s1;
+    const size_t t1 = (table_lds_bytes(g.Lx, g.Ly, g.NX, g.NY) + 255) & ~(size_t)255;
+    if (t1 > tables) tables = t1;
+  }
   int dev = 0, ncu = 0, lds_max = 0;
   MF_CHECK(hipGetDevice(&dev));
   MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   MF_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + 64;
-  const size_t tables = (table_lds_bytes(g.Lx, g.Ly, NX, NY) + 255) & ~(size_t)255;
-  const int lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
-  if (lds_tables) sh += tables;
+  const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + sizeof(BarCtx) + 64;
+  L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
+  if (L->lds_tables) sh += tables;
+  L->sh = sh;
   int per_cu = 0;
-  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, MF_BLOCK, sh));
+  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L->fn, MF_BLOCK, sh));
   if (per_cu < 1) return MFISTA_E_HIP;
   const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 2;
-  const int nblk = ncu * (per_cu < want ? per_cu : want);
-  const size_t npart = (size_t)nblk * kPW;
+  L->nblk = ncu * (per_cu < want ? per_cu : want);
+  return 0;
+}
+
+// Allocates and initialises one problem's workspace and fills its PArgs for
+// nblk workgroups.  P.xinit and P.xout are device pointers.
+static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P, int nblk, int lds_tables,
+                              int hier, hipStream_t st, PArgs* out) {
+  const GridGeom& g = N.geom;
+  const int NX = P.NX, NY = P.NY, n = NX * NY;
+  const int n_row = (NX + 1) / 2, n_ky = NY + 1, n_elem = elem_blocks(n);
+  const int n_deg = degrid_blocks(N.M, N.MSP);
   const size_t rb = (size_t)n * 4;
-  if (C.px.ensure(2 * rb) || C.pz.ensure(2 * rb) || C.pxn.ensure(2 * rb) || C.g.ensure(rb) ||
-      C.vmod.ensure(3 * (size_t)N.M * sizeof(double2)) ||
-      C.part.ensure(npart * 8) || C.state.ensure(sizeof(DevState)) ||
-      C.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || C.bar.ensure(kBarBytes) ||
-      C.fgpc.ensure((size_t)2 * (P.fgp_iter > 0 ? P.fgp_iter : 1) * 8))
+  const size_t nky = (size_t)NY + 1;
+  if (W.px.ensure(2 * rb) || W.pz.ensure(2 * rb) || W.pxn.ensure(2 * rb) || W.g.ensure(rb) ||
+      W.vmod.ensure(3 * (size_t)N.M * sizeof(double2)) || W.T64.ensure(nky * NX * sizeof(double2)) ||
+      W.Gf64.ensure(((size_t)NY + N.MSP) * g.Lx * sizeof(double2)) ||
+      W.Hg.ensure(nky * g.Lx * sizeof(float2)) || W.U.ensure((size_t)NX * nky * sizeof(float2)) ||
+      W.part.ensure((size_t)nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
+      W.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || W.bar.ensure(kBarBytes) ||
+      W.fgpc.ensure((size_t)2 * (P.fgp_iter > 0 ? P.fgp_iter : 1) * 8) ||
+      W.granges.ensure((size_t)(nblk + 1) * 4))
     return MFISTA_E_HIP;
-  if (P.tv_mode && C.fgp.ensure(7 * rb)) return MFISTA_E_HIP;
-  if (nufft_ensure64(C)) return MFISTA_E_HIP;
+  if (P.tv_mode && W.fgp.ensure(7 * rb)) return MFISTA_E_HIP;
   {
     std::vector<double> cc(2 * (size_t)(P.fgp_iter > 0 ? P.fgp_iter : 1), 0.0);
     double t = 1;
@@ -914,11 +960,27 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
       cc[2 * k + 1] = -(t - 1) / tn;
       t = tn;
     }
-    MF_CHECK(hipMemcpyAsync(C.fgpc.p, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemcpyAsync(W.fgpc.p, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, st));
   }
-  MF_CHECK(hipMemcpyAsync(C.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
-  MF_CHECK(hipMemsetAsync(C.bar.p, 0, kBarBytes, st));
-  PArgs A;
+  {  // contiguous gridding chunk ranges with balanced entry counts
+    std::vector<int> rg(nblk + 1, 0);
+    const std::vector<int4>& ch = N.h_chunks;
+    long tot = 0;
+    for (const int4& c4 : ch) tot += c4.z - c4.y;
+    long acc = 0;
+    int b = 1;
+    for (int c = 0; c < (int)ch.size() && b < nblk; c++) {
+      acc += ch[c].z - ch[c].y;
+      while (b < nblk && acc * nblk >= tot * (long)b) rg[b++] = c + 1;
+    }
+    for (; b <= nblk; b++) rg[b] = (int)ch.size();
+    MF_CHECK(hipMemcpy(W.granges.p, rg.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice));
+  }
+  if (P.xinit != W.px.p)
+    MF_CHECK(hipMemcpyAsync(W.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  MF_CHECK(hipMemsetAsync(W.bar.p, 0, kBarBytes, st));
+  MF_CHECK(hipMemsetAsync(W.Hg.p, 0, nky * g.Lx * sizeof(float2), st));  // the gridder adds into zeros
+  PArgs& A = *out;
   memset(&A, 0, sizeof(A));
   A.g = g;
   A.Px = N.Px;
@@ -933,21 +995,22 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   A.e4yf = (const float*)N.e4yf.p;
   A.e4xd = (const double*)N.e4xd.p;
   A.e4yd = (const double*)N.e4yd.p;
-  A.Hg = (float2*)N.Hg.p;
-  A.U = (float2*)N.U.p;
-  A.T64 = (double2*)N.T64.p;
-  A.Gf64 = (double2*)N.Gf64.p;
-  for (int q = 0; q < 3; q++) A.vm[q] = (double2*)C.vmod.p + (size_t)q * N.M;
+  A.Hg = (float2*)W.Hg.p;
+  A.U = (float2*)W.U.p;
+  A.T64 = (double2*)W.T64.p;
+  A.Gf64 = (double2*)W.Gf64.p;
+  for (int q = 0; q < 3; q++) A.vm[q] = (double2*)W.vmod.p + (size_t)q * N.M;
   A.chunks = (const int4*)N.chunks.p;
   A.nchunks = N.nchunks;
+  A.granges = (const int*)W.granges.p;
   A.ent = (const unsigned*)N.ent.p;
-  A.x = (double*)C.px.p;
-  A.z = (double*)C.pz.p;
-  A.xnew = (double*)C.pxn.p;
-  A.gr = (float*)C.g.p;
+  A.x = (double*)W.px.p;
+  A.z = (double*)W.pz.p;
+  A.xnew = (double*)W.pxn.p;
+  A.gr = (float*)W.g.p;
   A.box = P.box_flag ? P.box : nullptr;
   if (P.tv_mode) {
-    float* fb = (float*)C.fgp.p;
+    float* fb = (float*)W.fgp.p;
     A.rA = fb;
     A.sA = fb + n;
     A.rB = fb + 2 * (size_t)n;
@@ -955,7 +1018,7 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
     A.pp = fb + 4 * (size_t)n;
     A.qq = fb + 5 * (size_t)n;
   }
-  A.fgp_c = (const double*)C.fgpc.p;
+  A.fgp_c = (const double*)W.fgpc.p;
   A.fgp_iter = P.tv_mode ? P.fgp_iter : 0;
   A.tv_mode = P.tv_mode;
   A.maxiter = P.maxiter;
@@ -968,67 +1031,193 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   A.eta_first = P.eta_first;
   A.cinit = P.cinit;
   A.adj_scale = 1.0 / (8.0 * (double)NX * NY);
-  A.part = (double*)C.part.p;
+  A.part = (double*)W.part.p;
   A.n_deg = n_deg;
   A.n_row = n_row;
   A.n_ky = n_ky;
   A.n_elem = n_elem;
-  A.cost = (double*)C.cost.p;
-  A.S = (DevState*)C.state.p;
-  A.bar = (unsigned long long*)C.bar.p;
   A.lds_tables = lds_tables;
-  {  // contiguous chunk ranges with balanced entry counts
-    std::vector<int> rg(nblk + 1, 0);
-    const std::vector<int4>& ch = N.h_chunks;
-    long tot = 0;
-    for (const int4& c4 : ch) tot += c4.z - c4.y;
-    long acc = 0;
-    int b = 1;
-    for (int c = 0; c < (int)ch.size() && b < nblk; c++) {
-      acc += ch[c].z - ch[c].y;
-      while (b < nblk && acc * nblk >= tot * (long)b) rg[b++] = c + 1;
-    }
-    for (; b <= nblk; b++) rg[b] = (int)ch.size();
-    if (C.granges.ensure((size_t)(nblk + 1) * 4)) return MFISTA_E_HIP;
-    MF_CHECK(hipMemcpyAsync(C.granges.p, rg.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice, st));
-    MF_CHECK(hipStreamSynchronize(st));
-    A.granges = (const int*)C.granges.p;
-  }
-  A.hier_barrier = C.persist_hier_barrier;
-  void* args[] = {&A};
-  hipEvent_t e0, e1;
-  MF_CHECK(hipEventCreate(&e0));
-  MF_CHECK(hipEventCreate(&e1));
-  MF_CHECK(hipEventRecord(e0, st));
-  MF_CHECK(hipLaunchCooperativeKernel(fn, dim3(nblk), dim3(MF_BLOCK), args, (unsigned)sh, st));
-  MF_CHECK(hipEventRecord(e1, st));
-  MF_CHECK(hipStreamSynchronize(st));
-  float kms = 0;
-  MF_CHECK(hipEventElapsedTime(&kms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  out->kernel_ms = kms;
-  C.persist_blocks = nblk;
-  MF_CHECK(hipGetLastError());
+  A.nblk = nblk;
+  A.cost = (double*)W.cost.p;
+  A.S = (DevState*)W.state.p;
+  A.bar = (unsigned long long*)W.bar.p;
+  A.hier_barrier = hier;
+  return 0;
+}
+
+// Reads one problem's outcome after the launch and copies x to P.xout.
+static int persistent_collect(PersistWork& W, const RunParams& P, hipStream_t st, RunStats* out) {
   unsigned long long hb[8];
-  MF_CHECK(hipMemcpy(hb, C.bar.p, sizeof(hb), hipMemcpyDeviceToHost));
+  MF_CHECK(hipMemcpy(hb, W.bar.p, sizeof(hb), hipMemcpyDeviceToHost));
   if (hb[1]) {
     fprintf(stderr,
-            "[mfista_amd] persistent solver: grid barrier timed out (%s barrier; workgroup %llu "
-            "waited on word %llu for %llu, saw %llu, after %llu polls)\n",
-            A.hier_barrier ? "XCD-hierarchical" : "flat", hb[7] & 0xfffff, hb[4], hb[5], hb[6], hb[7] >> 20);
+            "[mfista_amd] persistent solver: grid barrier timed out (workgroup %llu waited on word %llu "
+            "for %llu, saw %llu, after %llu polls)\n",
+            hb[7] & 0xfffff, hb[4], hb[5], hb[6], hb[7] >> 20);
     return MFISTA_E_HIP;
   }
   DevState h;
-  MF_CHECK(hipMemcpy(&h, C.state.p, sizeof(h), hipMemcpyDeviceToHost));
+  MF_CHECK(hipMemcpy(&h, W.state.p, sizeof(h), hipMemcpyDeviceToHost));
   out->ITER = h.ITER;
   out->c = h.c;
   out->max_trials = h.max_trials;
   out->total_trials = h.total_trials;
   out->status = h.status;
   out->steps = h.total_trials;
-  MF_CHECK(hipMemcpyAsync(P.xout, C.px.p, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  MF_CHECK(hipMemcpyAsync(P.xout, W.px.p, (size_t)P.NX * P.NY * 8, hipMemcpyDeviceToDevice, st));
+  return 0;
+}
+
+static int timed_launch(const PLaunch& L, void** args, int nblk, hipStream_t st, float* ms) {
+  hipEvent_t e0, e1;
+  MF_CHECK(hipEventCreate(&e0));
+  MF_CHECK(hipEventCreate(&e1));
+  MF_CHECK(hipEventRecord(e0, st));
+  MF_CHECK(hipLaunchCooperativeKernel(L.fn, dim3(nblk), dim3(MF_BLOCK), args, (unsigned)L.sh, st));
+  MF_CHECK(hipEventRecord(e1, st));
   MF_CHECK(hipStreamSynchronize(st));
+  MF_CHECK(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MF_CHECK(hipGetLastError());
+  return 0;
+}
+
+int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
+  NufftState& N = C.nu;
+  hipStream_t st = C.stream;
+  PLaunch L;
+  int rc;
+  if ((rc = persistent_config(C, N.MSP, {&N}, false, &L))) return rc;
+  PArgs A;
+  if ((rc = persistent_prepare(N, C.pw, P, L.nblk, L.lds_tables, C.persist_hier_barrier, st, &A))) return rc;
+  void* args[] = {&A};
+  float kms = 0;
+  if ((rc = timed_launch(L, args, L.nblk, st, &kms))) return rc;
+  out->kernel_ms = kms;
+  C.persist_blocks = L.nblk;
+  if ((rc = persistent_collect(C.pw, P, st, out))) return rc;
+  MF_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Batched solves: problem i uses sets[i] (its NUFFT set-up) and C.bwork[i].
+// The launch's workgroups are dealt to problems XCD by XCD (workgroup b runs
+// on XCD b mod 8): each problem goes whole to the XCD with the least work so
+// far, and each XCD's workgroups are split among its problems in proportion
+// to their work per iteration, so a problem's data stays in one L2.
+// Problems beyond one workgroup each go to further launches.
+int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                      std::vector<RunStats>& out, float* kernel_ms) {
+  const int np = (int)P.size();
+  hipStream_t st = C.stream;
+  if (np == 0) return 0;
+  const int MSP = sets[0]->MSP;
+  std::vector<const NufftState*> cs(sets.begin(), sets.end());
+  for (const NufftState* N : cs)
+    if (N->MSP != MSP) return MFISTA_E_ARG;
+  PLaunch L;
+  int rc;
+  if ((rc = persistent_config(C, MSP, cs, true, &L))) return rc;
+  if ((int)C.bwork.size() < np) C.bwork.resize(np);
+  out.assign(np, RunStats());
+  *kernel_ms = 0;
+  const int kX = 8;
+  const int per_x = L.nblk / kX;
+  for (int p0 = 0; p0 < np; p0 += L.nblk) {
+    const int pn = std::min(np - p0, L.nblk);
+    // work per iteration: pixels and visibility taps
+    std::vector<double> w(pn);
+    for (int i = 0; i < pn; i++)
+      w[i] = (double)P[p0 + i].NX * P[p0 + i].NY + 4.0 * sets[p0 + i]->M * sets[p0 + i]->MSP;
+    std::vector<int> order(pn), xcd(pn, 0), nb(pn, 1);
+    for (int i = 0; i < pn; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+    std::vector<double> load(kX, 0.0);
+    std::vector<int2> blk(L.nblk, make_int2(-1, 0));
+    if (pn >= kX) {
+      std::vector<int> cnt(kX, 0);
+      for (int i : order) {
+        int best = -1;
+        for (int x = 0; x < kX; x++)
+          if (cnt[x] < per_x && (best < 0 || load[x] < load[best])) best = x;
+        xcd[i] = best;
+        load[best] += w[i];
+        cnt[best]++;
+      }
+      // split each XCD's workgroups among its problems (at least one each)
+      std::vector<std::vector<int>> slots(kX);
+      for (int b = 0; b < L.nblk; b++) slots[b % kX].push_back(b);
+      for (int x = 0; x < kX; x++) {
+        std::vector<int> mine;
+        for (int i = 0; i < pn; i++)
+          if (xcd[i] == x) mine.push_back(i);
+        if (mine.empty()) continue;
+        const int avail = (int)slots[x].size();
+        const int left = avail - (int)mine.size();
+        int used = 0;
+        for (int i : mine) {
+          nb[i] = 1 + (int)((double)left * w[i] / std::max(load[x], 1e-300));
+          used += nb[i];
+        }
+        for (size_t k = 0; used < avail; k = (k + 1) % mine.size(), used++) nb[mine[k]]++;
+        int s0 = 0;
+        for (int i : mine) {
+          for (int j = 0; j < nb[i]; j++) blk[slots[x][s0 + j]] = make_int2(i, j);
+          s0 += nb[i];
+        }
+      }
+    } else {
+      // fewer problems than XCDs: each spans several XCDs, workgroups in
+      // proportion to work, interleaved
+      double wt = 0;
+      for (int i = 0; i < pn; i++) wt += w[i];
+      int used = 0;
+      for (int i = 0; i < pn; i++) {
+        nb[i] = std::max(1, (int)((double)L.nblk * w[i] / wt));
+        used += nb[i];
+      }
+      while (used > L.nblk)
+        for (int i = 0; i < pn && used > L.nblk; i++)
+          if (nb[i] > 1) nb[i]--, used--;
+      std::vector<int> next(pn, 0);
+      int b = 0;
+      for (bool any = true; any;) {
+        any = false;
+        for (int i = 0; i < pn; i++)
+          if (next[i] < nb[i]) {
+            blk[b++] = make_int2(i, next[i]++);
+            any = true;
+          }
+      }
+    }
+    int nlaunch = 0;
+    for (int b = 0; b < L.nblk; b++)
+      if (blk[b].x >= 0) nlaunch = b + 1;
+    // workgroups past the last dealt one are not launched; undealt ones in
+    // between return at once
+    std::vector<PArgs> A(pn);
+    for (int i = 0; i < pn; i++)
+      if ((rc = persistent_prepare(*sets[p0 + i], C.bwork[p0 + i], P[p0 + i], nb[i], L.lds_tables,
+                                   C.persist_hier_barrier, st, &A[i])))
+        return rc;
+    if (C.batch_args.ensure((size_t)pn * sizeof(PArgs)) || C.batch_blk.ensure((size_t)nlaunch * sizeof(int2)))
+      return MFISTA_E_HIP;
+    MF_CHECK(hipMemcpyAsync(C.batch_args.p, A.data(), (size_t)pn * sizeof(PArgs), hipMemcpyHostToDevice, st));
+    MF_CHECK(hipMemcpyAsync(C.batch_blk.p, blk.data(), (size_t)nlaunch * sizeof(int2), hipMemcpyHostToDevice, st));
+    MF_CHECK(hipStreamSynchronize(st));
+    const PArgs* dargs = (const PArgs*)C.batch_args.p;
+    const int2* dblk = (const int2*)C.batch_blk.p;
+    void* args[] = {&dargs, &dblk};
+    float kms = 0;
+    if ((rc = timed_launch(L, args, nlaunch, st, &kms))) return rc;
+    *kernel_ms += kms;
+    for (int i = 0; i < pn; i++) {
+      out[p0 + i].kernel_ms = kms;
+      if ((rc = persistent_collect(C.bwork[p0 + i], P[p0 + i], st, &out[p0 + i]))) return rc;
+    }
+    MF_CHECK(hipStreamSynchronize(st));
+  }
   return 0;
 }
 

@@ -39,6 +39,24 @@ struct NufftState {
   }
 };
 
+// Device workspace of one persistent-engine solve.
+struct PersistWork {
+  DevMem px, pxn, pz;      // FP64 iterates x, xnew, z
+  DevMem g;                // FP32 gradient
+  DevMem vmod;             // three FP64 weighted model vectors
+  DevMem T64, Gf64;        // FP64 forward transform of the step
+  DevMem Hg, U;            // FP32 adjoint grid and column-pass output
+  DevMem part, state, cost, bar, fgp, fgpc, granges;
+  DevMem x0, xo, box;      // batched host-pointer solves: staged xinit, xout, cl_box
+};
+
+// One visibility set of a batch, with its NUFFT set-up.
+struct BatchSet {
+  DevMem u, v, vr, vi, sd;
+  NufftState nu;
+  double vis_power = 0;  // sum |V|^2 (eps scaling)
+};
+
 struct Context {
   int device = -1;
   hipStream_t stream = nullptr;
@@ -50,11 +68,14 @@ struct Context {
   DevMem x, z, xnew, g, dstep, x64, part, stat, stat64, chi64, state, cost;
   DevMem fgp;   // TV prox duals
   DevMem fgpc;  // FGP momentum coefficients
-  DevMem bar;   // grid barrier of the persistent solver
-  DevMem px, pxn, pz;  // persistent solver: FP64 iterates x, xnew, z
+  PersistWork pw;  // the persistent solver's workspace (single solves)
+  // batched solves: visibility sets (each with its NUFFT set-up) and one
+  // workspace per problem
+  std::vector<BatchSet> bset;
+  std::vector<PersistWork> bwork;
+  DevMem batch_args, batch_blk;  // PArgs array and workgroup table of a batch launch
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
-  DevMem vmod;  // persistent solver: three FP64 weighted model vectors
-  DevMem granges;  // persistent solver: first gridding chunk of each workgroup (nblk + 1)
+
   int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
   int persist_hier_barrier = 1;  // MFISTA_PERSISTENT_FLAT_BARRIER=1 selects the flat barrier

@@ -618,6 +618,155 @@ int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double
   return rc;
 }
 
+// Batched NUFFT solves (one persistent launch); include/mfista.h describes
+// the contract.  Each problem follows mfista_imaging_core_nufft
+// (c/mfista_nufft_lib.c:909-957): eps scaled by its mean |V|^2, the same
+// argument checks and messages, and the FP64 RESULT pass.
+int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, struct RESULT* res,
+                                    mfista_opts* opts) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (nprob <= 0 || !pr || !res) return MFISTA_E_ARG;
+  const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
+  const int fgp_iter = (opts && opts->fgp_iter > 0) ? opts->fgp_iter : FGPITER;
+  const double rel_tol = opts ? opts->rel_tol : 0.0;
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> tv(nprob, 0);
+  for (int i = 0; i < nprob; i++) {
+    const mfista_nufft_problem& q = pr[i];
+    if (q.M <= 0 || q.NX <= 0 || q.NY <= 0 || !q.u_dx || !q.v_dy || !q.vis_r || !q.vis_i || !q.vis_std ||
+        !q.xinit || !q.xout || (q.box_flag && !q.cl_box) || q.maxiter <= 0)
+      return MFISTA_E_ARG;
+    if (q.lambda_tv == 0)
+      tv[i] = 0;
+    else if (q.lambda_tsv == 0)
+      tv[i] = 1;
+    else {
+      printf("You cannot set both of lambda_TV and lambda_TSV positive.\n");
+      return MFISTA_E_BOTH_REG;
+    }
+    if (tv[i] && preset == MFISTA_PRESET_CXX12) {
+      printf("We have not implemented TV option.\n");
+      return MFISTA_E_TV_PRESET;
+    }
+  }
+  // visibility sets: problems with the same arrays share one NUFFT set-up
+  std::vector<int> set_of(nprob, -1);
+  std::vector<int> set_rep;  // a problem of each set
+  for (int i = 0; i < nprob; i++) {
+    for (size_t k = 0; k < set_rep.size(); k++) {
+      const mfista_nufft_problem &a = pr[i], &b = pr[set_rep[k]];
+      if (a.u_dx == b.u_dx && a.v_dy == b.v_dy && a.vis_r == b.vis_r && a.vis_i == b.vis_i &&
+          a.vis_std == b.vis_std && a.M == b.M && a.NX == b.NX && a.NY == b.NY) {
+        set_of[i] = (int)k;
+        break;
+      }
+    }
+    if (set_of[i] < 0) {
+      set_of[i] = (int)set_rep.size();
+      set_rep.push_back(i);
+    }
+  }
+  const int nset = (int)set_rep.size();
+  if ((int)C->bset.size() < nset) C->bset.resize(nset);
+  int rc;
+  for (int k = 0; k < nset; k++) {
+    const mfista_nufft_problem& q = pr[set_rep[k]];
+    BatchSet& S = C->bset[k];
+    const size_t Mb = (size_t)q.M * 8;
+    if ((rc = upload(*C, S.u, q.u_dx, Mb)) || (rc = upload(*C, S.v, q.v_dy, Mb)) ||
+        (rc = upload(*C, S.vr, q.vis_r, Mb)) || (rc = upload(*C, S.vi, q.vis_i, Mb)) ||
+        (rc = upload(*C, S.sd, q.vis_std, Mb)))
+      return rc;
+    double pw = 0;
+    for (int m = 0; m < q.M; m++) pw += q.vis_r[m] * q.vis_r[m] + q.vis_i[m] * q.vis_i[m];
+    S.vis_power = pw;
+    std::swap(C->nu, S.nu);
+    rc = nufft_setup(*C, preset, (const double*)S.u.p, (const double*)S.v.p, (const double*)S.vr.p,
+                     (const double*)S.vi.p, (const double*)S.sd.p, q.M, q.NX, q.NY);
+    std::swap(C->nu, S.nu);
+    if (rc) {
+      if (rc == MFISTA_E_FOV) printf("Error: the field of view is too small for the (u,v) coverage.\n");
+      return rc;
+    }
+  }
+  if ((int)C->bwork.size() < nprob) C->bwork.resize(nprob);
+  std::vector<NufftState*> sets;
+  std::vector<RunParams> P;
+  std::vector<int> which;  // problem index of each solve
+  for (int i = 0; i < nprob; i++) {
+    const mfista_nufft_problem& q = pr[i];
+    memset(&res[i], 0, sizeof(RESULT));
+    if (!tv[i] && q.nonneg_flag != 0 && q.nonneg_flag != 1) {  // c/mfista_nufft_lib.c:634-637
+      printf("nonneg_flag must be chosen properly.\n");
+      continue;
+    }
+    PersistWork& W = C->bwork[i];
+    const size_t nb = (size_t)q.NX * q.NY * 8;
+    if ((rc = upload(*C, W.x0, q.xinit, nb)) || (rc = upload(*C, W.xo, nullptr, nb))) return rc;
+    if (q.box_flag && (rc = upload(*C, W.box, q.cl_box, (size_t)q.NX * q.NY * 4))) return rc;
+    RunParams R;
+    R.NX = q.NX;
+    R.NY = q.NY;
+    R.maxiter = q.maxiter;
+    R.nonneg = q.nonneg_flag;
+    R.box_flag = q.box_flag;
+    R.tv_mode = tv[i];
+    R.fgp_iter = fgp_iter;
+    R.eps_abs = q.eps * C->bset[set_of[i]].vis_power / (double)q.M;
+    R.rel_tol = rel_tol;
+    R.lambda_l1 = q.lambda_l1;
+    R.lambda_tv = q.lambda_tv;
+    R.lambda_tsv = q.lambda_tsv;
+    R.cinit = q.cinit;
+    R.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
+    R.xinit = (const double*)W.x0.p;
+    R.xout = (double*)W.xo.p;
+    R.box = q.box_flag ? (const float*)W.box.p : nullptr;
+    sets.push_back(&C->bset[set_of[i]].nu);
+    P.push_back(R);
+    which.push_back(i);
+  }
+  std::vector<RunStats> st;
+  float kms = 0;
+  if ((rc = solve_nufft_batch(*C, sets, P, st, &kms))) return rc;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  int total_trials = 0, max_trials = 0;
+  for (size_t j = 0; j < which.size(); j++) {
+    const int i = which[j];
+    const mfista_nufft_problem& q = pr[i];
+    PersistWork& W = C->bwork[i];
+    RESULT* R = &res[i];
+    R->ITER = st[j].ITER;
+    R->Lip_const = st[j].c;
+    R->nonneg = q.nonneg_flag;
+    R->maxiter = q.maxiter;
+    R->comp_time = secs;
+    total_trials += st[j].total_trials;
+    max_trials = std::max(max_trials, st[j].max_trials);
+    MF_CHECK(hipMemcpy(q.xout, W.xo.p, (size_t)q.NX * q.NY * 8, hipMemcpyDeviceToHost));
+    // FP64 result pass (calc_result_nufft, c/mfista_nufft_lib.c:791-905)
+    double chi2 = 0, stats[4] = {0, 0, 0, 0};
+    NufftState& N = C->bset[set_of[i]].nu;
+    std::swap(C->nu, N);
+    rc = nufft_chi2_64(*C, (const double*)W.xo.p, &chi2);
+    if (!rc) rc = image_stats64(*C, (const double*)W.xo.p, q.NX, q.NY, stats);
+    std::swap(C->nu, N);
+    if (rc) return rc;
+    fill_result_common(R, q.NX, q.NY, q.lambda_l1, q.lambda_tv, q.lambda_tsv, chi2, stats);
+    R->M = (preset == MFISTA_PRESET_CXX12) ? q.M : (int)(q.M / 2);
+    R->mean_sq_error = R->sq_error / (double)q.M;
+  }
+  if (opts) {
+    opts->solve_ms = secs * 1e3;
+    opts->kernel_ms = kms;
+    opts->total_trials = total_trials;
+    opts->max_trials = max_trials;
+  }
+  return 0;
+}
+
 int mfista_imaging_core_nufft_ex(const double* u_dx, const double* v_dy, const double* vis_r,
                                  const double* vis_i, const double* vis_std, int M, int NX,
                                  int NY, int maxiter, double eps, double lambda_l1,

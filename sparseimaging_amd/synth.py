@@ -136,3 +136,26 @@ def make_fft_problem(N=64, ncell=1500, seed=1001, noise_frac=0.01):
         "y_r": np.array(yr), "y_i": np.array(yi), "noise": np.full(M, sig), "x_true": x,
         "N": N, "M": M, "seed": seed,
     }
+
+
+def make_movie_frames(nframes=32, N=128, seed=1004, mmin=5000, mmax=40000, drift=0.5, noise_frac=0.01):
+    """Movie of a drifting ring: frame f is a ring of radius 0.16 N (FWHM
+    0.05 N) whose centre moves `drift` px per frame along the diagonal, plus
+    2 fixed Gaussians; its (u,v) coverage is 12 elliptical arcs whose rotation
+    advances with the frame index, with M_f drawn uniformly from
+    {mmin .. mmax}.  Returns a list of make_nufft_problem-style dicts."""
+    rng = np.random.default_rng(seed)
+    blobs = gaussian_blobs(N, rng, 2)
+    frames = []
+    for f in range(nframes):
+        M = int(rng.integers(mmin, mmax + 1))
+        c = N / 2 - drift * nframes / 2 + drift * f
+        x = np.maximum(ring(N, 0.16 * N, 0.05 * N, c, c) + blobs, 0.0)
+        x /= x.max()
+        u, v = elliptical_arcs(M, rng, rot=np.pi * f / nframes)
+        vis = dft_vis(x, u, v)
+        sig = noise_frac * np.abs(vis).max()
+        vis = vis + sig * (rng.standard_normal(M) + 1j * rng.standard_normal(M))
+        frames.append({"u": u, "v": v, "vis_r": vis.real.copy(), "vis_i": vis.imag.copy(),
+                       "std": np.full(M, sig), "x_true": x, "N": N, "M": M, "seed": seed, "frame": f})
+    return frames

@@ -52,6 +52,22 @@ def test_result_layout_matches_reference(tmp_path):
     assert sa.RESULT.residual.offset == 112 and sa.RESULT.Lip_const.offset == 120
 
 
+def test_batch_problem_and_opts_layout_match_ctypes(tmp_path):
+    prog = tmp_path / "layout2.c"
+    prog.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "mfista.h"\n'
+        "int main(void){\n"
+        'printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mfista_nufft_problem), offsetof(mfista_nufft_problem, M),'
+        " offsetof(mfista_nufft_problem, eps), offsetof(mfista_nufft_problem, xinit),"
+        " offsetof(mfista_nufft_problem, cl_box), sizeof(mfista_opts)); return 0;}\n")
+    exe = tmp_path / "layout2"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    P = sa.NufftProblem
+    assert out == [ctypes.sizeof(P), P.M.offset, P.eps.offset, P.xinit.offset, P.cl_box.offset,
+                   ctypes.sizeof(sa.Opts)]
+
+
 def test_product_does_not_import_the_oracle():
     pkg = os.path.join(ROOT, "sparseimaging_amd")
     for dirpath, _, files in os.walk(pkg):

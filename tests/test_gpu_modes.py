@@ -83,25 +83,31 @@ def test_nufft_tv_c24():
 @pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
 def test_nufft_reference_data(preset):
     """c/nufft_data.txt (200x200, M=7648) with the author's run parameters
-    (matlab/testdata4nufft.mat: lambda_l1=100, lambda_tsv=100, cinit=5e8, nonneg),
-    run to a 1e-8 relative objective tolerance and compared with the FP64
-    oracle's converged solution of the same problem (ref_nufft_solution.npz,
-    written by make_golden.py)."""
+    (matlab/testdata4nufft.mat: lambda_l1=100, lambda_tsv=100, cinit=5e8,
+    nonneg), run to a 1e-10 relative tolerance and compared with the optimum
+    of the same problem (ref_nufft_solution.npz: L-BFGS-B on the oracle's FP64
+    operators, pinned on the CPU by its KKT conditions).
+
+    The objective of this problem is very flat: the oracle's own MFISTA at a
+    1e-8 tolerance stops 7.6e-6 (CXX-12) and 2.1e-6 (C-24) above the optimum
+    with images 5.4e-2 and 2.6e-2 away from it.  The GPU solve must reach the
+    optimum's objective to 1e-6 (measured ~2e-7) and be at least 2x closer to
+    the optimal image than that (bar 1e-2; measured 6e-3 .. 9e-3)."""
     d = np.load(os.path.join(G, "ref_nufft_data.npz"))
     sol = np.load(os.path.join(G, "ref_nufft_solution.npz"))
     N = int(d["NX"])
-    kw = dict(maxiter=20000, eps=0.0, lambda_l1=100.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=5e8, rel_tol=1e-8)
+    kw = dict(maxiter=50000, eps=0.0, lambda_l1=100.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=5e8, rel_tol=1e-10)
     print(f"case reference nufft_data.txt {N}x{N} M={d['u'].size} preset={preset}", flush=True)
     xg, rg, _ = sa.mfista_imaging_core_nufft(d["u"], d["v"], d["vis_r"], d["vis_i"], d["std"], N, N,
                                              nonneg_flag=1, preset=preset, **kw)
     xo = sol[f"x{preset}"].astype(np.float64)
     fo = float(sol[f"finalcost{preset}"])
     l2 = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
-    dc = abs(rg["finalcost"] - fo) / abs(fo)
-    print(f"  nufft ref-data: ITER gpu={rg['ITER']} oracle={int(sol[f'iter{preset}'])} image relL2={l2:.3e} "
-          f"cost rel={dc:.3e}", flush=True)
-    assert l2 <= 1e-3, l2
-    assert dc <= 1e-6, dc
+    dc = (rg["finalcost"] - fo) / abs(fo)
+    print(f"  nufft ref-data: ITER gpu={rg['ITER']} image relL2={l2:.3e} cost (gpu - optimum)/optimum={dc:.3e}",
+          flush=True)
+    assert abs(dc) <= 1e-6, dc
+    assert l2 <= 1e-2, l2
 
 
 def test_clean_box_and_nonneg_flag():
@@ -115,3 +121,29 @@ def test_clean_box_and_nonneg_flag():
     x2, r2, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
                                              nonneg_flag=2, preset=sa.PRESET_CXX12, **kw)
     assert r2["ITER"] == 0  # c/mfista_nufft_lib.c:634-637
+
+
+def test_nufft_tv_512_m200000_against_oracle():
+    """The 512^2 L1+TV workload (20 Gaussians + 2 rings, M = 200000 points in a
+    disc of radius 0.48 cycles/px with density ~ 1/r, FGP 50 iterations,
+    lambda_l1 = 0.5, lambda_tv = 5, nonneg, 24-tap preset) at full size.  The
+    FP64 oracle needs ~4 s per iteration here, so both run the same first 8
+    MFISTA iterations (no stopping rule involved) and are compared there; the
+    full 500-iteration GPU solve must then lower the objective further."""
+    P = synth.make_nufft_problem(N=512, M=200000, seed=1003, kind="tv", coverage="disc", amax=0.48)
+    kw = dict(eps=0.0, lambda_l1=0.5, lambda_tv=5.0, lambda_tsv=0.0, cinit=1e4)
+    print("case nufft L1+TV 512x512 M=200000 fgp=50, 8 iterations", flush=True)
+    xg, rg, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512,
+                                             maxiter=8, nonneg_flag=1, preset=sa.PRESET_C24, fgp_iter=50, **kw)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512, 8, kw["eps"],
+                                  kw["lambda_l1"], kw["lambda_tv"], 0.0, kw["cinit"], nonneg=1,
+                                  preset=oracle.PRESET_C24, fgp_iter=50)
+    _cmp("nufft TV 512 (8 it)", xg, rg, o, 1e-3, 1e-6)
+    assert rg["ITER"] == o["result"]["ITER"] == 8
+    print("case nufft L1+TV 512x512 M=200000 fgp=50, 500 iterations", flush=True)
+    xf, rf, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512,
+                                                maxiter=500, nonneg_flag=1, preset=sa.PRESET_C24, fgp_iter=50,
+                                                **kw)
+    print(f"  500 iterations: {info['solve_ms']:.0f} ms, finalcost {rf['finalcost']:.6e} "
+          f"(8 iterations: {rg['finalcost']:.6e})", flush=True)
+    assert rf["finalcost"] < rg["finalcost"]
