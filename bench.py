@@ -151,6 +151,47 @@ def bench_movie(sa, synth):
     return out
 
 
+def bench_warm_start(sa, nprob):
+    """Cold vs learned warm start on held-out 256^2 problems (random ring +
+    Gaussian skies, M = 20000 arcs, lambda_l1 = 1, lambda_tsv = 100, 12 taps,
+    1e-6 relative tolerance).  Warm = network inference + MFISTA refinement;
+    the acceptance rule (warm kept only if its converged objective is not
+    higher than the cold start's) is applied to every problem."""
+    import numpy as np
+
+    from sparseimaging_amd import learn
+
+    if not os.path.exists(learn.WEIGHTS):
+        return {"status": "no trained weights (sparseimaging_amd/weights/unet_256.npz)"}
+    net = learn.UNet.load(learn.WEIGHTS)
+    rng = np.random.default_rng(9000)
+    kw = dict(maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, rel_tol=1e-6)
+    tc = tw = 0.0
+    acc = 0
+    it_c, it_w, rel = [], [], []
+    for k in range(nprob):
+        P = learn.make_problem(256, 20000, rng)
+        args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"])
+        t0 = time.perf_counter()
+        _, rc, _ = sa.mfista_imaging_core_nufft(*args, 256, 256, lambda_l1=1.0, lambda_tsv=100.0,
+                                                preset=sa.PRESET_CXX12, **kw)
+        tc += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, rw, info = learn.solve_warm(net, *args, 256, lambda_l1=1.0, lambda_tsv=100.0, **kw)
+        tw += time.perf_counter() - t0
+        ok = rw["finalcost"] <= rc["finalcost"]
+        acc += int(ok)
+        it_c.append(rc["ITER"])
+        it_w.append(rw["ITER"])
+        rel.append((rw["finalcost"] - rc["finalcost"]) / rc["finalcost"])
+    out = {"problems": nprob, "cold_solves_per_s": round(nprob / tc, 3), "warm_solves_per_s": round(nprob / tw, 3),
+           "speedup": round(tc / tw, 3), "warm_accepted": acc, "mean_iter_cold": float(np.mean(it_c)),
+           "mean_iter_warm": float(np.mean(it_w)), "max_objective_rel_diff_warm_minus_cold": float(np.max(rel)),
+           "training": {k: net.meta.get(k) for k in ("n", "epochs", "N", "M")}}
+    log("bench: learned warm start " + json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +203,8 @@ def main():
     ap.add_argument("--engine", choices=["persistent", "stream"], default="persistent")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
+    ap.add_argument("--warm", type=int, default=8,
+                    help="held-out 256^2 problems for the learned warm start (0 = skip)")
     ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
                     help="also time the batched workloads (6x6 lambda grid; 32-frame movie) "
                          "against the same problems solved one by one")
@@ -250,6 +293,8 @@ def main():
     if args.batch in ("movie", "all"):
         batch["movie"] = bench_movie(sa, synth)
 
+    warm = bench_warm_start(sa, args.warm) if args.warm > 0 else None
+
     cpu = None
     if args.cpu_iters > 0:
         import oracle
@@ -288,6 +333,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batch or None,
+        "learned_warm_start": warm,
         "kernels": kern,
     }
     print(json.dumps(out), flush=True)

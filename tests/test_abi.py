@@ -12,10 +12,11 @@ import sparseimaging_amd as sa
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "mfista.h")
+LEARN_HEADER = os.path.join(ROOT, "include", "mfista_learn.h")
 
 
-def _declared_functions():
-    src = open(HEADER).read()
+def _declared_functions(header=HEADER):
+    src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char|long)\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", src, flags=re.M)
     return sorted(set(names))
@@ -28,6 +29,14 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(lib, n), n
     assert sorted(sa.EXPORTED) == names
+
+
+def test_learn_header_symbols_exported():
+    lib = sa.lib()
+    names = _declared_functions(LEARN_HEADER)
+    assert "unet_conv3x3_fwd" in names and "unet_adam" in names
+    for n in names:
+        assert hasattr(lib, n), n
 
 
 def test_reference_symbols_present():
