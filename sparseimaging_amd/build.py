@@ -1,8 +1,11 @@
 """Build libmfista_amd.so (HIP, gfx950) in-tree.
 
-Every .hip file under csrc/ is compiled by hipcc in parallel and linked into
+Every .hip file under csrc/ is compiled by hipcc in parallel, every .cpp
+file (host-only code) by g++, and all are linked into
 ``sparseimaging_amd/lib/libmfista_amd.so``.  Packed-FP32 instructions are
 disabled for all device code (``-Xclang -target-feature -Xclang -packed-fp32-ops``).
+The command-line tools under cli/ are then linked against the library into
+``sparseimaging_amd/bin/`` (run path ``$ORIGIN/../lib``).
 """
 from __future__ import annotations
 
@@ -17,7 +20,12 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libmfista_amd.so")
+CLIDIR = os.path.join(HERE, "cli")
+BINDIR = os.path.join(HERE, "bin")
+TOOLS = ("mfista_imaging_nufft", "mfista_imaging_fft", "mfista_imaging_dft")
+CXX = os.environ.get("CXX", "g++")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+HOST_CFLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFISTA_OFFLOAD_ARCH", "gfx950")
@@ -30,7 +38,10 @@ CFLAGS = [
 
 def _compile(src: str) -> str:
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [CXX, *HOST_CFLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -41,7 +52,7 @@ def _compile(src: str) -> str:
 def build(verbose: bool = True) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(OBJDIR, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=max(jobs, 1)) as ex:
         objs = list(ex.map(_compile, srcs))
@@ -54,7 +65,24 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_tools(verbose)
     return LIB
+
+
+def build_tools(verbose: bool = True) -> None:
+    """Links the command-line tools against the built library."""
+    os.makedirs(BINDIR, exist_ok=True)
+    for name in TOOLS:
+        out = os.path.join(BINDIR, name)
+        cmd = [CXX, *HOST_CFLAGS, os.path.join(CLIDIR, name + ".cpp"), "-o", out + ".tmp", f"-L{LIBDIR}",
+               "-lmfista_amd", "-Wl,-rpath,$ORIGIN/../lib", "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"build of {name} failed")
+        os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {', '.join(TOOLS)} in {BINDIR}")
 
 
 if __name__ == "__main__":

@@ -182,6 +182,181 @@ __global__ void __launch_bounds__(NT) k_conv_gemm(ConvArgs a, long M, int N, lon
       }
 }
 
+// Fast path of k_conv_gemm when every channel count involved is a multiple
+// of 32 (all layers but the first): a 32-wide K slice then lies in one tap
+// and one input tensor, so each thread stages 8 consecutive channels of one
+// pixel with two float4 loads and one address computation.
+struct PixelIdx {
+  int b, y, x;
+};
+
+__device__ inline PixelIdx pix(long p, int H, int W) {
+  PixelIdx r;
+  r.x = (int)(p % W);
+  const long q = p / W;
+  r.y = (int)(q % H);
+  r.b = (int)(q / H);
+  return r;
+}
+
+// 8 consecutive channels c0 .. c0+7 of `src` (C channels) at pixel (b, yy, xx), zero outside
+__device__ inline void load8(const float* src, int C, int c0, int b, int yy, int xx, int H, int W, int B,
+                             float (&v)[8]) {
+  if (b >= B || yy < 0 || yy >= H || xx < 0 || xx >= W) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = 0.f;
+    return;
+  }
+  const float4* q = reinterpret_cast<const float4*>(src + (((long)b * H + yy) * W + xx) * C + c0);
+  const float4 u0 = q[0], u1 = q[1];
+  v[0] = u0.x, v[1] = u0.y, v[2] = u0.z, v[3] = u0.w, v[4] = u1.x, v[5] = u1.y, v[6] = u1.z, v[7] = u1.w;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(NT) k_conv_gemm_vec(ConvArgs a, long M, int N, long K) {
+  __shared__ uint16_t As[BM][BK + 8];
+  __shared__ uint16_t Bs[BN][BK + 8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long m0 = (long)blockIdx.x * BM;
+  const int n0 = blockIdx.y * BN;
+  const int Cin = a.C1 + a.C2;
+  long k_begin = 0, k_end = K;
+  if (MODE == kBwdWeight) {
+    k_begin = (long)blockIdx.z * a.pix_per_split;
+    k_end = k_begin + a.pix_per_split;
+    if (k_end > K) k_end = K;
+  }
+  floatx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // per-thread staging coordinates that do not change along K
+  PixelIdx pa{0, 0, 0};
+  long gma = 0;
+  if (MODE != kBwdWeight) {  // A rows are pixels: thread -> (row tid / 4, channels (tid % 4) * 8)
+    gma = m0 + tid / 4;
+    pa = pix(gma, a.H, a.W);
+  }
+  for (long k0 = k_begin; k0 < k_end; k0 += BK) {
+    float v[8];
+    if (MODE == kFwd) {
+      const int t = (int)(k0 / Cin), c0 = (int)(k0 - (long)t * Cin) + (tid % 4) * 8;
+      const int yy = pa.y + t / 3 - 1, xx = pa.x + t % 3 - 1;
+      const int b = (gma < M) ? pa.b : a.B;
+      if (c0 < a.C1)
+        load8(a.in1, a.C1, c0, b, yy, xx, a.H, a.W, a.B, v);
+      else
+        load8(a.in2, a.C2, c0 - a.C1, b, yy, xx, a.H, a.W, a.B, v);
+      uint16_t* dst = &As[tid / 4][(tid % 4) * 8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) dst[i] = f2bf(v[i]);
+      // B[k][co] = W[k][co]: row kk = tid / 8, columns (tid % 8) * 8 .. +7
+      const int kk = tid / 8, nn = (tid % 8) * 8;
+      const uint16_t* w = a.w + (k0 + kk) * a.Cout + n0 + nn;
+      if (n0 + nn + 8 <= N) {
+        const uint4 u = *reinterpret_cast<const uint4*>(w);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&u);
+#pragma unroll
+        for (int i = 0; i < 8; i++) Bs[nn + i][kk] = h[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) Bs[nn + i][kk] = (n0 + nn + i < N) ? w[i] : (uint16_t)0;
+      }
+    } else if (MODE == kBwdData) {
+      const int t = (int)(k0 / a.Cout), c0 = (int)(k0 - (long)t * a.Cout) + (tid % 4) * 8;
+      const int yy = pa.y - (t / 3 - 1), xx = pa.x - (t % 3 - 1);
+      const int b = (gma < M) ? pa.b : a.B;
+      load8(a.dout, a.Cout, c0, b, yy, xx, a.H, a.W, a.B, v);
+      uint16_t* dst = &As[tid / 4][(tid % 4) * 8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) dst[i] = f2bf(v[i]);
+      // B[(t,co)][ci] = W[t][ci][co]: row n = tid / 4 (ci), co (tid % 4) * 8 .. +7 contiguous
+      const int nn = tid / 4, cc = (int)(k0 - (long)t * a.Cout) + (tid % 4) * 8;
+      uint16_t* bd = &Bs[nn][(tid % 4) * 8];
+      if (n0 + nn < N) {
+        const uint4 u = *reinterpret_cast<const uint4*>(a.w + ((long)t * Cin + n0 + nn) * a.Cout + cc);
+        const uint16_t* h = reinterpret_cast<const uint16_t*>(&u);
+#pragma unroll
+        for (int i = 0; i < 8; i++) bd[i] = h[i];
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) bd[i] = 0;
+      }
+    } else {
+      // A[(t,ci)][p]: pixel kk = tid % 32, rows (tid / 32) * 8 .. +7 (one tap, 8 channels)
+      const int kk = tid % 32, mm = (tid / 32) * 8;
+      const long gp = k0 + kk;
+      const long gm = m0 + mm;
+      const int t = (int)(gm / Cin), c0 = (int)(gm - (long)t * Cin);
+      PixelIdx q = pix(gp, a.H, a.W);
+      const int b = (gp < k_end && t < 9) ? q.b : a.B;
+      const int yy = q.y + t / 3 - 1, xx = q.x + t % 3 - 1;
+      if (c0 < a.C1)
+        load8(a.in1, a.C1, c0, b, yy, xx, a.H, a.W, a.B, v);
+      else
+        load8(a.in2, a.C2, c0 - a.C1, b, yy, xx, a.H, a.W, a.B, v);
+#pragma unroll
+      for (int i = 0; i < 8; i++) As[mm + i][kk] = f2bf(v[i]);
+      // B[p][co] = dout[p][co]: 8 consecutive co of pixel kk
+      const int nn = (tid / 32) * 8;
+      if (gp < k_end && n0 + nn + 8 <= N) {
+        const float4* d = reinterpret_cast<const float4*>(a.dout + gp * a.Cout + n0 + nn);
+        const float4 u0 = d[0], u1 = d[1];
+        const float w8[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+        for (int i = 0; i < 8; i++) Bs[nn + i][kk] = f2bf(w8[i]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+          Bs[nn + i][kk] = (gp < k_end && n0 + nn + i < N) ? f2bf(a.dout[gp * a.Cout + n0 + nn + i]) : (uint16_t)0;
+      }
+    }
+    __syncthreads();
+    bf16x8 af[2], bfr[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      const uint16_t* qa = &As[wm * 32 + i * 16 + (lane & 15)][(lane >> 4) * 8];
+      const uint16_t* qb = &Bs[wn * 32 + i * 16 + (lane & 15)][(lane >> 4) * 8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        af[i][q] = __builtin_bit_cast(__bf16, qa[q]);
+        bfr[i][q] = __builtin_bit_cast(__bf16, qb[q]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const long gm = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + q;
+        const int gn = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (gm >= M || gn >= N) continue;
+        float v = acc[i][j][q];
+        if (MODE == kFwd) {
+          if (a.bias) v += a.bias[gn];
+          if (a.relu && v < 0.f) v = 0.f;
+          a.out1[gm * a.Cout + gn] = v;
+        } else if (MODE == kBwdData) {
+          if (gn < a.C1) {
+            if (a.out1) a.out1[gm * a.C1 + gn] = v;
+          } else if (a.out2) {
+            a.out2[gm * a.C2 + (gn - a.C1)] = v;
+          }
+        } else {
+          unsafeAtomicAdd(&a.dw[gm * a.Cout + gn], v);
+        }
+      }
+}
+
 // bias gradient: db[co] += sum over pixels of dout[p][co]
 __global__ void k_bias_grad(const float* __restrict__ dout, float* __restrict__ db, long P, int C) {
   const int c = blockIdx.y * 64 + (threadIdx.x & 63);
@@ -355,6 +530,8 @@ using namespace unet;
     if ((x) != hipSuccess) return UNET_E_HIP;        \
   } while (0)
 
+static inline bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
 extern "C" {
 
 int unet_conv3x3_fwd(const float* in1, int C1, const float* in2, int C2, const uint16_t* w, const float* bias,
@@ -376,7 +553,10 @@ int unet_conv3x3_fwd(const float* in1, int C1, const float* in2, int C2, const u
   a.relu = relu;
   const long M = (long)B * H * W, K = 9L * (C1 + C2);
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((Cout + BN - 1) / BN));
-  hipLaunchKernelGGL(k_conv_gemm<kFwd>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, K);
+  if (C1 % 32 == 0 && C2 % 32 == 0 && Cout % 8 == 0 && al16(in1) && al16(in2) && al16(w))
+    hipLaunchKernelGGL(k_conv_gemm_vec<kFwd>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, K);
+  else
+    hipLaunchKernelGGL(k_conv_gemm<kFwd>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, K);
   UN_CHECK(hipGetLastError());
   return 0;
 }
@@ -398,7 +578,10 @@ int unet_conv3x3_bwd_data(const float* dout, int Cout, const uint16_t* w, float*
   const long M = (long)B * H * W, K = 9L * Cout;
   const int N = C1 + C2;
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((N + BN - 1) / BN));
-  hipLaunchKernelGGL(k_conv_gemm<kBwdData>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, N, K);
+  if (Cout % 32 == 0 && al16(dout) && al16(w))
+    hipLaunchKernelGGL(k_conv_gemm_vec<kBwdData>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, N, K);
+  else
+    hipLaunchKernelGGL(k_conv_gemm<kBwdData>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, N, K);
   UN_CHECK(hipGetLastError());
   return 0;
 }
@@ -428,7 +611,10 @@ int unet_conv3x3_bwd_weight(const float* in1, int C1, const float* in2, int C2, 
   splits = (P + per - 1) / per;
   a.pix_per_split = (int)per;
   dim3 grid((unsigned)((M + BM - 1) / BM), (unsigned)((Cout + BN - 1) / BN), (unsigned)splits);
-  hipLaunchKernelGGL(k_conv_gemm<kBwdWeight>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, P);
+  if (C1 % 32 == 0 && C2 % 32 == 0 && Cout % 8 == 0 && al16(in1) && al16(in2) && al16(dout))
+    hipLaunchKernelGGL(k_conv_gemm_vec<kBwdWeight>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, P);
+  else
+    hipLaunchKernelGGL(k_conv_gemm<kBwdWeight>, grid, dim3(NT), 0, (hipStream_t)stream, a, M, Cout, P);
   UN_CHECK(hipGetLastError());
   if (db) {
     dim3 g2((unsigned)std::min<long>((P + 3) / 4, 1024), (unsigned)((Cout + 63) / 64));

@@ -11,6 +11,10 @@ here comes from a closed form computed independently of the oracle:
                       the reference's example data files c/nufft_data.txt and
                       c/fft_data.txt, converted to arrays (read when this script
                       runs in a container that has the reference mounted)
+  * ref_data_files.json
+                      byte size and SHA-256 of those two text files, so the
+                      writers of the data-file formats can be checked to
+                      reproduce them byte for byte from the arrays
 Run:  python tests/golden/make_golden.py
 """
 import os
@@ -126,6 +130,22 @@ def reference_data():
                             v=v.astype(np.int32), y_r=yr, y_i=yi, noise=sd, NX=hdr["NX"], NY=hdr["NY"])
 
 
+def reference_file_digests():
+    import hashlib
+    import json
+
+    out = {}
+    for name in ("nufft_data.txt", "fft_data.txt"):
+        p = os.path.join(REF, "c", name)
+        if os.path.exists(p):
+            b = open(p, "rb").read()
+            out[name] = {"bytes": len(b), "sha256": hashlib.sha256(b).hexdigest()}
+    if out:
+        with open(os.path.join(HERE, "ref_data_files.json"), "w") as f:
+            json.dump(out, f, indent=1, sort_keys=True)
+            f.write("\n")
+
+
 def oracle_refdata_solutions():
     """Optimum of the reference's nufft_data.txt problem with the author's run
     parameters (matlab/testdata4nufft.mat: lambda_l1 = 100, lambda_tsv = 100,
@@ -188,4 +208,5 @@ if __name__ == "__main__":
     fftmode_golden()
     reg_golden()
     reference_data()
+    reference_file_digests()
     print("fixtures written to", HERE)

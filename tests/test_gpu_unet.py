@@ -36,9 +36,14 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_conv3x3_layers_match_oracle():
+@pytest.mark.parametrize("shape", [(2, 12, 10, 5, 3, 70), (2, 12, 10, 32, 32, 40), (3, 9, 11, 64, 32, 64)],
+                         ids=["gather", "vec-fwd-wgrad", "vec-all"])
+def test_conv3x3_layers_match_oracle(shape):
+    """Channel counts that are multiples of 32 take the vectorised staging
+    path (8 channels per load); the others the element gather.  Pixel counts
+    are not multiples of the 64-row tile in any case."""
     rng = np.random.default_rng(1)
-    B, H, W, C1, C2, Co = 2, 12, 10, 5, 3, 70
+    B, H, W, C1, C2, Co = shape
     x1 = rng.standard_normal((B, H, W, C1))
     x2 = rng.standard_normal((B, H, W, C2))
     w = U.to_bf16(rng.standard_normal((3, 3, C1 + C2, Co)) * 0.3)
@@ -48,7 +53,7 @@ def test_conv3x3_layers_match_oracle():
     out = torch.empty((B, H, W, Co), dtype=torch.float32, device=dev)
     g1, g2, gw16 = _t(x1), _t(x2), _bf16_bits(w)
     gb = _t(b)
-    print("case conv3x3 forward / input grad / weight grad, B=2 12x10, Cin=5+3, Cout=70", flush=True)
+    print(f"case conv3x3 forward / input grad / weight grad, B={B} {H}x{W}, Cin={C1}+{C2}, Cout={Co}", flush=True)
     assert L.unet_conv3x3_fwd(g1.data_ptr(), C1, g2.data_ptr(), C2, gw16.data_ptr(), gb.data_ptr(), out.data_ptr(),
                               B, H, W, Co, 0, st) == 0
     xc = np.concatenate([U.to_bf16(x1), U.to_bf16(x2)], axis=3)
