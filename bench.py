@@ -67,7 +67,7 @@ def stage_bytes(N, M, MSP, n_entries):
     }
 
 
-def persistent_bytes(N, M, MSP, n_entries, iters, trials):
+def persistent_bytes(N, M, MSP, n_entries, iters, trials, rounds=None):
     """Algorithmic (compulsory) bytes of one persistent-engine solve (one launch).
 
     P = N^2 pixels, T = N(N+1) row-pass cells, H = 2N(N+1) half-plane grid
@@ -86,6 +86,13 @@ def persistent_bytes(N, M, MSP, n_entries, iters, trials):
       T2 FP64 columns      read T64 (16 B/cell), write Gf64 (16 B/cell); read xnew, z (16 B/px)
       T3 FP64 degrid       read Gf64 (16 B/cell), parameters (56 B/vis), W A z (16 B/vis),
                            write W A xnew (16 B/vis)
+    Per paired round (the trials c and c*ETA together; `rounds` < `trials`):
+      T1                   read g, z once (12 B/px), write two xnew (16 B/px), two T64 (32 B/cell)
+      T2                   read two T64 (32 B/cell), write two Gf64 (32 B/cell); read two
+                           xnew and z (24 B/px)
+      T3                   read two Gf64 (32 B/cell), parameters and W A z once (72 B/vis),
+                           write two W A xnew (32 B/vis)
+    Returns (total, per iteration, per trial round).
     """
     P = N * N
     T = N * (N + 1)
@@ -93,7 +100,10 @@ def persistent_bytes(N, M, MSP, n_entries, iters, trials):
     Hf = 2 * N * (N + MSP)
     per_iter = (32 * P + 96 * M + 4 * n_entries + 8 * H) + (16 * H + 8 * T) + 8 * T
     per_trial = (20 * P + 16 * T) + (16 * T + 16 * Hf + 16 * P) + (16 * Hf + 88 * M)
-    return per_iter * iters + per_trial * trials, per_iter, per_trial
+    if rounds is None or rounds >= trials:
+        return per_iter * iters + per_trial * trials, per_iter, per_trial
+    per_pair = (28 * P + 32 * T) + (32 * T + 32 * Hf + 24 * P) + (32 * Hf + 104 * M)
+    return per_iter * iters + per_pair * rounds, per_iter, per_pair
 
 
 def _batch_vs_single(sa, probs, preset, rel_tol):
@@ -255,7 +265,8 @@ def main():
     value = args.steps / dt
     ITER = res["ITER"]
     trials = info["total_trials"]
-    log(f"bench: ITER={ITER} trials={trials} finalcost={res['finalcost']:.10e} "
+    rounds = info["trial_rounds"]
+    log(f"bench: ITER={ITER} trials={trials} rounds={rounds} finalcost={res['finalcost']:.10e} "
         f"ms/solve={ms_per_step:.2f} iters/s={ITER / (ms_per_step / 1e3):.1f}")
 
     # per-kernel durations of the stream engine's kernels (same problem, HIP events)
@@ -271,13 +282,14 @@ def main():
     if args.engine == "persistent":
         # the dominant (only) kernel is the persistent solver: one launch per solve
         kms = info["kernel_ms"]
-        tot, per_it, per_tr = persistent_bytes(N, M, MSP, n_entries, ITER, trials)
+        tot, per_it, per_tr = persistent_bytes(N, M, MSP, n_entries, ITER, trials, rounds)
         ach = tot / (kms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "kernel": "k_mfista_persistent", "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": MEASURED_TRAFFIC["bytes"], "traffic_solve": MEASURED_TRAFFIC,
                     "bytes_per_launch": tot, "bytes_per_iteration": per_it,
-                    "bytes_per_trial": per_tr, "launch_ms": round(kms, 3)}
+                    "bytes_per_trial_round": per_tr, "iterations": ITER, "trials": trials,
+                    "trial_rounds": rounds, "launch_ms": round(kms, 3)}
     else:
         share = {k: prof[k] * launches[k] for k in prof}
         dom = max(share, key=share.get)

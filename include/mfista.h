@@ -120,6 +120,10 @@ typedef struct mfista_opts {
   int engine;       /* NUFFT solves: MFISTA_ENGINE_PERSISTENT (default, one cooperative
                        launch per solve) or MFISTA_ENGINE_STREAM (one launch per kernel) */
   double kernel_ms; /* out: persistent engine, duration of the solver launch (HIP events) */
+  int trial_rounds; /* out: persistent engine, backtracking rounds evaluated; an L1+TSV round
+                       evaluates the trials c and c*ETA together (MFISTA_PERSISTENT_PAIRED=0:
+                       one trial per round) and takes them in order, as the reference does */
+  int single_trials; /* in: nonzero = one backtracking trial per round (persistent engine) */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);
@@ -170,6 +174,36 @@ typedef struct {
  * the whole batch. */
 int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem *probs,
                                     struct RESULT *results, mfista_opts *opts);
+
+/* Coupled movie: replaces matlab/MFISTA_movie_eht.m:1-239 (the reference's
+ * only movie solver, a MATLAB function over FINUFFT), same arguments in the
+ * same order and meaning.  The N_t = max(time_idx) frames are solved as one
+ * problem: one step-size constant and one momentum sequence, objective
+ *   sum_t |W_t (V_t - A_t x_t)|^2 / 2 + lambda_l1 sum_t |x_t|_1
+ *     + lambda_tsv sum_t TSV(x_t) + lambda_t sum_t w_t |x_t - x_{t+1}|^2,
+ * w_t = 1 / (t_{t+1} - t_t).
+ *   time_idx[M]   frame of each visibility, 1-based (time_idx(:,1))
+ *   time_val[M]   NULL, or observing time of each visibility (time_idx(:,2)):
+ *                 t_t = N_t (mean time of frame t) / (max - min), else t_t = t
+ *   c_box         NULL, or an NX*NY mask multiplied into xinit and the prox
+ *                 output (a zero pixel stays zero; other values are taken as 1)
+ *   xinit         NULL (zeros) or one NX*NY image used for every frame
+ *   xout          N_t * NX * NY doubles, frame-major, x[t][i*NY + j]
+ *   cost          NULL or maxiter doubles: the objective at the start of each
+ *                 iteration (cost(1:t))
+ *   terms         NULL or 4 doubles: sum_t |W_t (V_t - A_t x_t)|^2, sum |x|_1,
+ *                 sum TSV, d_X (the values the reference collects in `error`)
+ *   outvis_r/_i   NULL or M doubles: model visibilities A_t x_t (outvis)
+ * eps is the reference's absolute rule cost[t-TD] - cost[t] < eps (unscaled);
+ * opts->rel_tol > 0 selects the relative rule, opts->preset the gridding
+ * kernel (default C-24, the closest to FINUFFT at 1e-12).  Returns a status. */
+int mfista_movie_nufft(const double *vis_r, const double *vis_i, const double *u_dx,
+                       const double *v_dy, const double *noise_std, const int *time_idx,
+                       const double *time_val, int M, const float *c_box, int NX, int NY,
+                       const double *xinit, double lambda_l1, double lambda_tsv, double lambda_t,
+                       int nonneg_flag, double cinit, int maxiter, double eps, double *xout,
+                       double *cost, double *terms, double *outvis_r, double *outvis_i,
+                       int *iterations, mfista_opts *opts);
 
 /* same arguments as mfista_imaging_core_fft (without fftw_plan_flag) plus options */
 int mfista_imaging_core_fft_ex(const int *u_idx, const int *v_idx, const double *y_r,

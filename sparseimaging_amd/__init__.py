@@ -40,6 +40,7 @@ EXPORTED = [
     "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
     "mfista_amd_version", "mfista_profile_stages", "mfista_last_steps",
     "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
+    "mfista_movie_nufft",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
@@ -73,7 +74,8 @@ class Opts(C.Structure):
     _fields_ = [
         ("preset", C.c_int), ("fgp_iter", C.c_int), ("rel_tol", C.c_double), ("verbose", C.c_int),
         ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
-        ("engine", C.c_int), ("kernel_ms", C.c_double),
+        ("engine", C.c_int), ("kernel_ms", C.c_double), ("trial_rounds", C.c_int),
+        ("single_trials", C.c_int),
     ]
 
 
@@ -126,6 +128,9 @@ def _declare(L):
     L.mfista_nufft_solve_device.argtypes = [C.c_void_p] * 5 + [i, i, i, i, d, d, d, d, d] + [
         C.c_void_p, C.c_void_p, i, i, C.c_void_p, C.POINTER(RESULT), C.POINTER(Opts), C.c_void_p]
     L.mfista_nufft_forward.restype = i
+    L.mfista_movie_nufft.argtypes = [_dp, _dp, _dp, _dp, _dp, _ip, _dp, i, _fp, i, i, _dp, d, d, d, i, d, i, d,
+                                     _dp, _dp, _dp, _dp, _dp, _ip, C.POINTER(Opts)]
+    L.mfista_movie_nufft.restype = i
     L.mfista_nufft_forward.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
     L.mfista_nufft_adjoint.restype = i
     L.mfista_nufft_adjoint.argtypes = [i, i, _dp, _dp, i, i, i, _dp, _dp, _dp]
@@ -162,24 +167,28 @@ def _box(cl_box, n):
     return 1, b, b.ctypes.data_as(_fp)
 
 
-def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT):
+def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT,
+              single_trials=False):
     o = Opts()
     lib().mfista_default_opts(C.byref(o))
     o.preset, o.fgp_iter, o.rel_tol, o.verbose, o.engine = preset, fgp_iter, rel_tol, verbose, engine
+    o.single_trials = 1 if single_trials else 0
     return o
 
 
 def _info(o):
     return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials,
-            "kernel_ms": o.kernel_ms}
+            "kernel_ms": o.kernel_ms, "trial_rounds": o.trial_rounds}
 
 
 # ---------------------------------------------------------------- solvers
 def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
                               nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
-                              rel_tol=0.0, engine=ENGINE_PERSISTENT):
-    """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts)."""
+                              rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False):
+    """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts).
+    single_trials=True evaluates one backtracking trial per round in the persistent
+    engine (default: L1+TSV rounds evaluate the trials c and c*ETA together)."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
     vr, pvr = _d(vis_r)
@@ -191,7 +200,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     xout = np.zeros(NX * NY)
     bflag, b, pb = _box(cl_box, NX * NY)
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol, engine=engine)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials)
     rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
                                             lambda_tv, lambda_tsv, cinit, px0,
                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
@@ -201,7 +210,8 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     return xout.reshape(NX, NY), res.as_dict(), _info(o)
 
 
-def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0):
+def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0,
+                                    single_trials=False):
     """Independent NUFFT solves in one persistent launch (no reference counterpart:
     a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
     the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
@@ -248,12 +258,58 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
         keep.append(b)
         q.box_flag, q.cl_box = bflag, pb
     res = (RESULT * n)()
-    o = make_opts(preset, fgp_iter, rel_tol)
+    o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials)
     rc = lib().mfista_imaging_core_nufft_batch(n, arr, res, C.byref(o))
     if rc != 0:
         raise MfistaError(rc)
     xs = [xo.reshape(int(p["NX"]), int(p["NY"])) for xo, p in zip(outs, problems)]
     return xs, [r.as_dict() for r in res], _info(o)
+
+
+def mfista_movie_nufft(vis, u, v, noise_std, time_idx, c_box, Nx, Ny, xinit, lambda_l1, lambda_tsv, lambda_t,
+                       nonneg_flag=1, cinit=1e4, maxiter=MAXITER, eps=EPS, preset=PRESET_C24, rel_tol=0.0,
+                       single_trials=False):
+    """Coupled movie solve on the GPU with the arguments of
+    matlab/MFISTA_movie_eht.m:1-4 (vis complex; time_idx an (M,) array of
+    1-based frame numbers or an (M, 2) array [frame, time]; c_box None or an
+    (Nx, Ny) mask; xinit None or one (Nx, Ny) image).  Returns
+    (x[N_t, Nx, Ny], cost, terms, outvis, info): cost the objective at the
+    start of each iteration, terms (chisq, l1, TSV, d_X), outvis the complex
+    model visibilities A_t x_t in input order."""
+    vis = np.asarray(vis)
+    vr, pvr = _d(vis.real)
+    vi, pvi = _d(vis.imag)
+    uu, pu = _d(u)
+    vv, pv = _d(v)
+    sd, psd = _d(noise_std)
+    M = uu.size
+    ti = np.asarray(time_idx)
+    if ti.ndim == 2:
+        frames, tval = ti[:, 0], np.ascontiguousarray(ti[:, 1], dtype=np.float64)
+        ptv = tval.ctypes.data_as(_dp)
+    else:
+        frames, tval, ptv = ti, None, None
+    fi = np.ascontiguousarray(frames, dtype=np.int32)
+    nt = int(fi.max())
+    box = None if c_box is None else np.ascontiguousarray(c_box, dtype=np.float32).ravel()
+    x0 = None if xinit is None else np.ascontiguousarray(xinit, dtype=np.float64).ravel()
+    xout = np.zeros(nt * Nx * Ny)
+    cost = np.zeros(int(maxiter))
+    terms = np.zeros(4)
+    om_r, om_i = np.zeros(M), np.zeros(M)
+    iters = C.c_int(0)
+    o = make_opts(preset, FGPITER, rel_tol, single_trials=single_trials)
+    rc = lib().mfista_movie_nufft(pvr, pvi, pu, pv, psd, fi.ctypes.data_as(_ip), ptv, M,
+                                  None if box is None else box.ctypes.data_as(_fp), Nx, Ny,
+                                  None if x0 is None else x0.ctypes.data_as(_dp), lambda_l1, lambda_tsv,
+                                  lambda_t, int(nonneg_flag), cinit, int(maxiter), eps, xout.ctypes.data_as(_dp),
+                                  cost.ctypes.data_as(_dp), terms.ctypes.data_as(_dp), om_r.ctypes.data_as(_dp),
+                                  om_i.ctypes.data_as(_dp), C.byref(iters), C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    info = _info(o)
+    info["iterations"] = iters.value
+    return (xout.reshape(nt, Nx, Ny), cost[:iters.value].copy(), terms, om_r + 1j * om_i, info)
 
 
 def mfista_imaging_core_fft(u_idx, v_idx, y_r, y_i, noise_stdev, NX, NY, maxiter=MAXITER, eps=EPS,
@@ -308,7 +364,7 @@ def mfista_imaging_core_dft(y, A, NX, NY, maxiter=MAXITER, eps=EPS, lambda_l1=1.
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
                        lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
                        t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None,
-                       engine=ENGINE_PERSISTENT):
+                       engine=ENGINE_PERSISTENT, single_trials=False):
     """HBM-resident solve: every array argument is a device pointer (int) or a
     torch CUDA tensor (float64; cl_box float32).  Returns (RESULT dict, opts dict)."""
 
@@ -321,7 +377,7 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
     if M is None:
         raise ValueError("pass torch tensors (M is taken from t_u)")
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol, engine=engine)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials)
     rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
                                          maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
                                          ptr(t_xinit), ptr(t_xout), nonneg_flag,

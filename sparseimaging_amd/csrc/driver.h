@@ -25,6 +25,7 @@ struct RunParams {
   int NX = 0, NY = 0, maxiter = 0, nonneg = 0, box_flag = 0, tv_mode = 0, fgp_iter = 100;
   double eps_abs = 0, rel_tol = 0, lambda_l1 = 0, lambda_tv = 0, lambda_tsv = 0, cinit = 0;
   double eta_first = 1.1;
+  int single_trials = 0;          // persistent engine: one backtracking trial per round
   const double* xinit = nullptr;  // device, FP64
   double* xout = nullptr;         // device, FP64
   const float* box = nullptr;     // device
@@ -35,6 +36,7 @@ struct RunStats {
   double c = 0;
   double kernel_ms = 0;  // persistent engine: duration of the solver launch
   long steps = 0;
+  int rounds = 0;        // persistent engine: trial rounds (one or two trials each)
 };
 
 template <typename R>
@@ -44,6 +46,10 @@ int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms);
+// the frames of a coupled movie in one launch with one control step (persistent.hip)
+int solve_nufft_movie(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                      double lambda_t, const std::vector<double>& w, std::vector<RunStats>& out,
+                      float* kernel_ms);
 
 // FGP TV prox kernels (reference c/mfista_TV_lib.c:189-384)
 template <typename R>

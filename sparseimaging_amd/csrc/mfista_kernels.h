@@ -32,6 +32,7 @@ struct DevState {
   int iter, trial, done, ITER, maxiter, nonneg, tv_mode, max_trials, accept_x, status;
   int total_trials, lip_mode;
   int gate_prologue, gate_trial, gate_momentum;  // nonzero: the kernel returns at once
+  int rounds;                                    // persistent engine: trial rounds evaluated
 };
 
 // offsets (in doubles) of the partial-sum regions written by the kernels

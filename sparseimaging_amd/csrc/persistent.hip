@@ -40,13 +40,15 @@ struct PArgs {
   const float *e4xf, *e4yf;
   const double *e4xd, *e4yd;
   float2 *Hg, *U;
-  double2 *T64, *Gf64;
-  double2* vm[3];  // weighted model vectors W A (.), roles rotate on acceptance
+  double2 *T64, *Gf64;    // FP64 forward transform of the step (first trial of a round)
+  double2 *T64b, *Gf64b;  // the same for the second trial of a paired round
+  double2* vm[4];         // weighted model vectors W A (.), roles rotate on acceptance
   const int4* chunks;
   int nchunks;
   const int* granges;  // chunk range [granges[b], granges[b+1]) of workgroup b
   const unsigned* ent;
-  double *x, *xnew, *z;  // iterates in FP64 (FP32 iterates stagnate near a tight tolerance)
+  double *x, *z;  // iterates in FP64 (FP32 iterates stagnate near a tight tolerance)
+  double* xn[2];  // xnew of the first and second trial of a round
   float* gr;
   const float* box;
   float *rA, *sA, *rB, *sB, *pp, *qq;
@@ -61,6 +63,17 @@ struct PArgs {
   DevState* S;
   unsigned long long* bar;  // barrier words, see BarCtx
   int hier_barrier;         // XCD-hierarchical grid barrier
+  int pair;                 // L1+TSV: evaluate two backtracking trials per round
+  // Coupled movie (frames of one solve, matlab/MFISTA_movie_eht.m): every
+  // frame is one PArgs of a batch launch; all frames share one barrier, one
+  // partial-sum table and so one control step.  ngb = 0: an ordinary solve.
+  int ngb;                   // workgroups of all frames (barrier, partial rows)
+  int gb0;                   // partial row of this frame's workgroup 0
+  double lambda_t;           // weight of d_X = sum_t w_t |x_t - x_{t+1}|^2
+  double wt_prev, wt_next;   // w_{t-1}, w_t (0 at the first / last frame)
+  const double *z_prev, *z_next;  // the neighbouring frames' z (nullptr at the ends)
+  const double* x_next;           // the next frame's x (initial cost)
+  const double* xn_next[2];       // the next frame's trial images
 };
 
 // Grid-wide barriers for a cooperative launch.  bar[] layout (64-bit words):
@@ -209,10 +222,12 @@ __device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, unsigned nb
 // Per-workgroup partial sums: one row of kPW doubles per workgroup, written
 // by the stage that produces them and reduced (in workgroup order, the same
 // in every workgroup) by the control step.
-constexpr int kPW = 8;
-enum { kPD2 = 0, kPL1 = 1, kPReg = 2, kPTsvd = 3, kPChi = 4, kPLip = 5 };
-// init-phase slots reuse the same row: |x|_1, TSV(x), TV(x), chi^2(x)
-enum { kPIl1 = 0, kPItsv = 1, kPItv = 2, kPIchi = 4 };
+constexpr int kPW = 16;
+// trial k of a round uses slots 6 k + (0 .. 5)
+// (kPTx, kPTd: this frame's share of d_X(xnew) and d_X(d) in a coupled movie)
+enum { kPD2 = 0, kPL1 = 1, kPReg = 2, kPTsvd = 3, kPChi = 4, kPLip = 5, kPTx = 6, kPTd = 7, kPTrial = 8 };
+// init-phase slots reuse the same row: |x|_1, TSV(x), TV(x), d_X(x) share, chi^2(x)
+enum { kPIl1 = 0, kPItsv = 1, kPItv = 2, kPIdx = 3, kPIchi = 4 };
 
 template <int NV>
 __device__ inline void part_reduce(const double* part, int nb, const int (&slot)[NV], double (&out)[NV],
@@ -231,6 +246,9 @@ struct BlockState {
   double c, mu, costtmp, l1_x, mom_a, mom_b;
   int iter, trial, accept, mom_pending, stop, ITER, max_trials, total_trials, status, prologue;
   int ix, in, iz;  // roles of the model vectors: W A x, W A xnew, W A z
+  int in1;         // W A xnew of the second trial of a paired round
+  int kacc;        // trial of the last round that was accepted (0 or 1): xn[kacc]
+  int rounds;      // trial rounds evaluated
 };
 
 // regulariser sums at pixel f over its right and lower neighbours:
@@ -265,6 +283,17 @@ __device__ inline void reg_terms(const double* xn, const double* z, int f, int W
   }
 }
 
+
+// this frame's share of the temporal terms of trial k (coupled movie): r[0] +=
+// w_t (xnew_t - xnew_{t+1})^2 and r[1] += w_t (d_t - d_{t+1})^2 at pixel f,
+// d = xnew - z (matlab/lib/d_X.m)
+template <typename PA>
+__device__ inline void temporal_terms(const PA& A, int k, int f, double r[2]) {
+  const double a = A.xn[k][f], b = A.xn_next[k][f];
+  const double e = a - b, d = (a - A.z[f]) - (b - A.z_next[f]);
+  r[0] += A.wt_next * e * e;
+  r[1] += A.wt_next * d * d;
+}
 
 // soft threshold (c/mfista_tools.c:81-141): v -> 0 on the box and for |v| <= eta
 template <typename R>
@@ -307,24 +336,29 @@ __device__ inline float lpq_f(const float* p, const float* q, int i, int j, int 
 // entry of the chunk onto them, so there are no LDS or intra-tile atomics and
 // the sums stay in registers across consecutive chunks of one tile.  The
 // entries of up to kGB chunks are loaded at once, one thread per entry, into
-// LDS (cell, tap slopes, deposit value E1 w conj(r)); each is then read by all
-// lanes as a broadcast.  A tile is flushed when it changes: with plain stores
+// LDS records (tap offsets relative to the tile, tap slopes, deposit value
+// E1 w conj(r)); each record is then read by all lanes as a broadcast.  A tile is flushed when it changes: with plain stores
 // when this workgroup processed all of the tile's chunks (Hg is zero there),
 // otherwise with one global atomic per touched cell.  Deposits as st_grid
 // (stages.h): visibility s at taps (l, l') with weight
 // exp(l a_x - l^2 t3x + l' a_y - l'^2 t3y), conjugated at -k for mirror entries.
 constexpr int kGB = MF_BLOCK / kGridChunk;
 
+// One gridding entry as the gather loop reads it: the tap offsets of the
+// tile's cell (0, 0), the direction (+1 direct, -1 mirror), the tap slopes and
+// the deposit; 32 bytes, read with two 16-byte LDS loads.
+struct alignas(16) GridEnt {
+  int oy, ox, sg, pad;
+  float ax, ay, dx, dy;
+};
+
 template <int CPT, typename Resid>
 __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp, Resid resid,
                                    const int4* __restrict__ chunks, int cb, int ce,
                                    const unsigned* __restrict__ ent, float2* __restrict__ Hg, char* lds) {
   const int TX = g.TX, TY = g.TY;
-  int2* em = reinterpret_cast<int2*>(lds);
-  float2* ea = reinterpret_cast<float2*>(em + MF_BLOCK);
-  float2* ed = ea + MF_BLOCK;
-  int* eflag = reinterpret_cast<int*>(ed + MF_BLOCK);  // -1 empty, 0 direct, 1 mirror
-  int4* cmeta = reinterpret_cast<int4*>(eflag + MF_BLOCK);  // kGB chunk records
+  GridEnt* er = reinterpret_cast<GridEnt*>(lds);
+  int4* cmeta = reinterpret_cast<int4*>(er + MF_BLOCK);  // kGB chunk records
   const int MSP = g.MSP, NX = g.NX;
   const float t3x = (float)g.t3x, t3y = (float)g.t3y;
   // the cells this thread owns: (rr[c], cc[c]) within the tile
@@ -361,7 +395,6 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
     __syncthreads();  // the previous batch's entries are consumed
     {
       const int q = threadIdx.x, j = q / kGridChunk;
-      int flag = -1;
       if (j < nc) {
         const int4 ch = chunks[c0 + j];
         if ((q % kGridChunk) == 0) cmeta[j] = ch;
@@ -369,15 +402,27 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
         if (e < ch.z) {
           const unsigned en = ent[e];
           const int sv = (int)(en & 0x7fffffffu);
-          flag = (int)(en >> 31);
-          em[q] = vp.mxy[sv];
-          ea[q] = vp.a[sv];
+          const bool mir = (en >> 31) != 0;
+          const int2 m = vp.mxy[sv];
+          const float2 a = vp.a[sv];
           const float ew = vp.e1[sv] * vp.w[sv];
           const float2 r = resid(sv);
-          ed[q] = make_float2(ew * r.x, flag ? ew * r.y : -ew * r.y);
+          const int tr = ch.x / g.ntx, tc = ch.x - tr * g.ntx;
+          const int trb = tr * TY, tcb = tc * TX;
+          // tap offsets of cell (row, col): direct l' = row - my, l = col - mx - NX;
+          // mirror l' = -row - my, l = NX - mx - col
+          GridEnt ge;
+          ge.oy = mir ? -(trb + m.y) : trb - m.y;
+          ge.ox = mir ? NX - m.x - tcb : tcb - m.x - NX;
+          ge.sg = mir ? -1 : 1;
+          ge.pad = 0;
+          ge.ax = a.x;
+          ge.ay = a.y;
+          ge.dx = ew * r.x;
+          ge.dy = mir ? ew * r.y : -ew * r.y;
+          er[q] = ge;
         }
       }
-      eflag[q] = flag;
     }
     __syncthreads();
     for (int j = 0; j < nc; j++) {
@@ -390,26 +435,19 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
         rb = tr * TY;
         cbx = tc * TX;
       }
-      for (int q = j * kGridChunk; q < (j + 1) * kGridChunk; q++) {
-        const int flag = eflag[q];
-        if (flag < 0) break;  // a chunk's entries are contiguous from its start
-        const int2 m = em[q];
-        const float2 a = ea[q];
-        const float2 d = ed[q];
-        // tap offsets of cell (row, col): direct l' = row - my, l = col - mx - NX;
-        // mirror l' = -row - my, l = NX - mx - col
-        const int oy = flag ? -(rb + m.y) : rb - m.y;
-        const int ox = flag ? NX - m.x - cbx : cbx - m.x - NX;
-        const int sg = flag ? -1 : 1;
+      const int q0 = j * kGridChunk, q1 = q0 + min(kGridChunk, ch.z - ch.y);
+#pragma unroll 4
+      for (int q = q0; q < q1; q++) {
+        const GridEnt ge = er[q];
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-          const int lp = oy + sg * rr[c];
-          const int l = ox + sg * cc[c];
+          const int lp = ge.oy + ge.sg * rr[c];
+          const int l = ge.ox + ge.sg * cc[c];
           if (lp > -MSP && lp <= MSP && l > -MSP && l <= MSP) {
             const float fl = (float)l, flp = (float)lp;
-            const float wgt = dev_exp(fl * a.x - fl * fl * t3x + flp * a.y - flp * flp * t3y);
-            acc[c].x += wgt * d.x;
-            acc[c].y += wgt * d.y;
+            const float wgt = dev_exp(fl * ge.ax - fl * fl * t3x + flp * ge.ay - flp * flp * t3y);
+            acc[c].x += wgt * ge.dx;
+            acc[c].y += wgt * ge.dy;
           }
         }
       }
@@ -423,11 +461,11 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
   __syncthreads();
 }
 
-// LDS bytes of the batched gridder's entry arrays
+// LDS bytes of the batched gridder's entry records
 __host__ __device__ inline size_t grid_lds_bytes(int TX, int TY) {
   (void)TX;
   (void)TY;
-  return (size_t)MF_BLOCK * (8 + 8 + 8 + 4) + (size_t)kGB * 16;
+  return (size_t)MF_BLOCK * sizeof(GridEnt) + (size_t)kGB * 16;
 }
 
 // LDS bytes of the staged twiddle (FP32 and FP64, both axes) and E4 tables
@@ -449,9 +487,11 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   const bool tsv = A.lambda_tsv > 0.0;
   const bool tv = A.tv_mode != 0;
   unsigned long long epoch = 0;  // barriers passed by this workgroup
-  double* prow = A.part + (size_t)lb * kPW;
+  const bool movie = A.ngb > 0;
+  const unsigned ngrp = movie ? (unsigned)A.ngb : nb;  // workgroups of the barrier and the reductions
+  double* prow = A.part + (size_t)(movie ? A.gb0 + (int)lb : (int)lb) * kPW;
 
-  if (!bar_setup(B, A.bar, nb, A.hier_barrier)) return;
+  if (!bar_setup(B, A.bar, ngrp, A.hier_barrier)) return;
 
   // LDS: [tables][stage scratch]
   char* lds = reinterpret_cast<char*>(smem_d4);
@@ -501,12 +541,17 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   // ---------------- initial cost and models: W A x0 -> vm[0] (x), vm[2] (z)
   {
     double st[4] = {0, 0, 0, 0};
+    double sdx[1] = {0};
     for (int t = lb; t < A.n_row; t += nb) {
       auto load = [&](int i, int j) -> double {
         const int f = i * NY + j;
         const double xc = A.x[f];
         A.z[f] = xc;
         st[0] += fabs(xc);
+        if (A.x_next) {
+          const double e = xc - A.x_next[f];
+          sdx[0] += A.wt_next * e * e;
+        }
         const int c = f % W, r = f / W;
         double dh = 0, dv = 0;
         if (c < W - 1) dh = xc - A.x[f + 1];
@@ -523,10 +568,12 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, sm64, load);
     }
     block_sum<4>(st, red);
+    block_sum<1>(sdx, red);
     if (threadIdx.x == 0) {
       prow[kPIl1] = st[0];
       prow[kPItsv] = st[1];
       prow[kPItv] = st[2];
+      prow[kPIdx] = sdx[0];
     }
   }
   if (!grid_sync(B, epoch)) return;
@@ -541,15 +588,16 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   }
   if (!grid_sync(B, epoch)) return;
   {
-    const int slot[4] = {kPIchi, kPIl1, kPItsv, kPItv};
-    double sum[4];
-    part_reduce<4>(A.part, nb, slot, sum, red);
+    const int slot[5] = {kPIchi, kPIl1, kPItsv, kPItv, kPIdx};
+    double sum[5];
+    part_reduce<5>(A.part, ngrp, slot, sum, red);
     if (threadIdx.x == 0) {
       double c0 = sum[0] / 2 + A.lambda_l1 * sum[1];
       if (tv)
         c0 += A.lambda_tv * sum[3];
       else if (tsv)
         c0 += A.lambda_tsv * sum[2];
+      if (movie) c0 += A.lambda_t * sum[4];
       bs.c = A.cinit;
       bs.mu = 1.0;
       bs.costtmp = c0;
@@ -567,6 +615,9 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       bs.ix = 0;
       bs.in = 1;
       bs.iz = 2;
+      bs.in1 = 3;
+      bs.kacc = 0;
+      bs.rounds = 0;
     }
     __syncthreads();
   }
@@ -579,12 +630,14 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const bool mom = bs.mom_pending != 0;
       const double ma = bs.mom_a, mb = bs.mom_b;
       const bool acc = bs.accept != 0;
+      const int ka = bs.kacc;  // the accepted trial's xnew and model
+      const double* xnw = A.xn[ka];
       const double2* mx = A.vm[bs.ix];
-      const double2* mn = A.vm[bs.in];
+      const double2* mn = A.vm[ka ? bs.in1 : bs.in];
       double2* mz = A.vm[bs.iz];
       if (mom) {
         for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-          const double xf = A.x[f], xn = A.xnew[f];
+          const double xf = A.x[f], xn = xnw[f];
           A.z[f] = ma * xf + mb * xn;
           if (acc) A.x[f] = xn;
         }
@@ -609,9 +662,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
                                          A.Hg, lds);
       if (!grid_sync(B, epoch)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles
+        int& rn = ka ? bs.in1 : bs.in;
         const int tmp = bs.ix;
-        bs.ix = bs.in;
-        bs.in = tmp;
+        bs.ix = rn;
+        rn = tmp;
       }
       // P2: inverse column pass (clears Hg behind it)
       for (int t = lb; t < A.n_ky; t += nb)
@@ -638,6 +692,13 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         if (r < H - 1) d += zc - A.z[f + W];
         if (r > 0) d += zc - A.z[f - W];
         gv = gv - lt * (float)(2.0 * d);
+      }
+      if (movie) {  // - lambda_t dt_X(z) (matlab/lib/dt_X.m)
+        const double zc = A.z[f];
+        double d = 0.0;
+        if (A.z_next) d += A.wt_next * (zc - A.z_next[f]);
+        if (A.z_prev) d += A.wt_prev * (zc - A.z_prev[f]);
+        gv = gv - (float)(A.lambda_t * 2.0 * d);
       }
       A.gr[f] = gv;
     };
@@ -705,8 +766,42 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
     }
     // T1: [c2r + gradient on the first trial] -> xnew (prox or final FGP step),
-    // d = xnew - z (FP64), partials, FP64 forward row pass of d
-    {
+    // d = xnew - z (FP64), partials, FP64 forward row pass of d.  A paired
+    // round (L1+TSV) computes the steps of c and c * eta side by side.
+    const bool pr = A.pair && !tv;
+    if (pr) {
+      const double etaf = (bs.iter == 0) ? A.eta_first : kEta;
+      const double c1 = c * etaf;  // the value the control step gives c after a failed trial
+      const double ci0 = 1.0 / c, e0 = A.lambda_l1 / c, ci1 = 1.0 / c1, e1 = A.lambda_l1 / c1;
+      double v[4] = {0, 0, 0, 0};
+      for (int t = lb; t < A.n_row; t += nb) {
+        if (trial_first) st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+        auto load2 = [&](int i, int j, double& o0, double& o1) {
+          const int f = i * NY + j;
+          const double zf = A.z[f], gf = (double)A.gr[f];
+          const bool boxed = A.box && A.box[f] == 0.0f;
+          const double xa = prox_p<double>(gf * ci0 + zf, e0, A.nonneg, boxed);
+          const double xb = prox_p<double>(gf * ci1 + zf, e1, A.nonneg, boxed);
+          A.xn[0][f] = xa;
+          A.xn[1][f] = xb;
+          const double da = xa - zf, db = xb - zf;
+          v[0] += da * da;
+          v[1] += fabs(xa);
+          v[2] += db * db;
+          v[3] += fabs(xb);
+          o0 = da * e4xd[i] * e4yd[j];
+          o1 = db * e4xd[i] * e4yd[j];
+        };
+        st_rowfft_fwd_pair<double>(t, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, A.T64b, sm64, load2);
+      }
+      block_sum<4>(v, red);
+      if (threadIdx.x == 0) {
+        prow[kPD2] = v[0];
+        prow[kPL1] = v[1];
+        prow[kPTrial + kPD2] = v[2];
+        prow[kPTrial + kPL1] = v[3];
+      }
+    } else {
       double v[2] = {0, 0};
       const double cinv_d = 1.0 / c, eta_d = A.lambda_l1 / c, lam_d = A.lambda_tv / c;
       const double bshift = (tv && A.nonneg == 1) ? -eta_d : 0.0;
@@ -727,7 +822,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
             const double L = have_dual ? (double)lpq_f(A.pp, A.qq, ii, jj, W, H) : 0.0;
             xn = fgp_proj_d(bb - lam_d * L, eta_d, A.nonneg == 1, boxed);
           }
-          A.xnew[f] = xn;
+          A.xn[0][f] = xn;
           const double d = xn - zf;
           v[0] += d * d;
           v[1] += fabs(xn);
@@ -743,109 +838,176 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     }
     if (!grid_sync(B, epoch)) return;
     // T2: FP64 column pass of d; regulariser partials of xnew and d
-    for (int t = lb; t < A.n_ky; t += nb)
-      st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
-    {
-      double r[2] = {0, 0};
-      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x)
-        reg_terms(A.xnew, A.z, f, W, H, tv, r);
-      block_sum<2>(r, red);
+    if (pr) {
+      for (int t = lb; t < A.n_ky; t += nb)
+        st_colfft_fwd_pair<double>(t, A.T64, A.T64b, A.Gf64, A.Gf64b, NX, NY + 1, g.Lx, NX / 2, g.MSP, A.Px,
+                                   twxd, sm64);
+      double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+        reg_terms(A.xn[0], A.z, f, W, H, false, r);
+        reg_terms(A.xn[1], A.z, f, W, H, false, r + 2);
+        if (A.x_next) {
+#pragma unroll
+          for (int k = 0; k < 2; k++) temporal_terms(A, k, f, r + 4 + 2 * k);
+        }
+      }
+      block_sum<8>(r, red);
       if (threadIdx.x == 0) {
         prow[kPReg] = r[0];
         prow[kPTsvd] = r[1];
+        prow[kPTrial + kPReg] = r[2];
+        prow[kPTrial + kPTsvd] = r[3];
+        prow[kPTx] = r[4];
+        prow[kPTd] = r[5];
+        prow[kPTrial + kPTx] = r[6];
+        prow[kPTrial + kPTd] = r[7];
+      }
+    } else {
+      for (int t = lb; t < A.n_ky; t += nb)
+        st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
+      double r[4] = {0, 0, 0, 0};
+      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+        reg_terms(A.xn[0], A.z, f, W, H, tv, r);
+        if (A.x_next) temporal_terms(A, 0, f, r + 2);
+      }
+      block_sum<4>(r, red);
+      if (threadIdx.x == 0) {
+        prow[kPReg] = r[0];
+        prow[kPTsvd] = r[1];
+        prow[kPTx] = r[2];
+        prow[kPTd] = r[3];
       }
     }
     if (!grid_sync(B, epoch)) return;
     // T3: FP64 degrid of d: W A xnew = W A z + W A d, chi^2(xnew), |W A d|^2
     {
       const double2* mz = A.vm[bs.iz];
-      double2* mn = A.vm[bs.in];
-      double v[2] = {0, 0};
-      for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, mz, mn, nullptr, v);
-      block_sum<2>(v, red);
-      if (threadIdx.x == 0) {
-        prow[kPChi] = v[0];
-        prow[kPLip] = v[1];
+      if (pr) {
+        double v[4] = {0, 0, 0, 0};
+        for (int t = lb; t < A.n_deg; t += nb)
+          st_degrid_model_pair<MSPC>(t, g, A.pd, A.Gf64, A.Gf64b, mz, A.vm[bs.in], A.vm[bs.in1], v);
+        block_sum<4>(v, red);
+        if (threadIdx.x == 0) {
+          prow[kPChi] = v[0];
+          prow[kPLip] = v[1];
+          prow[kPTrial + kPChi] = v[2];
+          prow[kPTrial + kPLip] = v[3];
+        }
+      } else {
+        double v[2] = {0, 0};
+        for (int t = lb; t < A.n_deg; t += nb)
+          st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, mz, A.vm[bs.in], nullptr, v);
+        block_sum<2>(v, red);
+        if (threadIdx.x == 0) {
+          prow[kPChi] = v[0];
+          prow[kPLip] = v[1];
+        }
       }
     }
     if (!grid_sync(B, epoch)) return;
 
     // ---------------- control (every workgroup, identical arithmetic)
-    const int slot[6] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd};
-    double sum[6];
-    part_reduce<6>(A.part, nb, slot, sum, red);
+    // The trials of a round are taken in order, exactly as the reference's
+    // backtracking loop (c/mfista_nufft_lib.c:692-723) takes them: the first
+    // that passes is accepted; one that fails multiplies c by eta.
+    double sum[2 * kPTrial];
+    if (pr) {
+      const int sl[2 * kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd,
+                                   kPTrial + kPChi, kPTrial + kPLip, kPTrial + kPD2, kPTrial + kPL1,
+                                   kPTrial + kPReg, kPTrial + kPTsvd, kPTrial + kPTx, kPTrial + kPTd};
+      part_reduce<2 * kPTrial>(A.part, ngrp, sl, sum, red);
+    } else {
+      const int sl[kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd};
+      double s1[kPTrial];
+      part_reduce<kPTrial>(A.part, ngrp, sl, s1, red);
+#pragma unroll
+      for (int q = 0; q < kPTrial; q++) sum[q] = s1[q];
+    }
     if (threadIdx.x == 0) {
-      const double chi_x = sum[0], lip = sum[1], s_d2 = sum[2], l1n = sum[3], regn = sum[4],
-                   tsvd = sum[5];
       const int it = bs.iter;
       if (prologue) {
         ring[it % (kTD + 1)] = bs.costtmp;
         if (lb == 0) A.cost[it] = bs.costtmp;
         bs.prologue = 0;
       }
-      double F = chi_x / 2 + ((!tv && tsv) ? A.lambda_tsv * regn : 0.0);
-      const double lhs = lip / 2 + ((!tv && tsv) ? A.lambda_tsv * tsvd : 0.0);
-      const double rhs = bs.c * s_d2 / 2;
-      bs.trial += 1;
-      bs.total_trials += 1;
+      bs.rounds += 1;
       const double etaf = (it == 0) ? A.eta_first : kEta;
       const long cap = (long)A.maxiter * 64 + 100000;
-      if (bs.total_trials > cap) {  // cannot happen for a convergent run; bounds the launch
-        bs.status = 2;
-        bs.stop = 1;
-        bs.ITER = it + 1;
-        bs.mom_pending = 0;
-      } else if (!isfinite(F) || !isfinite(lhs)) {
-        bs.status = 1;
-        bs.stop = 1;
-        bs.ITER = it + 1;
-        bs.mom_pending = 0;
-      } else {
+      const int K = pr ? 2 : 1;
+      for (int k = 0; k < K; k++) {
+        const double* sk = sum + k * kPTrial;
+        const double chi_x = sk[0], lip = sk[1], s_d2 = sk[2], l1n = sk[3], regn = sk[4], tsvd = sk[5];
+        double F = chi_x / 2 + ((!tv && tsv) ? A.lambda_tsv * regn : 0.0);
+        double lhs = lip / 2 + ((!tv && tsv) ? A.lambda_tsv * tsvd : 0.0);
+        if (movie) {  // + lambda_t d_X(xnew); d_X is quadratic, so its step term is d_X(d)
+          F += A.lambda_t * sk[6];
+          lhs += A.lambda_t * sk[7];
+        }
+        const double rhs = bs.c * s_d2 / 2;
+        bs.trial += 1;
+        bs.total_trials += 1;
+        if (bs.total_trials > cap) {  // cannot happen for a convergent run; bounds the launch
+          bs.status = 2;
+          bs.stop = 1;
+          bs.ITER = it + 1;
+          bs.mom_pending = 0;
+          break;
+        }
+        if (!isfinite(F) || !isfinite(lhs)) {
+          bs.status = 1;
+          bs.stop = 1;
+          bs.ITER = it + 1;
+          bs.mom_pending = 0;
+          break;
+        }
         const bool ok = lhs <= rhs;
         if (!ok && bs.trial < A.maxiter) {
           bs.c *= etaf;
+          continue;  // the next trial of this round, or a new round
+        }
+        if (!ok) bs.c *= etaf;
+        if (bs.trial > bs.max_trials) bs.max_trials = bs.trial;
+        bs.kacc = k;
+        bs.c /= kEta;
+        const double mu = bs.mu;
+        const double munew = (1 + sqrt(1 + 4 * mu * mu)) / 2;
+        F += A.lambda_l1 * l1n;
+        if (tv) F += A.lambda_tv * regn;
+        bool stop = false;
+        const double cit = ring[it % (kTD + 1)];
+        if (F < cit) {
+          bs.costtmp = F;
+          bs.mom_a = (1 - mu) / munew;
+          bs.mom_b = 1 + ((mu - 1) / munew);
+          bs.accept = 1;
+          bs.l1_x = l1n;
         } else {
-          if (!ok) bs.c *= etaf;
-          if (bs.trial > bs.max_trials) bs.max_trials = bs.trial;
-          bs.c /= kEta;
-          const double mu = bs.mu;
-          const double munew = (1 + sqrt(1 + 4 * mu * mu)) / 2;
-          F += A.lambda_l1 * l1n;
-          if (tv) F += A.lambda_tv * regn;
-          bool stop = false;
-          const double cit = ring[it % (kTD + 1)];
-          if (F < cit) {
-            bs.costtmp = F;
-            bs.mom_a = (1 - mu) / munew;
-            bs.mom_b = 1 + ((mu - 1) / munew);
-            bs.accept = 1;
-            bs.l1_x = l1n;
-          } else {
-            bs.mom_a = 1 - (mu / munew);
-            bs.mom_b = mu / munew;
-            bs.accept = 0;
-            if (it > 1 && bs.l1_x == 0) stop = true;
-          }
-          if (!stop && it >= kMinIter) {
-            const double d = ring[(it - kTD) % (kTD + 1)] - cit;
-            stop = (A.rel_tol > 0) ? (d <= A.rel_tol * cit) : (d < A.eps_abs);
-          }
-          bs.mu = munew;
-          bs.trial = 0;
-          bs.mom_pending = 1;
-          if (stop) {
+          bs.mom_a = 1 - (mu / munew);
+          bs.mom_b = mu / munew;
+          bs.accept = 0;
+          // c/mfista_nufft_lib.c:737 (the movie solver has no such rule)
+          if (!movie && it > 1 && bs.l1_x == 0) stop = true;
+        }
+        if (!stop && it >= kMinIter) {
+          const double d = ring[(it - kTD) % (kTD + 1)] - cit;
+          stop = (A.rel_tol > 0) ? (d <= A.rel_tol * cit) : (d < A.eps_abs);
+        }
+        bs.mu = munew;
+        bs.trial = 0;
+        bs.mom_pending = 1;
+        if (stop) {
+          bs.stop = 1;
+          bs.ITER = it + 1;
+        } else {
+          bs.iter = it + 1;
+          if (bs.iter >= A.maxiter) {
             bs.stop = 1;
-            bs.ITER = it + 1;
+            bs.ITER = A.maxiter;
           } else {
-            bs.iter = it + 1;
-            if (bs.iter >= A.maxiter) {
-              bs.stop = 1;
-              bs.ITER = A.maxiter;
-            } else {
-              bs.prologue = 1;
-            }
+            bs.prologue = 1;
           }
         }
+        break;
       }
     }
     __syncthreads();
@@ -853,8 +1015,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   }
 
   // the accepted image of the last iteration (the reference's x = xnew)
-  if (bs.mom_pending && bs.accept)
-    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) A.x[f] = A.xnew[f];
+  if (bs.mom_pending && bs.accept) {
+    const double* xnw = A.xn[bs.kacc];
+    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) A.x[f] = xnw[f];
+  }
   if (lb == 0 && threadIdx.x == 0) {
     DevState* S = A.S;
     S->c = bs.c;
@@ -865,13 +1029,18 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     S->max_trials = bs.max_trials;
     S->total_trials = bs.total_trials;
     S->status = bs.status;
+    S->rounds = bs.rounds;
     S->done = 1;
   }
 }
 
+// The arguments are read from device memory rather than the kernel-argument
+// segment: passed by value they would be held in scalar registers, whose
+// spills into vector registers cost the 24-tap kernel its second workgroup
+// per CU.
 template <int MSPC>
-__global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(PArgs A) {
-  mfista_body<MSPC>(A, blockIdx.x, gridDim.x);
+__global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(const PArgs* __restrict__ A) {
+  mfista_body<MSPC>(*A, blockIdx.x, gridDim.x);
 }
 
 // A batch of independent solves in one cooperative launch: blk[b] =
@@ -894,56 +1063,66 @@ struct PLaunch {
   size_t sh = 0;       // dynamic LDS bytes
   int lds_tables = 0;  // tables staged in LDS
   int nblk = 0;        // workgroups of the launch
+  int pair = 0;        // paired trial rounds for the L1+TSV problems of the launch
 };
 
 static int persistent_config(Context& C, int MSP, const std::vector<const NufftState*>& sets, bool batch,
-                             PLaunch* L) {
+                             bool allow_pair, PLaunch* L) {
   if (MSP != 6 && MSP != 12) return MFISTA_E_ARG;
   if (batch)
     L->fn = (MSP == 6) ? (const void*)k_mfista_batch<6> : (const void*)k_mfista_batch<12>;
   else
     L->fn = (MSP == 6) ? (const void*)k_mfista_persistent<6> : (const void*)k_mfista_persistent<12>;
   // dynamic LDS: [twiddle + E4 tables, when they fit][stage scratch: FP64
-  // row/column FFT or the batched gridder's entries]
-  size_t sh = 0, tables = 0;
-  for (const NufftState* N : sets) {
-    const GridGeom& g = N->geom;
-    const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
-    size_t s1 = 2 * (size_t)Lm * sizeof(double2);
-    if (grid_lds_bytes(g.TX, g.TY) > s1) s1 = grid_lds_bytes(g.TX, g.TY);
-    if (s1 > sh) sh = s1;
-    const size_t t1 = (table_lds_bytes(g.Lx, g.Ly, g.NX, g.NY) + 255) & ~(size_t)255;
-    if (t1 > tables) tables = t1;
-  }
+  // row/column FFT (two sequences per transform in a paired round) or the
+  // batched gridder's entries].  Paired rounds are used when their larger
+  // scratch still leaves the wanted number of workgroups per CU.
   int dev = 0, ncu = 0, lds_max = 0;
   MF_CHECK(hipGetDevice(&dev));
   MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   MF_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + sizeof(BarCtx) + 64;
-  L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
-  if (L->lds_tables) sh += tables;
-  L->sh = sh;
-  int per_cu = 0;
-  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L->fn, MF_BLOCK, sh));
-  if (per_cu < 1) return MFISTA_E_HIP;
   const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 2;
-  L->nblk = ncu * (per_cu < want ? per_cu : want);
-  return 0;
+  for (int pair = (C.persist_pair && allow_pair) ? 1 : 0; pair >= 0; pair--) {
+    size_t sh = 0, tables = 0;
+    for (const NufftState* N : sets) {
+      const GridGeom& g = N->geom;
+      const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
+      size_t s1 = (size_t)(pair ? 4 : 2) * Lm * sizeof(double2);
+      if (grid_lds_bytes(g.TX, g.TY) > s1) s1 = grid_lds_bytes(g.TX, g.TY);
+      if (s1 > sh) sh = s1;
+      const size_t t1 = (table_lds_bytes(g.Lx, g.Ly, g.NX, g.NY) + 255) & ~(size_t)255;
+      if (t1 > tables) tables = t1;
+    }
+    L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
+    if (L->lds_tables) sh += tables;
+    L->sh = sh;
+    int per_cu = 0;
+    MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L->fn, MF_BLOCK, sh));
+    if (per_cu < 1 && !pair) return MFISTA_E_HIP;
+    if (pair && per_cu < want) continue;
+    L->pair = pair;
+    L->nblk = ncu * (per_cu < want ? per_cu : want);
+    return 0;
+  }
+  return MFISTA_E_HIP;
 }
 
 // Allocates and initialises one problem's workspace and fills its PArgs for
 // nblk workgroups.  P.xinit and P.xout are device pointers.
 static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P, int nblk, int lds_tables,
-                              int hier, hipStream_t st, PArgs* out) {
+                              int hier, int pair_ok, hipStream_t st, PArgs* out) {
   const GridGeom& g = N.geom;
   const int NX = P.NX, NY = P.NY, n = NX * NY;
   const int n_row = (NX + 1) / 2, n_ky = NY + 1, n_elem = elem_blocks(n);
   const int n_deg = degrid_blocks(N.M, N.MSP);
   const size_t rb = (size_t)n * 4;
   const size_t nky = (size_t)NY + 1;
-  if (W.px.ensure(2 * rb) || W.pz.ensure(2 * rb) || W.pxn.ensure(2 * rb) || W.g.ensure(rb) ||
-      W.vmod.ensure(3 * (size_t)N.M * sizeof(double2)) || W.T64.ensure(nky * NX * sizeof(double2)) ||
-      W.Gf64.ensure(((size_t)NY + N.MSP) * g.Lx * sizeof(double2)) ||
+  const int pair = (pair_ok && !P.tv_mode) ? 1 : 0;
+  const size_t t64 = nky * NX * sizeof(double2), gf64 = ((size_t)NY + N.MSP) * g.Lx * sizeof(double2);
+  if (W.px.ensure(2 * rb) || W.pz.ensure(2 * rb) || W.pxn.ensure(4 * rb) || W.g.ensure(rb) ||
+      W.vmod.ensure(4 * (size_t)N.M * sizeof(double2)) || W.T64.ensure((pair ? 2 : 1) * t64) ||
+      W.Gf64.ensure((pair ? 2 : 1) * gf64) ||
       W.Hg.ensure(nky * g.Lx * sizeof(float2)) || W.U.ensure((size_t)NX * nky * sizeof(float2)) ||
       W.part.ensure((size_t)nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
       W.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || W.bar.ensure(kBarBytes) ||
@@ -999,14 +1178,18 @@ static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P,
   A.U = (float2*)W.U.p;
   A.T64 = (double2*)W.T64.p;
   A.Gf64 = (double2*)W.Gf64.p;
-  for (int q = 0; q < 3; q++) A.vm[q] = (double2*)W.vmod.p + (size_t)q * N.M;
+  A.T64b = pair ? (double2*)((char*)W.T64.p + t64) : nullptr;
+  A.Gf64b = pair ? (double2*)((char*)W.Gf64.p + gf64) : nullptr;
+  A.pair = pair;
+  for (int q = 0; q < 4; q++) A.vm[q] = (double2*)W.vmod.p + (size_t)q * N.M;
   A.chunks = (const int4*)N.chunks.p;
   A.nchunks = N.nchunks;
   A.granges = (const int*)W.granges.p;
   A.ent = (const unsigned*)N.ent.p;
   A.x = (double*)W.px.p;
   A.z = (double*)W.pz.p;
-  A.xnew = (double*)W.pxn.p;
+  A.xn[0] = (double*)W.pxn.p;
+  A.xn[1] = (double*)W.pxn.p + n;
   A.gr = (float*)W.g.p;
   A.box = P.box_flag ? P.box : nullptr;
   if (P.tv_mode) {
@@ -1064,6 +1247,7 @@ static int persistent_collect(PersistWork& W, const RunParams& P, hipStream_t st
   out->total_trials = h.total_trials;
   out->status = h.status;
   out->steps = h.total_trials;
+  out->rounds = h.rounds;
   MF_CHECK(hipMemcpyAsync(P.xout, W.px.p, (size_t)P.NX * P.NY * 8, hipMemcpyDeviceToDevice, st));
   return 0;
 }
@@ -1088,10 +1272,14 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   hipStream_t st = C.stream;
   PLaunch L;
   int rc;
-  if ((rc = persistent_config(C, N.MSP, {&N}, false, &L))) return rc;
+  if ((rc = persistent_config(C, N.MSP, {&N}, false, !P.single_trials, &L))) return rc;
   PArgs A;
-  if ((rc = persistent_prepare(N, C.pw, P, L.nblk, L.lds_tables, C.persist_hier_barrier, st, &A))) return rc;
-  void* args[] = {&A};
+  if ((rc = persistent_prepare(N, C.pw, P, L.nblk, L.lds_tables, C.persist_hier_barrier, L.pair, st, &A)))
+    return rc;
+  if (C.pw.pargs.ensure(sizeof(PArgs))) return MFISTA_E_HIP;
+  MF_CHECK(hipMemcpyAsync(C.pw.pargs.p, &A, sizeof(PArgs), hipMemcpyHostToDevice, st));
+  const PArgs* dA = (const PArgs*)C.pw.pargs.p;
+  void* args[] = {&dA};
   float kms = 0;
   if ((rc = timed_launch(L, args, L.nblk, st, &kms))) return rc;
   out->kernel_ms = kms;
@@ -1101,11 +1289,114 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   return 0;
 }
 
+// Deals the launch's workgroups to problems p0 .. p0+pn-1: blk[b] = (problem,
+// index within the problem), nb[i] = workgroups of problem i.  Problems go
+// XCD by XCD (workgroup b runs on XCD b mod 8): each problem goes whole to the
+// XCD with the least work so far, and each XCD's workgroups are split among its
+// problems in proportion to their work per iteration, so a problem's data
+// stays in one L2.  With fewer problems than XCDs each spans several XCDs.
+static void deal_workgroups(const std::vector<NufftState*>& sets, const std::vector<RunParams>& P, int p0, int pn,
+                            int nblk, std::vector<int2>& blk, std::vector<int>& nb) {
+  const int kX = 8;
+  const int per_x = nblk / kX;
+  std::vector<double> w(pn);
+  for (int i = 0; i < pn; i++)
+    w[i] = (double)P[p0 + i].NX * P[p0 + i].NY + 4.0 * sets[p0 + i]->M * sets[p0 + i]->MSP;
+  std::vector<int> order(pn), xcd(pn, 0);
+  nb.assign(pn, 1);
+  for (int i = 0; i < pn; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
+  std::vector<double> load(kX, 0.0);
+  blk.assign(nblk, make_int2(-1, 0));
+  if (pn >= kX) {
+    std::vector<int> cnt(kX, 0);
+    for (int i : order) {
+      int best = -1;
+      for (int x = 0; x < kX; x++)
+        if (cnt[x] < per_x && (best < 0 || load[x] < load[best])) best = x;
+      xcd[i] = best;
+      load[best] += w[i];
+      cnt[best]++;
+    }
+    // split each XCD's workgroups among its problems (at least one each)
+    std::vector<std::vector<int>> slots(kX);
+    for (int b = 0; b < nblk; b++) slots[b % kX].push_back(b);
+    for (int x = 0; x < kX; x++) {
+      std::vector<int> mine;
+      for (int i = 0; i < pn; i++)
+        if (xcd[i] == x) mine.push_back(i);
+      if (mine.empty()) continue;
+      const int avail = (int)slots[x].size();
+      const int left = avail - (int)mine.size();
+      int used = 0;
+      for (int i : mine) {
+        nb[i] = 1 + (int)((double)left * w[i] / std::max(load[x], 1e-300));
+        used += nb[i];
+      }
+      for (size_t k = 0; used < avail; k = (k + 1) % mine.size(), used++) nb[mine[k]]++;
+      int s0 = 0;
+      for (int i : mine) {
+        for (int j = 0; j < nb[i]; j++) blk[slots[x][s0 + j]] = make_int2(i, j);
+        s0 += nb[i];
+      }
+    }
+  } else {
+    // fewer problems than XCDs: each spans several XCDs, workgroups in
+    // proportion to work, interleaved
+    double wt = 0;
+    for (int i = 0; i < pn; i++) wt += w[i];
+    int used = 0;
+    for (int i = 0; i < pn; i++) {
+      nb[i] = std::max(1, (int)((double)nblk * w[i] / wt));
+      used += nb[i];
+    }
+    while (used > nblk)
+      for (int i = 0; i < pn && used > nblk; i++)
+        if (nb[i] > 1) nb[i]--, used--;
+    std::vector<int> next(pn, 0);
+    int b = 0;
+    for (bool any = true; any;) {
+      any = false;
+      for (int i = 0; i < pn; i++)
+        if (next[i] < nb[i]) {
+          blk[b++] = make_int2(i, next[i]++);
+          any = true;
+        }
+    }
+  }
+}
+
+// Launches problems p0 .. p0+pn-1 (already dealt) and collects their outcomes.
+static int launch_dealt(Context& C, const PLaunch& L, std::vector<PArgs>& A, const std::vector<int2>& blk,
+                        const std::vector<RunParams>& P, int p0, std::vector<RunStats>& out, float* kernel_ms) {
+  const int pn = (int)A.size();
+  int nlaunch = 0;
+  for (int b = 0; b < L.nblk; b++)
+    if (blk[b].x >= 0) nlaunch = b + 1;
+  // workgroups past the last dealt one are not launched; undealt ones in
+  // between return at once
+  hipStream_t st = C.stream;
+  if (C.batch_args.ensure((size_t)pn * sizeof(PArgs)) || C.batch_blk.ensure((size_t)nlaunch * sizeof(int2)))
+    return MFISTA_E_HIP;
+  MF_CHECK(hipMemcpyAsync(C.batch_args.p, A.data(), (size_t)pn * sizeof(PArgs), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipMemcpyAsync(C.batch_blk.p, blk.data(), (size_t)nlaunch * sizeof(int2), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipStreamSynchronize(st));
+  const PArgs* dargs = (const PArgs*)C.batch_args.p;
+  const int2* dblk = (const int2*)C.batch_blk.p;
+  void* args[] = {&dargs, &dblk};
+  float kms = 0;
+  int rc;
+  if ((rc = timed_launch(L, args, nlaunch, st, &kms))) return rc;
+  *kernel_ms += kms;
+  for (int i = 0; i < pn; i++) {
+    out[p0 + i].kernel_ms = kms;
+    if ((rc = persistent_collect(C.bwork[p0 + i], P[p0 + i], st, &out[p0 + i]))) return rc;
+  }
+  MF_CHECK(hipStreamSynchronize(st));
+  return 0;
+}
+
 // Batched solves: problem i uses sets[i] (its NUFFT set-up) and C.bwork[i].
-// The launch's workgroups are dealt to problems XCD by XCD (workgroup b runs
-// on XCD b mod 8): each problem goes whole to the XCD with the least work so
-// far, and each XCD's workgroups are split among its problems in proportion
-// to their work per iteration, so a problem's data stays in one L2.
 // Problems beyond one workgroup each go to further launches.
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms) {
@@ -1118,107 +1409,76 @@ int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const st
     if (N->MSP != MSP) return MFISTA_E_ARG;
   PLaunch L;
   int rc;
-  if ((rc = persistent_config(C, MSP, cs, true, &L))) return rc;
+  if ((rc = persistent_config(C, MSP, cs, true, !P[0].single_trials, &L))) return rc;
   if ((int)C.bwork.size() < np) C.bwork.resize(np);
   out.assign(np, RunStats());
   *kernel_ms = 0;
-  const int kX = 8;
-  const int per_x = L.nblk / kX;
   for (int p0 = 0; p0 < np; p0 += L.nblk) {
     const int pn = std::min(np - p0, L.nblk);
-    // work per iteration: pixels and visibility taps
-    std::vector<double> w(pn);
-    for (int i = 0; i < pn; i++)
-      w[i] = (double)P[p0 + i].NX * P[p0 + i].NY + 4.0 * sets[p0 + i]->M * sets[p0 + i]->MSP;
-    std::vector<int> order(pn), xcd(pn, 0), nb(pn, 1);
-    for (int i = 0; i < pn; i++) order[i] = i;
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
-    std::vector<double> load(kX, 0.0);
-    std::vector<int2> blk(L.nblk, make_int2(-1, 0));
-    if (pn >= kX) {
-      std::vector<int> cnt(kX, 0);
-      for (int i : order) {
-        int best = -1;
-        for (int x = 0; x < kX; x++)
-          if (cnt[x] < per_x && (best < 0 || load[x] < load[best])) best = x;
-        xcd[i] = best;
-        load[best] += w[i];
-        cnt[best]++;
-      }
-      // split each XCD's workgroups among its problems (at least one each)
-      std::vector<std::vector<int>> slots(kX);
-      for (int b = 0; b < L.nblk; b++) slots[b % kX].push_back(b);
-      for (int x = 0; x < kX; x++) {
-        std::vector<int> mine;
-        for (int i = 0; i < pn; i++)
-          if (xcd[i] == x) mine.push_back(i);
-        if (mine.empty()) continue;
-        const int avail = (int)slots[x].size();
-        const int left = avail - (int)mine.size();
-        int used = 0;
-        for (int i : mine) {
-          nb[i] = 1 + (int)((double)left * w[i] / std::max(load[x], 1e-300));
-          used += nb[i];
-        }
-        for (size_t k = 0; used < avail; k = (k + 1) % mine.size(), used++) nb[mine[k]]++;
-        int s0 = 0;
-        for (int i : mine) {
-          for (int j = 0; j < nb[i]; j++) blk[slots[x][s0 + j]] = make_int2(i, j);
-          s0 += nb[i];
-        }
-      }
-    } else {
-      // fewer problems than XCDs: each spans several XCDs, workgroups in
-      // proportion to work, interleaved
-      double wt = 0;
-      for (int i = 0; i < pn; i++) wt += w[i];
-      int used = 0;
-      for (int i = 0; i < pn; i++) {
-        nb[i] = std::max(1, (int)((double)L.nblk * w[i] / wt));
-        used += nb[i];
-      }
-      while (used > L.nblk)
-        for (int i = 0; i < pn && used > L.nblk; i++)
-          if (nb[i] > 1) nb[i]--, used--;
-      std::vector<int> next(pn, 0);
-      int b = 0;
-      for (bool any = true; any;) {
-        any = false;
-        for (int i = 0; i < pn; i++)
-          if (next[i] < nb[i]) {
-            blk[b++] = make_int2(i, next[i]++);
-            any = true;
-          }
-      }
-    }
-    int nlaunch = 0;
-    for (int b = 0; b < L.nblk; b++)
-      if (blk[b].x >= 0) nlaunch = b + 1;
-    // workgroups past the last dealt one are not launched; undealt ones in
-    // between return at once
+    std::vector<int2> blk;
+    std::vector<int> nb;
+    deal_workgroups(sets, P, p0, pn, L.nblk, blk, nb);
     std::vector<PArgs> A(pn);
     for (int i = 0; i < pn; i++)
       if ((rc = persistent_prepare(*sets[p0 + i], C.bwork[p0 + i], P[p0 + i], nb[i], L.lds_tables,
-                                   C.persist_hier_barrier, st, &A[i])))
+                                   C.persist_hier_barrier, L.pair, st, &A[i])))
         return rc;
-    if (C.batch_args.ensure((size_t)pn * sizeof(PArgs)) || C.batch_blk.ensure((size_t)nlaunch * sizeof(int2)))
-      return MFISTA_E_HIP;
-    MF_CHECK(hipMemcpyAsync(C.batch_args.p, A.data(), (size_t)pn * sizeof(PArgs), hipMemcpyHostToDevice, st));
-    MF_CHECK(hipMemcpyAsync(C.batch_blk.p, blk.data(), (size_t)nlaunch * sizeof(int2), hipMemcpyHostToDevice, st));
-    MF_CHECK(hipStreamSynchronize(st));
-    const PArgs* dargs = (const PArgs*)C.batch_args.p;
-    const int2* dblk = (const int2*)C.batch_blk.p;
-    void* args[] = {&dargs, &dblk};
-    float kms = 0;
-    if ((rc = timed_launch(L, args, nlaunch, st, &kms))) return rc;
-    *kernel_ms += kms;
-    for (int i = 0; i < pn; i++) {
-      out[p0 + i].kernel_ms = kms;
-      if ((rc = persistent_collect(C.bwork[p0 + i], P[p0 + i], st, &out[p0 + i]))) return rc;
-    }
-    MF_CHECK(hipStreamSynchronize(st));
+    if ((rc = launch_dealt(C, L, A, blk, P, p0, out, kernel_ms))) return rc;
   }
   return 0;
+}
+
+// A coupled movie: frame i uses sets[i] and C.bwork[i]; all frames run in one
+// launch with one barrier and one control step (matlab/MFISTA_movie_eht.m).
+// w[t] = 1 / (t_{t+1} - t_t), t = 0 .. nframes-2.
+int solve_nufft_movie(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                      double lambda_t, const std::vector<double>& w, std::vector<RunStats>& out,
+                      float* kernel_ms) {
+  const int nf = (int)P.size();
+  hipStream_t st = C.stream;
+  if (nf == 0 || (int)w.size() != nf - 1) return MFISTA_E_ARG;
+  const int MSP = sets[0]->MSP;
+  std::vector<const NufftState*> cs(sets.begin(), sets.end());
+  for (const NufftState* N : cs)
+    if (N->MSP != MSP) return MFISTA_E_ARG;
+  for (const RunParams& R : P)
+    if (R.tv_mode || R.NX != P[0].NX || R.NY != P[0].NY) return MFISTA_E_ARG;
+  PLaunch L;
+  int rc;
+  if ((rc = persistent_config(C, MSP, cs, true, !P[0].single_trials, &L))) return rc;
+  if (nf > L.nblk) return MFISTA_E_SIZE;  // every frame needs a workgroup of the one launch
+  if ((int)C.bwork.size() < nf) C.bwork.resize(nf);
+  out.assign(nf, RunStats());
+  *kernel_ms = 0;
+  std::vector<int2> blk;
+  std::vector<int> nb;
+  deal_workgroups(sets, P, 0, nf, L.nblk, blk, nb);
+  std::vector<PArgs> A(nf);
+  for (int i = 0; i < nf; i++)
+    if ((rc = persistent_prepare(*sets[i], C.bwork[i], P[i], nb[i], L.lds_tables, C.persist_hier_barrier, L.pair,
+                                 st, &A[i])))
+      return rc;
+  int ngb = 0;
+  for (int i = 0; i < nf; i++) ngb += nb[i];
+  if (C.movie_part.ensure((size_t)ngb * kPW * 8)) return MFISTA_E_HIP;
+  int g0 = 0;
+  for (int i = 0; i < nf; i++) {
+    PArgs& a = A[i];
+    a.ngb = ngb;
+    a.gb0 = g0;
+    g0 += nb[i];
+    a.part = (double*)C.movie_part.p;
+    a.bar = A[0].bar;
+    a.lambda_t = lambda_t;
+    a.wt_prev = i > 0 ? w[i - 1] : 0.0;
+    a.wt_next = i < nf - 1 ? w[i] : 0.0;
+    a.z_prev = i > 0 ? A[i - 1].z : nullptr;
+    a.z_next = i < nf - 1 ? A[i + 1].z : nullptr;
+    a.x_next = i < nf - 1 ? A[i + 1].x : nullptr;
+    a.xn_next[0] = i < nf - 1 ? A[i + 1].xn[0] : nullptr;
+    a.xn_next[1] = i < nf - 1 ? A[i + 1].xn[1] : nullptr;
+  }
+  return launch_dealt(C, L, A, blk, P, 0, out, kernel_ms);
 }
 
 }  // namespace mf

@@ -48,6 +48,7 @@ struct PersistWork {
   DevMem Hg, U;            // FP32 adjoint grid and column-pass output
   DevMem part, state, cost, bar, fgp, fgpc, granges;
   DevMem x0, xo, box;      // batched host-pointer solves: staged xinit, xout, cl_box
+  DevMem pargs;            // the solver kernel's arguments (device copy)
 };
 
 // One visibility set of a batch, with its NUFFT set-up.
@@ -74,11 +75,13 @@ struct Context {
   std::vector<BatchSet> bset;
   std::vector<PersistWork> bwork;
   DevMem batch_args, batch_blk;  // PArgs array and workgroup table of a batch launch
+  DevMem movie_part;             // partial-sum table shared by the frames of a coupled movie
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
 
   int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
   int persist_hier_barrier = 1;  // MFISTA_PERSISTENT_FLAT_BARRIER=1 selects the flat barrier
+  int persist_pair = 1;          // MFISTA_PERSISTENT_PAIRED=0: one backtracking trial per round
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

@@ -482,6 +482,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     P.lambda_tsv = A.lambda_tsv;
     P.cinit = A.cinit;
     P.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
+    P.single_trials = (opts && opts->single_trials) ? 1 : 0;
     P.xinit = A.xinit;
     P.xout = A.xout;
     P.box = A.box;
@@ -501,6 +502,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     opts->max_trials = rs.max_trials;
     opts->total_trials = rs.total_trials;
     opts->kernel_ms = rs.kernel_ms;
+    opts->trial_rounds = rs.rounds;
   }
   if (rc) return rc;
   MF_CHECK(hipStreamSynchronize(C.stream));
@@ -548,6 +550,7 @@ Context* get_context() {
       if (v >= 1 && v <= 4) C->persist_blocks_per_cu = v;
     }
     if (const char* e = getenv("MFISTA_PERSISTENT_FLAT_BARRIER")) C->persist_hier_barrier = atoi(e) ? 0 : 1;
+    if (const char* e = getenv("MFISTA_PERSISTENT_PAIRED")) C->persist_pair = atoi(e) ? 1 : 0;
     g_ctx[dev] = C;
   }
   return g_ctx[dev];
@@ -721,6 +724,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     R.lambda_tsv = q.lambda_tsv;
     R.cinit = q.cinit;
     R.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
+    R.single_trials = (opts && opts->single_trials) ? 1 : 0;
     R.xinit = (const double*)W.x0.p;
     R.xout = (double*)W.xo.p;
     R.box = q.box_flag ? (const float*)W.box.p : nullptr;
@@ -732,7 +736,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
   float kms = 0;
   if ((rc = solve_nufft_batch(*C, sets, P, st, &kms))) return rc;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  int total_trials = 0, max_trials = 0;
+  int total_trials = 0, max_trials = 0, rounds = 0;
   for (size_t j = 0; j < which.size(); j++) {
     const int i = which[j];
     const mfista_nufft_problem& q = pr[i];
@@ -744,6 +748,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     R->maxiter = q.maxiter;
     R->comp_time = secs;
     total_trials += st[j].total_trials;
+    rounds += st[j].rounds;
     max_trials = std::max(max_trials, st[j].max_trials);
     MF_CHECK(hipMemcpy(q.xout, W.xo.p, (size_t)q.NX * q.NY * 8, hipMemcpyDeviceToHost));
     // FP64 result pass (calc_result_nufft, c/mfista_nufft_lib.c:791-905)
@@ -763,8 +768,181 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     opts->kernel_ms = kms;
     opts->total_trials = total_trials;
     opts->max_trials = max_trials;
+    opts->trial_rounds = rounds;
   }
   return 0;
+}
+
+// Coupled movie, matlab/MFISTA_movie_eht.m:1-239: frames split by time_idx
+// (1-based, :46-86), frame times t_list (:53-69), one persistent launch with
+// one control step for all frames (solve_nufft_movie), then the outputs of
+// :203-239 (model visibilities and the cost terms) from FP64 transforms.
+int mfista_movie_nufft(const double* vis_r, const double* vis_i, const double* u_dx, const double* v_dy,
+                       const double* noise_std, const int* time_idx, const double* time_val, int M,
+                       const float* c_box, int NX, int NY, const double* xinit, double lambda_l1,
+                       double lambda_tsv, double lambda_t, int nonneg_flag, double cinit, int maxiter,
+                       double eps, double* xout, double* cost, double* terms, double* outvis_r,
+                       double* outvis_i, int* iterations, mfista_opts* opts) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (M <= 0 || NX <= 0 || NY <= 0 || !vis_r || !vis_i || !u_dx || !v_dy || !noise_std || !time_idx ||
+      !xout || maxiter <= 0 || lambda_t < 0)
+    return MFISTA_E_ARG;
+  const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
+  auto t0 = std::chrono::steady_clock::now();
+  int nt = 0;
+  for (int i = 0; i < M; i++) {
+    if (time_idx[i] < 1) return MFISTA_E_ARG;
+    nt = std::max(nt, time_idx[i]);
+  }
+  std::vector<std::vector<int>> fr(nt);
+  for (int i = 0; i < M; i++) fr[time_idx[i] - 1].push_back(i);
+  for (int f = 0; f < nt; f++)
+    if (fr[f].empty()) return MFISTA_E_ARG;  // every frame needs data (its mean time, :62-64)
+  std::vector<double> tl(nt);
+  if (time_val) {
+    for (int f = 0; f < nt; f++) {
+      double sum = 0;
+      for (int i : fr[f]) sum += time_val[i];
+      tl[f] = sum / (double)fr[f].size();
+    }
+    const double span = *std::max_element(tl.begin(), tl.end()) - *std::min_element(tl.begin(), tl.end());
+    if (!(span > 0)) return MFISTA_E_ARG;
+    for (int f = 0; f < nt; f++) tl[f] = nt * tl[f] / span;
+  } else {
+    for (int f = 0; f < nt; f++) tl[f] = f + 1.0;
+  }
+  std::vector<double> w(nt > 1 ? nt - 1 : 0);
+  for (int f = 0; f + 1 < nt; f++) {
+    w[f] = 1.0 / (tl[f + 1] - tl[f]);
+    if (!std::isfinite(w[f])) return MFISTA_E_ARG;
+  }
+  const size_t nn = (size_t)NX * NY, nb = nn * 8;
+  std::vector<double> x0(nn, 0.0);
+  for (size_t p = 0; p < nn; p++) {
+    const double xv = xinit ? xinit[p] : 0.0;
+    x0[p] = c_box ? xv * (double)c_box[p] : xv;  // x = (xinit .* c_box) for every frame, :49
+  }
+  if ((int)C->bset.size() < nt) C->bset.resize(nt);
+  if ((int)C->bwork.size() < nt) C->bwork.resize(nt);
+  int rc;
+  std::vector<NufftState*> sets;
+  std::vector<RunParams> P;
+  std::vector<double> hb;
+  for (int f = 0; f < nt; f++) {
+    const int mf = (int)fr[f].size();
+    BatchSet& S = C->bset[f];
+    hb.resize((size_t)5 * mf);
+    for (int k = 0; k < mf; k++) {
+      const int i = fr[f][k];
+      hb[k] = u_dx[i];
+      hb[mf + k] = v_dy[i];
+      hb[2 * mf + k] = vis_r[i];
+      hb[3 * mf + k] = vis_i[i];
+      hb[4 * mf + k] = noise_std[i];
+    }
+    const size_t Mb = (size_t)mf * 8;
+    if ((rc = upload(*C, S.u, hb.data(), Mb)) || (rc = upload(*C, S.v, hb.data() + mf, Mb)) ||
+        (rc = upload(*C, S.vr, hb.data() + 2 * mf, Mb)) || (rc = upload(*C, S.vi, hb.data() + 3 * mf, Mb)) ||
+        (rc = upload(*C, S.sd, hb.data() + 4 * mf, Mb)))
+      return rc;
+    std::swap(C->nu, S.nu);
+    rc = nufft_setup(*C, preset, (const double*)S.u.p, (const double*)S.v.p, (const double*)S.vr.p,
+                     (const double*)S.vi.p, (const double*)S.sd.p, mf, NX, NY);
+    std::swap(C->nu, S.nu);
+    if (rc) {
+      if (rc == MFISTA_E_FOV) printf("Error: the field of view is too small for the (u,v) coverage.\n");
+      return rc;
+    }
+    PersistWork& W = C->bwork[f];
+    if ((rc = upload(*C, W.x0, x0.data(), nb)) || (rc = upload(*C, W.xo, nullptr, nb))) return rc;
+    if (c_box && (rc = upload(*C, W.box, c_box, nn * 4))) return rc;
+    RunParams R;
+    R.NX = NX;
+    R.NY = NY;
+    R.maxiter = maxiter;
+    R.nonneg = (nonneg_flag == 1) ? 1 : 0;  // softth_nonneg only for nonneg_flag == 1, :141-146
+    R.box_flag = c_box ? 1 : 0;
+    R.tv_mode = 0;
+    R.eps_abs = eps;  // the movie solver's rule is absolute and unscaled, :186
+    R.rel_tol = opts ? opts->rel_tol : 0.0;
+    R.lambda_l1 = lambda_l1;
+    R.lambda_tsv = lambda_tsv;
+    R.cinit = cinit;
+    R.eta_first = kEta;
+    R.single_trials = (opts && opts->single_trials) ? 1 : 0;
+    R.xinit = (const double*)W.x0.p;
+    R.xout = (double*)W.xo.p;
+    R.box = c_box ? (const float*)W.box.p : nullptr;
+    sets.push_back(&S.nu);
+    P.push_back(R);
+  }
+  std::vector<RunStats> st;
+  float kms = 0;
+  if ((rc = solve_nufft_movie(*C, sets, P, lambda_t, w, st, &kms))) return rc;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const int iters = st[0].ITER;
+  if (iterations) *iterations = iters;
+  if (cost && iters > 0)
+    MF_CHECK(hipMemcpy(cost, C->bwork[0].cost.p, (size_t)std::min(iters, maxiter) * 8, hipMemcpyDeviceToHost));
+  // outputs (:203-239): images, FP64 chi^2 / L1 / TSV per frame, d_X, model visibilities
+  double chisq = 0, l1 = 0, tsvc = 0;
+  std::vector<double> model;
+  for (int f = 0; f < nt; f++) {
+    PersistWork& W = C->bwork[f];
+    MF_CHECK(hipMemcpy(xout + (size_t)f * nn, W.xo.p, nb, hipMemcpyDeviceToHost));
+    NufftState& N = C->bset[f].nu;
+    std::swap(C->nu, N);
+    double chi2 = 0, stats[4] = {0, 0, 0, 0};
+    rc = nufft_chi2_64(*C, (const double*)W.xo.p, &chi2);
+    if (!rc) rc = image_stats64(*C, (const double*)W.xo.p, NX, NY, stats);
+    if (!rc && outvis_r && outvis_i) {
+      const int mf = (int)fr[f].size();
+      if (C->h_model.ensure((size_t)mf * sizeof(double2))) {
+        rc = MFISTA_E_HIP;
+      } else {
+        nufft_fwd<double>(*C, (const double*)W.xo.p, nullptr, nullptr, (double2*)C->h_model.p, nullptr);
+        model.resize((size_t)2 * mf);
+        if (hipMemcpy(model.data(), C->h_model.p, (size_t)mf * 16, hipMemcpyDeviceToHost) != hipSuccess)
+          rc = MFISTA_E_HIP;
+        for (int k = 0; k < mf && !rc; k++) {
+          outvis_r[fr[f][k]] = model[2 * k];
+          outvis_i[fr[f][k]] = model[2 * k + 1];
+        }
+      }
+    }
+    std::swap(C->nu, N);
+    if (rc) return rc;
+    chisq += chi2;
+    l1 += stats[0];
+    tsvc += stats[1];
+  }
+  double dx = 0;
+  for (int f = 0; f + 1 < nt; f++) {
+    const double* a = xout + (size_t)f * nn;
+    const double* b = a + nn;
+    double s1 = 0;
+    for (size_t p = 0; p < nn; p++) s1 += (a[p] - b[p]) * (a[p] - b[p]);
+    dx += w[f] * s1;
+  }
+  if (terms) {
+    terms[0] = chisq;
+    terms[1] = l1;
+    terms[2] = tsvc;
+    terms[3] = dx;
+  }
+  if (opts) {
+    int tt = 0, mt = 0;
+    for (const RunStats& r : st) mt = std::max(mt, r.max_trials);
+    tt = st[0].total_trials;
+    opts->solve_ms = secs * 1e3;
+    opts->kernel_ms = kms;
+    opts->total_trials = tt;
+    opts->max_trials = mt;
+    opts->trial_rounds = st[0].rounds;
+  }
+  return st[0].status ? MFISTA_E_HIP : 0;
 }
 
 int mfista_imaging_core_nufft_ex(const double* u_dx, const double* v_dy, const double* vis_r,

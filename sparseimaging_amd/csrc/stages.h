@@ -56,6 +56,87 @@ __device__ inline void st_rowfft_fwd(int task, int nseq, int NX, int NY, int Ly,
   __syncthreads();
 }
 
+// Forward row pass of two images at once (the two steps of a paired
+// backtracking round): sequence 0 packs rows (2 task, 2 task + 1) of image 0,
+// sequence 1 the same rows of image 1, so all 256 lanes have butterflies.
+// load(i, j, v0, v1) supplies pixel (i, j) of both images; the per-image
+// arithmetic is that of st_rowfft_fwd.
+template <typename R, typename Load2>
+__device__ inline void st_rowfft_fwd_pair(int task, int NX, int NY, int Ly, int shift, const FFTPlan& P,
+                                          const cplx<R>* __restrict__ tw, cplx<R>* __restrict__ T0,
+                                          cplx<R>* __restrict__ T1, cplx<R>* sm, Load2 load) {
+  using C = cplx<R>;
+  C* b0 = sm;
+  C* b1 = sm + 2 * Ly;
+  const int i0 = task * 2, i1 = i0 + 1;
+  for (int b = threadIdx.x; b < Ly; b += blockDim.x) {
+    int j = b + shift;
+    if (j >= Ly) j -= Ly;
+    R re0 = 0, im0 = 0, re1 = 0, im1 = 0;
+    if (j < NY) {
+      if (i0 < NX) load(i0, j, re0, re1);
+      if (i1 < NX) load(i1, j, im0, im1);
+    }
+    b0[b] = mk<R>(re0, im0);
+    b0[Ly + b] = mk<R>(re1, im1);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 2, P, tw, -1);
+  const int Lh = Ly / 2 + 1;
+  for (int t = threadIdx.x; t < Lh * 4; t += blockDim.x) {
+    const int k = t >> 2, r = t & 3;
+    const int q = r >> 1, i = i0 + (r & 1);
+    if (i >= NX) continue;
+    const C z = res[q * Ly + k], zm = res[q * Ly + (k == 0 ? 0 : Ly - k)];
+    C o;
+    if (!(r & 1))
+      o = mk<R>((R)0.5 * (z.x + zm.x), (R)0.5 * (z.y - zm.y));
+    else
+      o = mk<R>((R)0.5 * (z.y + zm.y), (R)-0.5 * (z.x - zm.x));
+    (q ? T1 : T0)[(size_t)k * NX + i] = o;
+  }
+  __syncthreads();
+}
+
+// Forward column pass of two transforms at once: ky row `task` of T0 -> out0
+// and of T1 -> out1, NUFFT layout (mode 0 of st_colfft_fwd).
+template <typename R>
+__device__ inline void st_colfft_fwd_pair(int task, const cplx<R>* __restrict__ T0,
+                                          const cplx<R>* __restrict__ T1, cplx<R>* __restrict__ out0,
+                                          cplx<R>* __restrict__ out1, int NX, int nky, int Lx, int shift,
+                                          int MSP, const FFTPlan& P, const cplx<R>* __restrict__ tw,
+                                          cplx<R>* sm) {
+  using C = cplx<R>;
+  C* b0 = sm;
+  C* b1 = sm + 2 * Lx;
+  const int ky = task;
+  for (int t = threadIdx.x; t < 2 * Lx; t += blockDim.x) {
+    const int q = t >= Lx, a = t - q * Lx;
+    int i = a + shift;
+    if (i >= Lx) i -= Lx;
+    b0[t] = (ky < nky && i < NX) ? (q ? T1 : T0)[(size_t)ky * NX + i] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 2, P, tw, -1);
+  const int NXh = Lx / 2;
+  if (ky < nky) {
+    for (int t = threadIdx.x; t < 2 * Lx; t += blockDim.x) {
+      const int q = t >= Lx, k = t - q * Lx;
+      C* out = q ? out1 : out0;
+      const C v = res[t];
+      int col = k + NXh;
+      if (col >= Lx) col -= Lx;
+      out[(size_t)(ky + MSP - 1) * Lx + col] = v;
+      if (ky >= 1 && ky <= MSP - 1) {
+        int colm = NXh - k;
+        if (colm < 0) colm += Lx;
+        out[(size_t)(MSP - 1 - ky) * Lx + colm] = cconj(v);
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // Forward column pass: c2c along x for ky rows ky0 .. ky0+nseq-1 of T.
 // mode 0: NUFFT layout Gf[ky + MSP - 1][(kx + NX) mod Lx] plus the MSP-1
 //         conjugate-mirrored rows of negative ky
@@ -300,6 +381,87 @@ __device__ inline void st_degrid_model(int task, const GridGeom& g, const VisPar
       v[0] += rx * rx + ry * ry;
       v[1] += tx * tx + ty * ty;
       out0[s] = make_double2(mz.x + tx, mz.y + ty);
+    }
+  }
+}
+
+// Tap gather of two spectra with the same taps (degrid_core on Gf0 and Gf1).
+template <int MSPC>
+__device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisParams<double>& vp,
+                                        const double2* __restrict__ Gf0, const double2* __restrict__ Gf1,
+                                        int& s, double2& F0, double2& F1) {
+  constexpr int S = DegridShape<MSPC>::S, G = DegridShape<MSPC>::G;
+  const int gid = task * blockDim.x + threadIdx.x;
+  s = gid / G;
+  const int lane = threadIdx.x & (G - 1);
+  const double t3x = g.t3x, t3y = g.t3y;
+  double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+  if (s < g.M && lane < S) {
+    const int2 m = vp.mxy[s];
+    const double2 a = vp.a[s];
+    const double l = (double)(lane - MSPC + 1);
+    const double gx = dev_exp(l * a.x - l * l * t3x);
+    const size_t off = (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
+    const double2* c0 = Gf0 + off;
+    const double2* c1 = Gf1 + off;
+#pragma unroll
+    for (int b = 0; b < S; b++) {
+      const double2 u0 = c0[(size_t)b * g.Lx], u1 = c1[(size_t)b * g.Lx];
+      const double lp = (double)(b - MSPC + 1);
+      const double gy = dev_exp(lp * a.y - lp * lp * t3y);
+      a0.x += gy * u0.x;
+      a0.y += gy * u0.y;
+      a1.x += gy * u1.x;
+      a1.y += gy * u1.y;
+    }
+    a0.x *= gx;
+    a0.y *= gx;
+    a1.x *= gx;
+    a1.y *= gx;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0.x += __shfl_xor(a0.x, off, 64);
+    a0.y += __shfl_xor(a0.y, off, 64);
+    a1.x += __shfl_xor(a1.x, off, 64);
+    a1.y += __shfl_xor(a1.y, off, 64);
+  }
+  if (s < g.M && lane == 0) {
+    const double e1s = vp.e1[s] * g.inv_MM;
+    F0 = make_double2(e1s * a0.x, -e1s * a0.y);
+    F1 = make_double2(e1s * a1.x, -e1s * a1.y);
+    return true;
+  }
+  return false;
+}
+
+// Step mode of st_degrid_model for the two steps of a paired round:
+// out_k = mz + W A d_k; v[2k] += |W V' - mz - W A d_k|^2, v[2k+1] += |W A d_k|^2.
+template <int MSPC>
+__device__ inline void st_degrid_model_pair(int task, const GridGeom& g, const VisParams<double>& vp,
+                                            const double2* __restrict__ Gf0, const double2* __restrict__ Gf1,
+                                            const double2* __restrict__ mz_in, double2* __restrict__ out0,
+                                            double2* __restrict__ out1, double v[4]) {
+  int s;
+  double2 F0, F1;
+  if (degrid_core_pair<MSPC>(task, g, vp, Gf0, Gf1, s, F0, F1)) {
+    const double w = vp.w[s];
+    const double2 V = vp.vis[s];
+    const double2 mz = mz_in[s];
+    const double bx = V.x * w - mz.x, by = V.y * w - mz.y;
+    {
+      const double tx = F0.x * w, ty = F0.y * w;
+      const double rx = bx - tx, ry = by - ty;
+      v[0] += rx * rx + ry * ry;
+      v[1] += tx * tx + ty * ty;
+      out0[s] = make_double2(mz.x + tx, mz.y + ty);
+    }
+    {
+      const double tx = F1.x * w, ty = F1.y * w;
+      const double rx = bx - tx, ry = by - ty;
+      v[2] += rx * rx + ry * ry;
+      v[3] += tx * tx + ty * ty;
+      out1[s] = make_double2(mz.x + tx, mz.y + ty);
     }
   }
 }

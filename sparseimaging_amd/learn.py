@@ -19,7 +19,10 @@ convolution and its gradients run on the MFMA kernels of csrc/unet.hip
 
 Input normalisation: with s = max(dirty) / sum(w^2) (the peak flux the dirty
 image implies), channel 0 is dirty / (s sum(w^2)) and channel 1 the constant
-(log10 lambda_tsv - 2) / 2; the target is x / s and the warm start s y."""
+(log10 lambda_tsv - 2) / 2; the target is x / s and the warm start
+alpha s max(y, 0), with alpha the exact minimiser of the objective along that
+ray (one FP64 forward transform): MFISTA's iteration count is far more
+sensitive to a wrong global scale than to local errors of the same size."""
 import ctypes as C
 import json
 import os
@@ -396,18 +399,49 @@ def predict(net, x_in, s):
     return np.maximum(y[0].cpu().numpy().astype(np.float64), 0.0) * s
 
 
+def objective(u, v, vis_r, vis_i, vis_std, x, lambda_l1, lambda_tsv, preset=PRESET_CXX12):
+    """The L1+TSV objective of an NX x NY image x (FP64 NUFFT), as RESULT.finalcost
+    (c/mfista_nufft_lib.c:791-905): |W(V - A x)|^2 / 2 + lambda_l1 |x|_1 + lambda_tsv TSV(x)."""
+    x = np.asarray(x, dtype=np.float64)
+    m = nufft_forward(u, v, x, preset=preset, fp64=True)
+    r = ((np.asarray(vis_r) + 1j * np.asarray(vis_i)) - m) / np.asarray(vis_std)
+    tsv = float(np.sum(np.diff(x, axis=0) ** 2) + np.sum(np.diff(x, axis=1) ** 2))
+    return 0.5 * float(np.sum(np.abs(r) ** 2)) + lambda_l1 * float(np.abs(x).sum()) + lambda_tsv * tsv
+
+
+def ray_scale(u, v, vis_r, vis_i, vis_std, x0, lambda_l1, lambda_tsv, preset=PRESET_CXX12):
+    """alpha >= 0 minimising F(alpha x0), a quadratic in alpha:
+    F = |W(V - alpha A x0)|^2 / 2 + alpha lambda_l1 |x0|_1 + alpha^2 lambda_tsv TSV(x0), so
+    alpha = (Re<W A x0, W V> - lambda_l1 |x0|_1) / (|W A x0|^2 + 2 lambda_tsv TSV(x0)).
+    One FP64 forward transform; corrects the global flux scale of a prediction,
+    to which MFISTA's iteration count is most sensitive."""
+    x0 = np.asarray(x0, dtype=np.float64)
+    w = 1.0 / np.asarray(vis_std)
+    m = nufft_forward(u, v, x0, preset=preset, fp64=True) * w
+    V = (np.asarray(vis_r) + 1j * np.asarray(vis_i)) * w
+    tsv = float(np.sum(np.diff(x0, axis=0) ** 2) + np.sum(np.diff(x0, axis=1) ** 2))
+    den = float(np.sum(np.abs(m) ** 2)) + 2.0 * lambda_tsv * tsv
+    if den <= 0.0:
+        return 0.0
+    return max(0.0, (float(np.real(np.vdot(m, V))) - lambda_l1 * float(np.abs(x0).sum())) / den)
+
+
 def solve_warm(net, u, v, vis_r, vis_i, vis_std, N, lambda_l1=1.0, lambda_tsv=100.0, verify=False,
                preset=PRESET_CXX12, **kw):
-    """MFISTA from the network's warm start.  With verify=True the cold start is
-    solved too and wins if its converged objective is lower (the acceptance
-    rule).  Returns (x, RESULT dict, info) with info["warm_accepted"]."""
+    """MFISTA from the network's warm start, rescaled by the exact line search
+    along its ray (ray_scale).  With verify=True the cold start is solved too
+    and wins if its converged objective is lower (the acceptance rule).
+    Returns (x, RESULT dict, info) with info["warm_accepted"], info["alpha"]
+    and info["x0"] (the starting image)."""
     x_in, s = net_input(u, v, vis_r, vis_i, vis_std, N, lambda_tsv, preset)
     t0 = time.perf_counter()
     x0 = predict(net, x_in, s)
+    alpha = ray_scale(u, v, vis_r, vis_i, vis_std, x0, lambda_l1, lambda_tsv, preset)
+    x0 = alpha * x0
     t_net = time.perf_counter() - t0
     xw, rw, iw = mfista_imaging_core_nufft(u, v, vis_r, vis_i, vis_std, N, N, xinit=x0, lambda_l1=lambda_l1,
                                            lambda_tsv=lambda_tsv, preset=preset, **kw)
-    info = dict(iw, net_ms=t_net * 1e3, warm_accepted=True, warm_iter=rw["ITER"])
+    info = dict(iw, net_ms=t_net * 1e3, warm_accepted=True, warm_iter=rw["ITER"], alpha=alpha, x0=x0)
     if verify:
         xc, rc, ic = mfista_imaging_core_nufft(u, v, vis_r, vis_i, vis_std, N, N, lambda_l1=lambda_l1,
                                                lambda_tsv=lambda_tsv, preset=preset, **kw)
