@@ -107,6 +107,9 @@ void y2x_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *yin_r,
 #define MFISTA_E_ARG -5        /* invalid argument */
 #define MFISTA_E_NONNEG -6     /* nonneg_flag not in {0,1}: ITER = 0, as the reference */
 #define MFISTA_E_HIP -100      /* HIP runtime error */
+#define MFISTA_E_COOP -101     /* the runtime refused the cooperative launch (another process
+                                  holds the device's cooperative-launch resources); single
+                                  solves then use the stream engine instead */
 
 typedef struct mfista_opts {
   int preset;       /* MFISTA_PRESET_* (NUFFT only); default C24 */

@@ -32,6 +32,7 @@ STATUS = {
     0: "ok", -1: "lambda_tv and lambda_tsv both positive", -2: "TV needs the C-24 preset",
     -3: "field of view too small for the (u,v) coverage", -4: "unsupported image size",
     -5: "invalid argument", -6: "nonneg_flag must be 0 or 1", -100: "HIP runtime error",
+    -101: "cooperative launch refused (another process holds the device's cooperative-launch resources)",
 }
 
 EXPORTED = [

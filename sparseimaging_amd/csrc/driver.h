@@ -42,7 +42,9 @@ struct RunStats {
 template <typename R>
 int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
 
-// the whole NUFFT solve in one cooperative launch (persistent.hip)
+// the whole NUFFT solve in one cooperative launch (persistent.hip); returns
+// kCoopRefused, with nothing run, if the runtime refuses the cooperative launch
+constexpr int kCoopRefused = MFISTA_E_COOP;
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms);

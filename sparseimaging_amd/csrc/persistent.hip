@@ -67,6 +67,9 @@ struct PArgs {
   // Coupled movie (frames of one solve, matlab/MFISTA_movie_eht.m): every
   // frame is one PArgs of a batch launch; all frames share one barrier, one
   // partial-sum table and so one control step.  ngb = 0: an ordinary solve.
+  unsigned long long* trace;  // per-barrier clocks (nullptr: no trace), see BarCtx
+  int* ttag;
+  int tcap;
   int ngb;                   // workgroups of all frames (barrier, partial rows)
   int gb0;                   // partial row of this frame's workgroup 0
   double lambda_t;           // weight of d_X = sum_t w_t |x_t - x_{t+1}|^2
@@ -98,6 +101,11 @@ struct BarCtx {
   int xcc;         // this workgroup's XCD
   int leader;      // first workgroup of its XCD
   int hier;        // 1: XCD-hierarchical barrier, 0: flat
+  // tracing (MFISTA_PERSISTENT_TRACE): per barrier and workgroup, the clock at
+  // arrival and at release; the stage tag of each barrier from workgroup 0
+  unsigned long long* trace;
+  int* ttag;
+  int tcap, trow, tnrow;
 };
 
 // bounded wait until bar[i] >= target; false (and the error flag set) on
@@ -168,12 +176,23 @@ __device__ inline bool xcd_sync(const BarCtx& B, unsigned long long epoch) {
   return true;
 }
 
-__device__ inline bool grid_sync(const BarCtx& B, unsigned long long& epoch) {
+// tag: the stage that ends at this barrier (for the trace)
+__device__ inline bool grid_sync(const BarCtx& B, unsigned long long& epoch, int tag = 0) {
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
   __shared__ int fail;
   epoch += 1;
-  if (threadIdx.x == 0) fail = (B.hier ? xcd_sync(B, epoch) : flat_sync(B.bar, 0, B.nblk, epoch)) ? 0 : 1;
+  if (threadIdx.x == 0) {
+    const unsigned long long ta = wall_clock64();
+    fail = (B.hier ? xcd_sync(B, epoch) : flat_sync(B.bar, 0, B.nblk, epoch)) ? 0 : 1;
+    if (B.trace && epoch <= (unsigned long long)B.tcap) {
+      const unsigned long long tr = wall_clock64();
+      unsigned long long* t = B.trace + ((epoch - 1) * (unsigned long long)B.tnrow + (unsigned long long)B.trow) * 2;
+      t[0] = ta;
+      t[1] = tr;
+      if (B.trow == 0) B.ttag[epoch - 1] = tag;
+    }
+  }
   __syncthreads();
   return fail == 0;
 }
@@ -492,6 +511,14 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   double* prow = A.part + (size_t)(movie ? A.gb0 + (int)lb : (int)lb) * kPW;
 
   if (!bar_setup(B, A.bar, ngrp, A.hier_barrier)) return;
+  if (threadIdx.x == 0) {
+    B.trace = A.trace;
+    B.ttag = A.ttag;
+    B.tcap = A.tcap;
+    B.trow = movie ? A.gb0 + (int)lb : (int)lb;
+    B.tnrow = (int)ngrp;
+  }
+  __syncthreads();
 
   // LDS: [tables][stage scratch]
   char* lds = reinterpret_cast<char*>(smem_d4);
@@ -576,17 +603,17 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       prow[kPIdx] = sdx[0];
     }
   }
-  if (!grid_sync(B, epoch)) return;
+  if (!grid_sync(B, epoch, 1)) return;
   for (int t = lb; t < A.n_ky; t += nb)
     st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
-  if (!grid_sync(B, epoch)) return;
+  if (!grid_sync(B, epoch, 2)) return;
   {
     double v[2] = {0, 0};
     for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, nullptr, A.vm[0], A.vm[2], v);
     block_sum<2>(v, red);
     if (threadIdx.x == 0) prow[kPIchi] = v[0];
   }
-  if (!grid_sync(B, epoch)) return;
+  if (!grid_sync(B, epoch, 3)) return;
   {
     const int slot[5] = {kPIchi, kPIl1, kPItsv, kPItv, kPIdx};
     double sum[5];
@@ -660,7 +687,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       };
       grid_chunks<(MSPC <= 8) ? 2 : 4>(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent,
                                          A.Hg, lds);
-      if (!grid_sync(B, epoch)) return;
+      if (!grid_sync(B, epoch, 4)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles
         int& rn = ka ? bs.in1 : bs.in;
         const int tmp = bs.ix;
@@ -670,7 +697,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       // P2: inverse column pass (clears Hg behind it)
       for (int t = lb; t < A.n_ky; t += nb)
         st_colfft_inv<float>(t, 1, A.Hg, A.U, NX, NY + 1, g.Lx, NX / 2, 0, A.Px, twxf, sm32);
-      if (!grid_sync(B, epoch)) return;
+      if (!grid_sync(B, epoch, 5)) return;
     }
     const int trial_first = prologue;
     const double c = bs.c;
@@ -706,7 +733,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       if (trial_first) {
         for (int t = lb; t < A.n_row; t += nb)
           st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
-        if (!grid_sync(B, epoch)) return;
+        if (!grid_sync(B, epoch, 7)) return;
       }
       // FGP iterations (c/mfista_TV_lib.c:189-384), b = z + g/c (nonneg: -l1/c + g/c + z)
       const float bshift = (A.nonneg == 1) ? (float)(-A.lambda_l1 / c) : 0.f;
@@ -756,7 +783,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
             A.qq[kk] = nq_;
           }
         }
-        if (!grid_sync(B, epoch)) return;
+        if (!grid_sync(B, epoch, 6)) return;
         float* tmp = ri;
         ri = ro;
         ro = tmp;
@@ -836,7 +863,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         prow[kPL1] = v[1];
       }
     }
-    if (!grid_sync(B, epoch)) return;
+    if (!grid_sync(B, epoch, 8)) return;
     // T2: FP64 column pass of d; regulariser partials of xnew and d
     if (pr) {
       for (int t = lb; t < A.n_ky; t += nb)
@@ -878,7 +905,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         prow[kPTd] = r[3];
       }
     }
-    if (!grid_sync(B, epoch)) return;
+    if (!grid_sync(B, epoch, 9)) return;
     // T3: FP64 degrid of d: W A xnew = W A z + W A d, chi^2(xnew), |W A d|^2
     {
       const double2* mz = A.vm[bs.iz];
@@ -904,7 +931,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         }
       }
     }
-    if (!grid_sync(B, epoch)) return;
+    if (!grid_sync(B, epoch, 10)) return;
 
     // ---------------- control (every workgroup, identical arithmetic)
     // The trials of a round are taken in order, exactly as the reference's
@@ -1252,12 +1279,22 @@ static int persistent_collect(PersistWork& W, const RunParams& P, hipStream_t st
   return 0;
 }
 
+// Returns kCoopRefused (nothing ran) when the runtime refuses the cooperative
+// launch, which it does in a process while another process holds the device's
+// cooperative-launch resources.
 static int timed_launch(const PLaunch& L, void** args, int nblk, hipStream_t st, float* ms) {
   hipEvent_t e0, e1;
   MF_CHECK(hipEventCreate(&e0));
   MF_CHECK(hipEventCreate(&e1));
   MF_CHECK(hipEventRecord(e0, st));
-  MF_CHECK(hipLaunchCooperativeKernel(L.fn, dim3(nblk), dim3(MF_BLOCK), args, (unsigned)L.sh, st));
+  const hipError_t le = hipLaunchCooperativeKernel(L.fn, dim3(nblk), dim3(MF_BLOCK), args, (unsigned)L.sh, st);
+  if (le != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    fprintf(stderr, "[mfista_amd] cooperative launch refused (%s)\n", hipGetErrorName(le));
+    return kCoopRefused;
+  }
   MF_CHECK(hipEventRecord(e1, st));
   MF_CHECK(hipStreamSynchronize(st));
   MF_CHECK(hipEventElapsedTime(ms, e0, e1));
@@ -1265,6 +1302,47 @@ static int timed_launch(const PLaunch& L, void** args, int nblk, hipStream_t st,
   (void)hipEventDestroy(e1);
   MF_CHECK(hipGetLastError());
   return 0;
+}
+
+// Trace summary (MFISTA_PERSISTENT_TRACE=<file>): per stage tag, the slowest
+// workgroup's time from leaving the previous barrier to arriving at this one,
+// and the barrier's tail (last arrival to last release), appended to the file.
+static void write_trace(const char* path, const std::vector<unsigned long long>& tr, const std::vector<int>& tag,
+                        int nbar, int nrow, double clock_khz, const RunStats& rs, const RunParams& P, int M) {
+  static const char* names[] = {"?",        "init-row", "init-col", "init-degrid", "P1-grid", "P2-colinv",
+                                "FGP-iter", "TV-c2r",   "T1-rows",  "T2-cols",     "T3-degrid"};
+  double st[11] = {0}, tl[11] = {0};
+  int cnt[11] = {0};
+  const double us = 1e3 / clock_khz;
+  for (int e = 1; e < nbar; e++) {
+    unsigned long long amax = 0, rmax = 0;
+    long long wmax = 0;
+    for (int r = 0; r < nrow; r++) {
+      const unsigned long long ta = tr[((size_t)e * nrow + r) * 2], rl = tr[((size_t)e * nrow + r) * 2 + 1];
+      const unsigned long long prev = tr[((size_t)(e - 1) * nrow + r) * 2 + 1];
+      amax = std::max(amax, ta);
+      rmax = std::max(rmax, rl);
+      wmax = std::max(wmax, (long long)(ta - prev));
+    }
+    const int k = (tag[e] >= 0 && tag[e] <= 10) ? tag[e] : 0;
+    st[k] += wmax * us;
+    tl[k] += (double)(rmax - amax) * us;
+    cnt[k]++;
+  }
+  FILE* f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "persistent solve %dx%d M=%d: ITER %d trials %d rounds %d kernel %.3f ms, %d barriers traced\n", P.NX,
+          P.NY, M, rs.ITER, rs.total_trials, rs.rounds, rs.kernel_ms, nbar);
+  fprintf(f, "%-12s %7s %14s %13s %10s\n", "stage", "count", "mean stage us", "mean tail us", "total ms");
+  double tot = 0;
+  for (int k = 1; k <= 10; k++) {
+    if (!cnt[k]) continue;
+    fprintf(f, "%-12s %7d %14.2f %13.2f %10.3f\n", names[k], cnt[k], st[k] / cnt[k], tl[k] / cnt[k],
+            (st[k] + tl[k]) * 1e-3);
+    tot += st[k] + tl[k];
+  }
+  fprintf(f, "sum over traced barriers %.3f ms\n\n", tot * 1e-3);
+  fclose(f);
 }
 
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
@@ -1276,6 +1354,15 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   PArgs A;
   if ((rc = persistent_prepare(N, C.pw, P, L.nblk, L.lds_tables, C.persist_hier_barrier, L.pair, st, &A)))
     return rc;
+  const char* trace_path = getenv("MFISTA_PERSISTENT_TRACE");
+  const int tcap = 16384;
+  if (trace_path && *trace_path) {
+    if (C.trace.ensure((size_t)tcap * L.nblk * 16) || C.trace_tag.ensure((size_t)tcap * 4)) return MFISTA_E_HIP;
+    MF_CHECK(hipMemsetAsync(C.trace.p, 0, (size_t)tcap * L.nblk * 16, st));  // unused entries stay 0
+    A.trace = (unsigned long long*)C.trace.p;
+    A.ttag = (int*)C.trace_tag.p;
+    A.tcap = tcap;
+  }
   if (C.pw.pargs.ensure(sizeof(PArgs))) return MFISTA_E_HIP;
   MF_CHECK(hipMemcpyAsync(C.pw.pargs.p, &A, sizeof(PArgs), hipMemcpyHostToDevice, st));
   const PArgs* dA = (const PArgs*)C.pw.pargs.p;
@@ -1286,6 +1373,19 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   C.persist_blocks = L.nblk;
   if ((rc = persistent_collect(C.pw, P, st, out))) return rc;
   MF_CHECK(hipStreamSynchronize(st));
+  if (A.trace) {
+    const long nbar = tcap;
+    std::vector<unsigned long long> tr((size_t)nbar * L.nblk * 2);
+    std::vector<int> tag(nbar);
+    MF_CHECK(hipMemcpy(tr.data(), C.trace.p, tr.size() * 8, hipMemcpyDeviceToHost));
+    MF_CHECK(hipMemcpy(tag.data(), C.trace_tag.p, tag.size() * 4, hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    MF_CHECK(hipGetDevice(&dev));
+    MF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    int used = 0;  // barriers actually passed: their release clocks are set
+    while (used < nbar && tr[((size_t)used * L.nblk) * 2 + 1] != 0) used++;
+    write_trace(trace_path, tr, tag, used, L.nblk, khz > 0 ? khz : 100000, *out, P, C.nu.M);
+  }
   return 0;
 }
 

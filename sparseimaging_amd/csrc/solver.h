@@ -76,6 +76,7 @@ struct Context {
   std::vector<PersistWork> bwork;
   DevMem batch_args, batch_blk;  // PArgs array and workgroup table of a batch launch
   DevMem movie_part;             // partial-sum table shared by the frames of a coupled movie
+  DevMem trace, trace_tag;       // persistent-engine barrier trace (MFISTA_PERSISTENT_TRACE)
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
 
   int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides

@@ -490,10 +490,14 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
     const int engine = opts ? opts->engine : MFISTA_ENGINE_PERSISTENT;
-    if (engine == MFISTA_ENGINE_PERSISTENT && coop)
-      rc = solve_nufft_persistent(C, P, &rs);
-    else
+    rc = kCoopRefused;
+    if (engine == MFISTA_ENGINE_PERSISTENT && coop) rc = solve_nufft_persistent(C, P, &rs);
+    if (rc == kCoopRefused) {  // the stream engine: the same loop, one launch per stage
+      if (engine == MFISTA_ENGINE_PERSISTENT && coop)
+        fprintf(stderr, "[mfista_amd] solving with the stream engine instead\n");
+      rs = RunStats();
       rc = mfista_run<double>(C, op, P, &rs);
+    }
     C.last_steps = rs.steps;
   }
   res->ITER = rs.ITER;
@@ -884,14 +888,18 @@ int mfista_movie_nufft(const double* vis_r, const double* vis_i, const double* u
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   const int iters = st[0].ITER;
   if (iterations) *iterations = iters;
-  if (cost && iters > 0)
-    MF_CHECK(hipMemcpy(cost, C->bwork[0].cost.p, (size_t)std::min(iters, maxiter) * 8, hipMemcpyDeviceToHost));
+  if (cost && iters > 0) {
+    MF_CHECK(hipMemcpyAsync(cost, C->bwork[0].cost.p, (size_t)std::min(iters, maxiter) * 8, hipMemcpyDeviceToHost,
+                            C->stream));
+    MF_CHECK(hipStreamSynchronize(C->stream));
+  }
   // outputs (:203-239): images, FP64 chi^2 / L1 / TSV per frame, d_X, model visibilities
   double chisq = 0, l1 = 0, tsvc = 0;
   std::vector<double> model;
   for (int f = 0; f < nt; f++) {
     PersistWork& W = C->bwork[f];
-    MF_CHECK(hipMemcpy(xout + (size_t)f * nn, W.xo.p, nb, hipMemcpyDeviceToHost));
+    MF_CHECK(hipMemcpyAsync(xout + (size_t)f * nn, W.xo.p, nb, hipMemcpyDeviceToHost, C->stream));
+    MF_CHECK(hipStreamSynchronize(C->stream));
     NufftState& N = C->bset[f].nu;
     std::swap(C->nu, N);
     double chi2 = 0, stats[4] = {0, 0, 0, 0};
@@ -904,7 +912,10 @@ int mfista_movie_nufft(const double* vis_r, const double* vis_i, const double* u
       } else {
         nufft_fwd<double>(*C, (const double*)W.xo.p, nullptr, nullptr, (double2*)C->h_model.p, nullptr);
         model.resize((size_t)2 * mf);
-        if (hipMemcpy(model.data(), C->h_model.p, (size_t)mf * 16, hipMemcpyDeviceToHost) != hipSuccess)
+        // the copy is ordered after the transform on the library's stream (a non-blocking stream)
+        if (hipMemcpyAsync(model.data(), C->h_model.p, (size_t)mf * 16, hipMemcpyDeviceToHost, C->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(C->stream) != hipSuccess)
           rc = MFISTA_E_HIP;
         for (int k = 0; k < mf && !rc; k++) {
           outvis_r[fr[f][k]] = model[2 * k];

@@ -342,8 +342,10 @@ def make_problem(N, M, rng, preset=PRESET_CXX12, noise_frac=0.01):
 
 def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.0, 4.0), batch=32,
                       rel_tol=1e-6, preset=PRESET_CXX12, log=None):
-    """n (input, target) pairs: random skies solved by the GPU MFISTA (batched,
-    cold start, `rel_tol` relative tolerance) with lambda_tsv log-uniform."""
+    """n (input, target) pairs: random skies solved by the GPU MFISTA (cold
+    start, `rel_tol` relative tolerance) with lambda_tsv log-uniform; `batch`
+    problems per launch (1: single solves, the fastest at 256^2, where a solve
+    fills the GPU)."""
     rng = np.random.default_rng(seed)
     X = np.empty((n, N, N, 2), dtype=np.float32)
     T = np.empty((n, N, N), dtype=np.float32)
@@ -360,10 +362,15 @@ def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.
             probs.append(dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"],
                               NX=N, NY=N, maxiter=50000, eps=0.0, lambda_l1=lambda_l1, lambda_tsv=lt, cinit=1e4,
                               nonneg_flag=1))
-        xs, _, _ = mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol)
+        if len(probs) == 1:  # one solve uses the whole GPU
+            q = dict(probs[0])
+            args = [q.pop(k) for k in ("u_dx", "v_dy", "vis_r", "vis_i", "vis_std", "NX", "NY")]
+            xs = [mfista_imaging_core_nufft(*args, preset=preset, rel_tol=rel_tol, **q)[0]]
+        else:
+            xs, _, _ = mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol)
         for j, xk in enumerate(xs):
             T[k0 + j] = xk / ss[j]
-        if log:
+        if log and ((k0 // batch) % max(1, 64 // batch) == 0 or k0 + batch >= n):
             log(f"training set: {min(n, k0 + batch)}/{n} solved ({time.time() - t0:.1f} s)")
     return X, T, lam
 

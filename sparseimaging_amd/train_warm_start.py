@@ -5,7 +5,7 @@
 
 Skies: a ring (random radius, width, centre) plus 1-4 Gaussians at 256^2,
 M = 20000 visibilities on 12 random elliptical arcs, 1 % noise, lambda_l1 =
-1 and lambda_tsv log-uniform in [1, 1e4]; targets are the batched GPU MFISTA
+1 and lambda_tsv log-uniform in [1, 1e4]; targets are the GPU MFISTA
 solutions at a 1e-6 relative tolerance (12-tap preset)."""
 import argparse
 import os
@@ -24,6 +24,7 @@ def main():
     ap.add_argument("--seed", type=int, default=5000)
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--solve-batch", type=int, default=1, help="problems per solver launch")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
@@ -33,7 +34,8 @@ def main():
         print(msg, flush=True)
 
     t0 = time.time()
-    X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=64, log=log)
+    X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=args.solve_batch,
+                                        log=log)
     t_data = time.time() - t0
     net = learn.UNet(base=args.base, levels=args.levels, seed=args.seed)
     t1 = time.time()

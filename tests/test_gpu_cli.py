@@ -39,23 +39,27 @@ def test_nufft_tool_against_library_and_oracle(tmp_path):
     box[:, :6] = 0
     (tmp_path / "box.bin").write_bytes(box.tobytes())
     xo, log = tmp_path / "x.out", tmp_path / "run.log"
-    out = _run("mfista_imaging_nufft", [data, 1.0, 0, 10.0, 1e4, xo, "-nonneg", "-maxiter", 400, "-cl_box",
-                                        tmp_path / "box.bin", "-preset", "cxx12", "-log", log])
+    # run to a tight relative tolerance: the tool's process may get the stream
+    # engine (a second process cannot always take the cooperative launch), whose
+    # trajectory differs from the persistent engine's; converged, the two engines
+    # agree to the parity bar (test_gpu_solve_nufft.py::test_engines_agree)
+    out = _run("mfista_imaging_nufft", [data, 1.0, 0, 10.0, 1e4, xo, "-nonneg", "-maxiter", 20000, "-cl_box",
+                                        tmp_path / "box.bin", "-preset", "cxx12", "-rel_tol", 1e-10, "-log", log])
     x = df.read_vector(xo, 64 * 64).reshape(64, 64)
-    kw = dict(maxiter=400, eps=1e-5, lambda_l1=1.0, lambda_tsv=10.0, cinit=1e4)
+    kw = dict(maxiter=20000, eps=0.0, lambda_l1=1.0, lambda_tsv=10.0, cinit=1e4)
     xl, rl, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
-                                             nonneg_flag=1, cl_box=box, preset=sa.PRESET_CXX12, **kw)
+                                             nonneg_flag=1, cl_box=box, preset=sa.PRESET_CXX12, rel_tol=1e-10, **kw)
     o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"], kw["eps"],
                                   kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"], nonneg=1,
-                                  preset=oracle.PRESET_CXX12, cl_box=box)
+                                  preset=oracle.PRESET_CXX12, cl_box=box, rel_tol=1e-10)
     e_lib = np.linalg.norm(x - xl) / np.linalg.norm(xl)
     e_or = np.linalg.norm(x - o["x"]) / np.linalg.norm(o["x"])
     print(f"  tool vs library relL2 {e_lib:.2e}, vs oracle {e_or:.2e}", flush=True)
-    assert e_lib <= 1e-5 and e_or <= 1e-3
+    assert e_lib <= 1e-3 and e_or <= 1e-3
     assert np.all(x[:, :6] == 0)
     rep = log.read_text()
     assert rep in out  # the log holds the same two report blocks as stdout
-    assert abs(float(_report_value(rep, "cost:")) - rl["finalcost"]) <= 1e-5 * rl["finalcost"]
+    assert abs(float(_report_value(rep, "cost:")) - rl["finalcost"]) <= 1e-6 * rl["finalcost"]
     assert int(_report_value(rep, "size of input vector:")) == 1500
     assert _report_value(rep, "size of image:") == "64 x 64"
 
