@@ -178,17 +178,18 @@ def bench_warm_start(sa, nprob):
     kw = dict(maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, rel_tol=1e-6)
     tc = tw = 0.0
     acc = 0
-    it_c, it_w, rel = [], [], []
+    it_c, it_w, rel, dist = [], [], [], []
     for k in range(nprob):
         P = learn.make_problem(256, 20000, rng)
         args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"])
         t0 = time.perf_counter()
-        _, rc, _ = sa.mfista_imaging_core_nufft(*args, 256, 256, lambda_l1=1.0, lambda_tsv=100.0,
-                                                preset=sa.PRESET_CXX12, **kw)
+        xc, rc, _ = sa.mfista_imaging_core_nufft(*args, 256, 256, lambda_l1=1.0, lambda_tsv=100.0,
+                                                 preset=sa.PRESET_CXX12, **kw)
         tc += time.perf_counter() - t0
         t0 = time.perf_counter()
         _, rw, info = learn.solve_warm(net, *args, 256, lambda_l1=1.0, lambda_tsv=100.0, **kw)
         tw += time.perf_counter() - t0
+        dist.append(float(np.linalg.norm(info["x0"] - xc) / np.linalg.norm(xc)))
         ok = rw["finalcost"] <= rc["finalcost"]
         acc += int(ok)
         it_c.append(rc["ITER"])
@@ -197,6 +198,7 @@ def bench_warm_start(sa, nprob):
     out = {"problems": nprob, "cold_solves_per_s": round(nprob / tc, 3), "warm_solves_per_s": round(nprob / tw, 3),
            "speedup": round(tc / tw, 3), "warm_accepted": acc, "mean_iter_cold": float(np.mean(it_c)),
            "mean_iter_warm": float(np.mean(it_w)), "max_objective_rel_diff_warm_minus_cold": float(np.max(rel)),
+           "mean_start_distance": round(float(np.mean(dist)), 4),  # |x0 - x_cold| / |x_cold|
            "training": {k: net.meta.get(k) for k in ("n", "epochs", "N", "M")}}
     log("bench: learned warm start " + json.dumps(out))
     return out
@@ -336,7 +338,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32/f64",  # FP32 adjoint and gridding; FP64 step transform, iterates, reductions
         "data": "synthetic (seeded ring + 3 Gaussians, 12 elliptical uv arcs)",
         "config": {"workload": "nufft_l1tsv_256_m20000_12tap", "N": N, "M": M, "taps": 12,
                    "lambda_l1": 1.0, "lambda_tsv": 100.0, "rel_tol": 1e-6, "start": "cold",

@@ -47,6 +47,10 @@ int unet_head_bwd(const float *in, int C, const uint16_t *w, const float *dout, 
                   float *dw, float *db, long P, void *stream);
 /* *loss = mean (y - t)^2 (FP64), dy = 2 (y - t) / n (dy may be NULL) */
 int unet_mse(const float *y, const float *t, float *dy, double *loss, long n, void *stream);
+/* per-sample weighted squared error: loss = sum_b wt[b] sum_i (y - t)^2 / n over
+ * samples of `per` elements (n a multiple of per), and its gradient (dy may be NULL) */
+int unet_mse_weighted(const float *y, const float *t, const float *wt, long per, float *dy, double *loss,
+                      long n, void *stream);
 /* one Adam step on the FP32 master weights w; refreshes the BF16 copy w16 */
 int unet_adam(float *w, uint16_t *w16, const float *g, float *m, float *v, long n, float lr,
               float beta1, float beta2, float eps, int step, void *stream);

@@ -93,11 +93,20 @@ _fp = C.POINTER(C.c_float)
 
 
 def lib():
-    """Load libmfista_amd.so; raises if it has not been built."""
+    """Load libmfista_amd.so; raises if it has not been built.
+
+    torch (when installed) is imported first: its wheel ships its own HIP
+    runtime, and a process must load only one.  Imported first, torch's runtime
+    is the one the library binds to; loaded after the library has started the
+    system runtime, torch cannot open the GPU ("No HIP GPUs are available")."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(LIB_PATH)
         _declare(_lib)
     return _lib

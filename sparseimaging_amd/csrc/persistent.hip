@@ -685,7 +685,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         }
         return make_float2((float)(V.x * w - m.x), (float)(V.y * w - m.y));
       };
-      grid_chunks<(MSPC <= 8) ? 2 : 4>(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent,
+      grid_chunks<(MSPC <= 8) ? 1 : 4>(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent,
                                          A.Hg, lds);
       if (!grid_sync(B, epoch, 4)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles

@@ -92,7 +92,9 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   g.MSP = MSP;
   g.Lx = 2 * NX;
   g.Ly = 2 * NY;
-  g.TX = 32;
+  // gridding tiles: 16 x 16 cells for the 12-tap footprint (one cell per
+  // thread; the footprint covers 56 % of the tile), 32 x 32 for 24 taps
+  g.TX = (MSP <= 8) ? 16 : 32;
   g.TY = (MSP <= 8) ? 16 : 32;
   g.ntx = (g.Lx + g.TX - 1) / g.TX;
   g.nty = (NY + 1 + g.TY - 1) / g.TY;

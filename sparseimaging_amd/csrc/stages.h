@@ -284,10 +284,15 @@ __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<
   const int lane = threadIdx.x & (G - 1);
   const R t3x = (R)g.t3x, t3y = (R)g.t3y;
   C acc = mk<R>(0, 0);
+  // lane b evaluates the row weight of tap row b once; the lanes of the group
+  // read it by shuffle (the weights depend on the row, not on the lane's
+  // column).  Inside the branch every source lane b < S of an active lane's
+  // group is active too (the group shares one visibility).
+  const C a = (s < g.M) ? vp.a[s] : mk<R>(0, 0);
+  const R l = (R)(lane - MSPC + 1);
+  const R gyl = dev_exp(l * a.y - l * l * t3y);
   if (s < g.M && lane < S) {
     const int2 m = vp.mxy[s];
-    const C a = vp.a[s];
-    const R l = (R)(lane - MSPC + 1);
     const R gx = dev_exp(l * a.x - l * l * t3x);
     const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
     C gv[S];
@@ -295,8 +300,7 @@ __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<
     for (int b = 0; b < S; b++) gv[b] = col[(size_t)b * g.Lx];
 #pragma unroll
     for (int b = 0; b < S; b++) {
-      const R lp = (R)(b - MSPC + 1);
-      const R gy = dev_exp(lp * a.y - lp * lp * t3y);
+      const R gy = __shfl(gyl, b, G);
       acc.x += gy * gv[b].x;
       acc.y += gy * gv[b].y;
     }
@@ -396,10 +400,11 @@ __device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisPa
   const int lane = threadIdx.x & (G - 1);
   const double t3x = g.t3x, t3y = g.t3y;
   double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
+  const double2 a = (s < g.M) ? vp.a[s] : make_double2(0, 0);
+  const double l = (double)(lane - MSPC + 1);
+  const double gyl = dev_exp(l * a.y - l * l * t3y);  // row weight of tap row `lane`, shared by shuffle
   if (s < g.M && lane < S) {
     const int2 m = vp.mxy[s];
-    const double2 a = vp.a[s];
-    const double l = (double)(lane - MSPC + 1);
     const double gx = dev_exp(l * a.x - l * l * t3x);
     const size_t off = (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
     const double2* c0 = Gf0 + off;
@@ -407,8 +412,7 @@ __device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisPa
 #pragma unroll
     for (int b = 0; b < S; b++) {
       const double2 u0 = c0[(size_t)b * g.Lx], u1 = c1[(size_t)b * g.Lx];
-      const double lp = (double)(b - MSPC + 1);
-      const double gy = dev_exp(lp * a.y - lp * lp * t3y);
+      const double gy = __shfl(gyl, b, G);
       a0.x += gy * u0.x;
       a0.y += gy * u0.y;
       a1.x += gy * u1.x;

@@ -494,6 +494,27 @@ __global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, 
   if (threadIdx.x == 0) atomicAdd(loss, red[0] / (double)n);
 }
 
+// Per-sample weighted squared error: loss = sum_b wt[b] |y_b - t_b|^2 / n and
+// its gradient; sample b is elements [b per, (b + 1) per)
+__global__ void k_mse_w(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ wt,
+                        long per, float* __restrict__ dy, double* __restrict__ loss, long n) {
+  __shared__ double red[NT];
+  double s = 0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float w = wt[i / per];
+    const float d = y[i] - t[i];
+    s += (double)w * d * d;
+    if (dy) dy[i] = 2.f * w * d / (float)n;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] / (double)n);
+}
+
 // Adam on the FP32 master weights; refreshes the BF16 copy
 __global__ void k_adam(float* __restrict__ w, uint16_t* __restrict__ w16, const float* __restrict__ g,
                        float* __restrict__ m, float* __restrict__ v, long n, float lr, float b1, float b2,
@@ -685,6 +706,16 @@ int unet_mse(const float* y, const float* t, float* dy, double* loss, long n, vo
   UN_CHECK(hipMemsetAsync(loss, 0, sizeof(double), (hipStream_t)stream));
   hipLaunchKernelGGL(k_mse, dim3((unsigned)std::min<long>(grid1(n), 1024)), dim3(NT), 0, (hipStream_t)stream, y, t, dy,
                      loss, n);
+  UN_CHECK(hipGetLastError());
+  return 0;
+}
+
+int unet_mse_weighted(const float* y, const float* t, const float* wt, long per, float* dy, double* loss, long n,
+                      void* stream) {
+  if (!y || !t || !wt || !loss || n <= 0 || per <= 0 || n % per != 0) return UNET_E_ARG;
+  UN_CHECK(hipMemsetAsync(loss, 0, sizeof(double), (hipStream_t)stream));
+  hipLaunchKernelGGL(k_mse_w, dim3((unsigned)std::min<long>(grid1(n), 1024)), dim3(NT), 0, (hipStream_t)stream, y, t,
+                     wt, per, dy, loss, n);
   UN_CHECK(hipGetLastError());
   return 0;
 }

@@ -55,6 +55,7 @@ def _lib():
             "unet_head_fwd": [_vp, i, _vp, _vp, _vp, l, i, _vp],
             "unet_head_bwd": [_vp, i, _vp, _vp, _vp, _vp, _vp, l, _vp],
             "unet_mse": [_vp, _vp, _vp, _vp, l, _vp],
+            "unet_mse_weighted": [_vp, _vp, _vp, l, _vp, _vp, l, _vp],
             "unet_adam": [_vp, _vp, _vp, _vp, _vp, l, f, f, f, f, i, _vp],
             "unet_to_bf16": [_vp, _vp, l, _vp],
         }.items():
@@ -292,10 +293,20 @@ class UNet:
                             self.m[name].data_ptr(), self.v[name].data_ptr(), self.w32[name].numel(), lr, beta1,
                             beta2, eps, self.step_count, st))
 
-    def train_step(self, x, t, lr):
+    def loss_weighted(self, y, t, wt):
+        """Per-sample weighted squared error: sum_b wt[b] |y_b - t_b|^2 / numel and its gradient."""
+        L = _lib()
+        dy = self._empty(*y.shape)
+        _ck(L.unet_mse_weighted(y.data_ptr(), t.data_ptr(), wt.data_ptr(), y[0].numel(), dy.data_ptr(),
+                                self.loss_buf.data_ptr(), y.numel(), self._stream()))
+        return float(self.loss_buf.item()), dy
+
+    def train_step(self, x, t, lr, wt=None):
+        """One Adam step on a batch; wt (per-sample weights, device float32)
+        selects the weighted loss."""
         self.zero_grad()
         y = self.forward(x)
-        loss, dy = self.loss(y, t)
+        loss, dy = self.loss(y, t) if wt is None else self.loss_weighted(y, t, wt)
         self.backward(dy)
         self.adam(lr)
         return loss
@@ -375,8 +386,12 @@ def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.
     return X, T, lam
 
 
-def train(net, X, T, epochs=10, batch=8, lr=1e-3, lr_final=1e-4, log=None):
-    """Adam with a geometric learning-rate decay; returns the loss history."""
+def train(net, X, T, epochs=10, batch=8, lr=1e-3, lr_final=1e-4, log=None, relative=False):
+    """Adam with a geometric learning-rate decay; returns the loss history.
+    relative=True weights sample b by numel / |T_b|^2, so the loss is the mean
+    squared relative error: MFISTA's iteration count depends on the start's
+    relative distance to the solution, and plain MSE on mostly-zero images
+    rewards a blurred average."""
     torch = net.torch
     n = X.shape[0]
     steps = epochs * ((n + batch - 1) // batch)
@@ -391,7 +406,11 @@ def train(net, X, T, epochs=10, batch=8, lr=1e-3, lr_final=1e-4, log=None):
             x = torch.from_numpy(X[idx]).to(net.dev)
             t = torch.from_numpy(T[idx]).to(net.dev)
             cur = lr * (lr_final / lr) ** (k / max(steps - 1, 1))
-            tot += net.train_step(x, t, cur) * len(idx)
+            wt = None
+            if relative:
+                tn = np.maximum((T[idx].reshape(len(idx), -1).astype(np.float64) ** 2).sum(1), 1e-30)
+                wt = torch.from_numpy((T[idx][0].size / tn).astype(np.float32)).to(net.dev)
+            tot += net.train_step(x, t, cur, wt) * len(idx)
             k += 1
         hist.append(tot / n)
         if log:

@@ -25,6 +25,8 @@ def main():
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--solve-batch", type=int, default=1, help="problems per solver launch")
+    ap.add_argument("--lr-final", type=float, default=None, help="final learning rate (default lr / 10)")
+    ap.add_argument("--mse", action="store_true", help="plain MSE instead of the relative squared error")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
@@ -33,17 +35,19 @@ def main():
     def log(msg):
         print(msg, flush=True)
 
+    net = learn.UNet(base=args.base, levels=args.levels, seed=args.seed)
     t0 = time.time()
     X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=args.solve_batch,
                                         log=log)
     t_data = time.time() - t0
-    net = learn.UNet(base=args.base, levels=args.levels, seed=args.seed)
     t1 = time.time()
-    hist = learn.train(net, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr, lr_final=args.lr / 10, log=log)
+    hist = learn.train(net, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr,
+                       lr_final=args.lr_final if args.lr_final else args.lr / 10, log=log, relative=not args.mse)
     t_train = time.time() - t1
     out = args.out or learn.WEIGHTS
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     net.save(out, meta={"n": args.n, "N": args.N, "M": args.M, "epochs": args.epochs, "seed": args.seed,
+                        "loss_kind": "mse" if args.mse else "relative",
                         "loss": hist, "data_s": round(t_data, 1), "train_s": round(t_train, 1)})
     log(f"saved {out}: data {t_data:.1f} s, training {t_train:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
     return 0

@@ -144,6 +144,25 @@ def test_pool_upsample_head_mse_adam():
     assert np.allclose(bits.view(np.float32), U.to_bf16(w32.cpu().numpy()), rtol=0, atol=0)
 
 
+def test_weighted_mse():
+    """unet_mse_weighted: sum_b w_b |y_b - t_b|^2 / n and its gradient, against numpy."""
+    rng = np.random.default_rng(4)
+    B, per = 3, 50
+    y, t = rng.standard_normal((B, per)), rng.standard_normal((B, per))
+    w = np.array([0.5, 2.0, 1.5])
+    L = learn._lib()
+    st = learn._vp(torch.cuda.current_stream(dev).cuda_stream)
+    gy, gt, gw = _t(y), _t(t), _t(w)
+    dy = torch.empty_like(gy)
+    loss = torch.zeros(1, dtype=torch.float64, device=dev)
+    assert L.unet_mse_weighted(gy.data_ptr(), gt.data_ptr(), gw.data_ptr(), per, dy.data_ptr(), loss.data_ptr(),
+                               B * per, st) == 0
+    ref = float(np.sum(w[:, None] * (y - t) ** 2) / (B * per))
+    print(f"case weighted MSE: {loss.item():.8e} vs {ref:.8e}", flush=True)
+    assert abs(loss.item() - ref) <= 1e-6 * ref
+    assert np.allclose(dy.cpu().numpy(), 2 * w[:, None] * (y - t) / (B * per), rtol=1e-5, atol=1e-8)
+
+
 def test_network_forward_and_gradients_match_oracle():
     rng = np.random.default_rng(3)
     base, levels = 4, 3
