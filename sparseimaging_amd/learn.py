@@ -313,16 +313,22 @@ class UNet:
 
 
 # ---------------------------------------------------------------------- data
-def net_input(u, v, vis_r, vis_i, vis_std, N, lambda_tsv, preset=PRESET_CXX12):
-    """(2, N, N) network input and the flux scale s (see the module docstring)."""
+def net_input(u, v, vis_r, vis_i, vis_std, N, lambda_tsv, preset=PRESET_CXX12, channels=2):
+    """(N, N, channels) network input and the flux scale s (see the module
+    docstring).  channels=3 adds the dirty beam Re(A^H W^2 1) / sum(w^2) (the
+    point-spread function of this (u,v) coverage, peak 1 at the centre): the
+    coverage differs from problem to problem, and without it the network
+    cannot know what the dirty image is convolved with."""
     w2 = 1.0 / np.asarray(vis_std) ** 2
     dirty = nufft_adjoint(u, v, w2 * (np.asarray(vis_r) + 1j * np.asarray(vis_i)), N, N, preset=preset, fp64=True)
     sw = float(w2.sum())
     peak = float(dirty.max())
     s = max(peak, 1e-30) / sw
-    x = np.empty((N, N, 2), dtype=np.float32)
+    x = np.empty((N, N, channels), dtype=np.float32)
     x[..., 0] = dirty / (s * sw)
     x[..., 1] = (np.log10(lambda_tsv) - 2.0) / 2.0
+    if channels == 3:
+        x[..., 2] = nufft_adjoint(u, v, w2.astype(np.complex128), N, N, preset=preset, fp64=True) / sw
     return x, s
 
 
@@ -352,13 +358,13 @@ def make_problem(N, M, rng, preset=PRESET_CXX12, noise_frac=0.01):
 
 
 def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.0, 4.0), batch=32,
-                      rel_tol=1e-6, preset=PRESET_CXX12, log=None):
+                      rel_tol=1e-6, preset=PRESET_CXX12, log=None, channels=2):
     """n (input, target) pairs: random skies solved by the GPU MFISTA (cold
     start, `rel_tol` relative tolerance) with lambda_tsv log-uniform; `batch`
     problems per launch (1: single solves, the fastest at 256^2, where a solve
     fills the GPU)."""
     rng = np.random.default_rng(seed)
-    X = np.empty((n, N, N, 2), dtype=np.float32)
+    X = np.empty((n, N, N, channels), dtype=np.float32)
     T = np.empty((n, N, N), dtype=np.float32)
     lam = np.empty(n)
     t0 = time.time()
@@ -368,7 +374,7 @@ def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.
             P = make_problem(N, M, rng, preset)
             lt = 10.0 ** rng.uniform(*log10_tsv)
             lam[k] = lt
-            X[k], s = net_input(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], N, lt, preset)
+            X[k], s = net_input(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], N, lt, preset, channels)
             ss.append(s)
             probs.append(dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"],
                               NX=N, NY=N, maxiter=50000, eps=0.0, lambda_l1=lambda_l1, lambda_tsv=lt, cinit=1e4,
@@ -459,7 +465,7 @@ def solve_warm(net, u, v, vis_r, vis_i, vis_std, N, lambda_l1=1.0, lambda_tsv=10
     and wins if its converged objective is lower (the acceptance rule).
     Returns (x, RESULT dict, info) with info["warm_accepted"], info["alpha"]
     and info["x0"] (the starting image)."""
-    x_in, s = net_input(u, v, vis_r, vis_i, vis_std, N, lambda_tsv, preset)
+    x_in, s = net_input(u, v, vis_r, vis_i, vis_std, N, lambda_tsv, preset, net.in_ch)
     t0 = time.perf_counter()
     x0 = predict(net, x_in, s)
     alpha = ray_scale(u, v, vis_r, vis_i, vis_std, x0, lambda_l1, lambda_tsv, preset)

