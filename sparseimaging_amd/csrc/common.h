@@ -80,6 +80,40 @@ __host__ __device__ inline C cscale(C a, R s) {
 __device__ inline float dev_exp(float a) { return __expf(a); }
 __device__ inline double dev_exp(double a) { return exp(a); }
 
+// Loads and stores through the global address space.  Pointers that the
+// kernels read out of argument structs are generic, so plain dereferences
+// compile to FLAT instructions.  A FLAT access counts against both the
+// vector-memory and the LDS counter, so every wait for an LDS access or a
+// cross-lane shuffle (s_waitcnt lgkmcnt) also waits for every FLAT load in
+// flight.  These helpers issue GLOBAL instructions, which only the
+// vector-memory counter tracks.  The pointer must address global memory.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p) {
+  if constexpr (sizeof(T) == 16) {
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) v4*)p);
+  } else if constexpr (sizeof(T) == 8) {
+    typedef int v2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) v2*)p);
+  } else {
+    static_assert(sizeof(T) == 4, "gld: 4, 8 or 16 bytes");
+    return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) int*)p);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* p, const T& v) {
+  if constexpr (sizeof(T) == 16) {
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    *(__attribute__((address_space(1))) v4*)p = __builtin_bit_cast(v4, v);
+  } else if constexpr (sizeof(T) == 8) {
+    typedef int v2 __attribute__((ext_vector_type(2)));
+    *(__attribute__((address_space(1))) v2*)p = __builtin_bit_cast(v2, v);
+  } else {
+    static_assert(sizeof(T) == 4, "gst: 4, 8 or 16 bytes");
+    *(__attribute__((address_space(1))) int*)p = __builtin_bit_cast(int, v);
+  }
+}
+
 // Fixed-order block reduction of NV doubles per thread; thread 0 of the block
 // receives the sums.  blockDim.x must be a multiple of 64 and <= 1024.
 template <int NV>

@@ -238,9 +238,11 @@ __device__ inline bool bar_setup(BarCtx& B, unsigned long long* bar, unsigned nb
   return fail == 0;
 }
 
-// Per-workgroup partial sums: one row of kPW doubles per workgroup, written
-// by the stage that produces them and reduced (in workgroup order, the same
-// in every workgroup) by the control step.
+// Per-workgroup partial sums: kPW slots per workgroup, written by the stage
+// that produces them and reduced by the control step of every workgroup, in
+// the same fixed order everywhere.  Stored slot-major (part[slot * nrow + row])
+// so that a wave reads a slot of all rows with coalesced loads: row-major, each
+// lane of a load touched its own cache line and the reduction took ~13 us.
 constexpr int kPW = 16;
 // trial k of a round uses slots 6 k + (0 .. 5)
 // (kPTx, kPTd: this frame's share of d_X(xnew) and d_X(d) in a coupled movie)
@@ -249,16 +251,23 @@ enum { kPD2 = 0, kPL1 = 1, kPReg = 2, kPTsvd = 3, kPChi = 4, kPLip = 5, kPTx = 6
 enum { kPIl1 = 0, kPItsv = 1, kPItv = 2, kPIdx = 3, kPIchi = 4 };
 
 template <int NV>
-__device__ inline void part_reduce(const double* part, int nb, const int (&slot)[NV], double (&out)[NV],
+__device__ inline void part_reduce(const double* part, int nrow, const int (&slot)[NV], double (&out)[NV],
                                    double* red) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
-  for (int q = 0; q < NV; q++) out[q] = 0.0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    const double* row = part + (size_t)i * kPW;
-#pragma unroll
-    for (int q = 0; q < NV; q++) out[q] += row[slot[q]];
+  for (int q = 0; q < NV; q++) {
+    if (q % nw != wid) continue;  // wave wid reduces slots wid, wid + nw, ...
+    const double* col = part + (size_t)slot[q] * nrow;
+    double s = 0.0;
+#pragma unroll 8
+    for (int i = lane; i < nrow; i += 64) s += gld(col + i);
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) red[q] = s;
   }
-  block_sum<NV>(out, red);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; q++) out[q] = red[q];
+  __syncthreads();
 }
 
 struct BlockState {
@@ -487,6 +496,11 @@ __host__ __device__ inline size_t grid_lds_bytes(int TX, int TY) {
   return (size_t)MF_BLOCK * sizeof(GridEnt) + (size_t)kGB * 16;
 }
 
+// Visibility records of the paired degrid tasks cached in LDS per workgroup
+// (64 bytes each): all tasks of a workgroup while n_deg <= 4 nb (12 taps) or
+// 8 nb (24 taps); later tasks load their records.
+constexpr int kDVisCap = 64;
+
 // LDS bytes of the staged twiddle (FP32 and FP64, both axes) and E4 tables
 __host__ __device__ inline size_t table_lds_bytes(int Lx, int Ly, int NX, int NY) {
   return (size_t)(Lx + Ly) * (16 + 8) + (size_t)(NX + NY) * (8 + 4);
@@ -508,7 +522,9 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   unsigned long long epoch = 0;  // barriers passed by this workgroup
   const bool movie = A.ngb > 0;
   const unsigned ngrp = movie ? (unsigned)A.ngb : nb;  // workgroups of the barrier and the reductions
-  double* prow = A.part + (size_t)(movie ? A.gb0 + (int)lb : (int)lb) * kPW;
+  // this workgroup's column of the slot-major partial table
+  double* prow = A.part + (movie ? A.gb0 + (int)lb : (int)lb);
+  const size_t pstr = ngrp;
 
   if (!bar_setup(B, A.bar, ngrp, A.hier_barrier)) return;
   if (threadIdx.x == 0) {
@@ -562,6 +578,16 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     lds += (table_lds_bytes(g.Lx, g.Ly, NX, NY) + 255) & ~(size_t)255;
     __syncthreads();
   }
+  // the visibility records of this workgroup's degrid tasks
+  constexpr int kVPT = MF_BLOCK / DegridShape<MSPC>::G;  // visibilities per task
+  constexpr int kTC = kDVisCap / kVPT;                    // tasks cached
+  DVis* dvc = reinterpret_cast<DVis*>(lds);
+  lds += kDVisCap * sizeof(DVis);
+  for (int q = threadIdx.x; q < kDVisCap; q += blockDim.x) {
+    const int k = q / kVPT, t = lb + k * (int)nb;
+    if (t < A.n_deg) dvc[q] = dvis_load(A.pd, g, t * kVPT + q % kVPT);
+  }
+  __syncthreads();
   float2* sm32 = reinterpret_cast<float2*>(lds);
   double2* sm64 = reinterpret_cast<double2*>(lds);
 
@@ -597,10 +623,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     block_sum<4>(st, red);
     block_sum<1>(sdx, red);
     if (threadIdx.x == 0) {
-      prow[kPIl1] = st[0];
-      prow[kPItsv] = st[1];
-      prow[kPItv] = st[2];
-      prow[kPIdx] = sdx[0];
+      prow[(kPIl1) * pstr] = st[0];
+      prow[(kPItsv) * pstr] = st[1];
+      prow[(kPItv) * pstr] = st[2];
+      prow[(kPIdx) * pstr] = sdx[0];
     }
   }
   if (!grid_sync(B, epoch, 1)) return;
@@ -611,7 +637,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     double v[2] = {0, 0};
     for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, nullptr, A.vm[0], A.vm[2], v);
     block_sum<2>(v, red);
-    if (threadIdx.x == 0) prow[kPIchi] = v[0];
+    if (threadIdx.x == 0) prow[(kPIchi) * pstr] = v[0];
   }
   if (!grid_sync(B, epoch, 3)) return;
   {
@@ -823,10 +849,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
       block_sum<4>(v, red);
       if (threadIdx.x == 0) {
-        prow[kPD2] = v[0];
-        prow[kPL1] = v[1];
-        prow[kPTrial + kPD2] = v[2];
-        prow[kPTrial + kPL1] = v[3];
+        prow[(kPD2) * pstr] = v[0];
+        prow[(kPL1) * pstr] = v[1];
+        prow[(kPTrial + kPD2) * pstr] = v[2];
+        prow[(kPTrial + kPL1) * pstr] = v[3];
       }
     } else {
       double v[2] = {0, 0};
@@ -859,8 +885,8 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
       block_sum<2>(v, red);
       if (threadIdx.x == 0) {
-        prow[kPD2] = v[0];
-        prow[kPL1] = v[1];
+        prow[(kPD2) * pstr] = v[0];
+        prow[(kPL1) * pstr] = v[1];
       }
     }
     if (!grid_sync(B, epoch, 8)) return;
@@ -880,14 +906,14 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
       block_sum<8>(r, red);
       if (threadIdx.x == 0) {
-        prow[kPReg] = r[0];
-        prow[kPTsvd] = r[1];
-        prow[kPTrial + kPReg] = r[2];
-        prow[kPTrial + kPTsvd] = r[3];
-        prow[kPTx] = r[4];
-        prow[kPTd] = r[5];
-        prow[kPTrial + kPTx] = r[6];
-        prow[kPTrial + kPTd] = r[7];
+        prow[(kPReg) * pstr] = r[0];
+        prow[(kPTsvd) * pstr] = r[1];
+        prow[(kPTrial + kPReg) * pstr] = r[2];
+        prow[(kPTrial + kPTsvd) * pstr] = r[3];
+        prow[(kPTx) * pstr] = r[4];
+        prow[(kPTd) * pstr] = r[5];
+        prow[(kPTrial + kPTx) * pstr] = r[6];
+        prow[(kPTrial + kPTd) * pstr] = r[7];
       }
     } else {
       for (int t = lb; t < A.n_ky; t += nb)
@@ -899,10 +925,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
       block_sum<4>(r, red);
       if (threadIdx.x == 0) {
-        prow[kPReg] = r[0];
-        prow[kPTsvd] = r[1];
-        prow[kPTx] = r[2];
-        prow[kPTd] = r[3];
+        prow[(kPReg) * pstr] = r[0];
+        prow[(kPTsvd) * pstr] = r[1];
+        prow[(kPTx) * pstr] = r[2];
+        prow[(kPTd) * pstr] = r[3];
       }
     }
     if (!grid_sync(B, epoch, 9)) return;
@@ -911,14 +937,15 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const double2* mz = A.vm[bs.iz];
       if (pr) {
         double v[4] = {0, 0, 0, 0};
-        for (int t = lb; t < A.n_deg; t += nb)
-          st_degrid_model_pair<MSPC>(t, g, A.pd, A.Gf64, A.Gf64b, mz, A.vm[bs.in], A.vm[bs.in1], v);
+        for (int t = lb, k = 0; t < A.n_deg; t += nb, k++)
+          st_degrid_model_pair<MSPC>(t, g, A.pd, k < kTC ? dvc + k * kVPT : nullptr, A.Gf64, A.Gf64b, mz,
+                                     A.vm[bs.in], A.vm[bs.in1], v);
         block_sum<4>(v, red);
         if (threadIdx.x == 0) {
-          prow[kPChi] = v[0];
-          prow[kPLip] = v[1];
-          prow[kPTrial + kPChi] = v[2];
-          prow[kPTrial + kPLip] = v[3];
+          prow[(kPChi) * pstr] = v[0];
+          prow[(kPLip) * pstr] = v[1];
+          prow[(kPTrial + kPChi) * pstr] = v[2];
+          prow[(kPTrial + kPLip) * pstr] = v[3];
         }
       } else {
         double v[2] = {0, 0};
@@ -926,8 +953,8 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
           st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, mz, A.vm[bs.in], nullptr, v);
         block_sum<2>(v, red);
         if (threadIdx.x == 0) {
-          prow[kPChi] = v[0];
-          prow[kPLip] = v[1];
+          prow[(kPChi) * pstr] = v[0];
+          prow[(kPLip) * pstr] = v[1];
         }
       }
     }
@@ -1064,16 +1091,21 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
 // The arguments are read from device memory rather than the kernel-argument
 // segment: passed by value they would be held in scalar registers, whose
 // spills into vector registers cost the 24-tap kernel its second workgroup
-// per CU.
+// per CU.  The 24-tap kernel also needs an explicit bound of two waves per
+// SIMD (two workgroups per CU) to keep them; the 12-tap kernel fits without one,
+// and the bound's scratch spills in its control step cost more than they save.
 template <int MSPC>
-__global__ void __launch_bounds__(MF_BLOCK) k_mfista_persistent(const PArgs* __restrict__ A) {
+constexpr int kMinWaves = (MSPC > 8) ? 2 : 1;
+
+template <int MSPC>
+__global__ void __launch_bounds__(MF_BLOCK, kMinWaves<MSPC>) k_mfista_persistent(const PArgs* __restrict__ A) {
   mfista_body<MSPC>(*A, blockIdx.x, gridDim.x);
 }
 
 // A batch of independent solves in one cooperative launch: blk[b] =
 // (problem, workgroup index within the problem) for workgroup b.
 template <int MSPC>
-__global__ void __launch_bounds__(MF_BLOCK) k_mfista_batch(const PArgs* __restrict__ args,
+__global__ void __launch_bounds__(MF_BLOCK, kMinWaves<MSPC>) k_mfista_batch(const PArgs* __restrict__ args,
                                                            const int2* __restrict__ blk) {
   const int2 pb = blk[blockIdx.x];
   if (pb.x < 0) return;  // a workgroup slot no problem was dealt
@@ -1100,9 +1132,9 @@ static int persistent_config(Context& C, int MSP, const std::vector<const NufftS
     L->fn = (MSP == 6) ? (const void*)k_mfista_batch<6> : (const void*)k_mfista_batch<12>;
   else
     L->fn = (MSP == 6) ? (const void*)k_mfista_persistent<6> : (const void*)k_mfista_persistent<12>;
-  // dynamic LDS: [twiddle + E4 tables, when they fit][stage scratch: FP64
-  // row/column FFT (two sequences per transform in a paired round) or the
-  // batched gridder's entries].  Paired rounds are used when their larger
+  // dynamic LDS: [twiddle + E4 tables, when they fit][degrid visibility
+  // records][stage scratch: FP64 row/column FFT (two sequences per transform
+  // in a paired round) or the batched gridder's entries].  Paired rounds are used when their larger
   // scratch still leaves the wanted number of workgroups per CU.
   int dev = 0, ncu = 0, lds_max = 0;
   MF_CHECK(hipGetDevice(&dev));
@@ -1117,6 +1149,7 @@ static int persistent_config(Context& C, int MSP, const std::vector<const NufftS
       const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
       size_t s1 = (size_t)(pair ? 4 : 2) * Lm * sizeof(double2);
       if (grid_lds_bytes(g.TX, g.TY) > s1) s1 = grid_lds_bytes(g.TX, g.TY);
+      s1 += kDVisCap * sizeof(DVis);
       if (s1 > sh) sh = s1;
       const size_t t1 = (table_lds_bytes(g.Lx, g.Ly, g.NX, g.NY) + 255) & ~(size_t)255;
       if (t1 > tables) tables = t1;

@@ -389,34 +389,86 @@ __device__ inline void st_degrid_model(int task, const GridGeom& g, const VisPar
   }
 }
 
-// Tap gather of two spectra with the same taps (degrid_core on Gf0 and Gf1).
+// The constants of one visibility that the degridder needs: tap slopes,
+// grid cell, E1 / (4 NX NY), weight and data (reflected frame).  They do not
+// change during a solve, so the persistent kernel caches the records of its
+// workgroup's degrid tasks in LDS once per launch (dvis_load for the rest).
+struct alignas(16) DVis {
+  double ax, ay, e1s, w, vx, vy;
+  int mx, my;
+};
+
+__device__ inline DVis dvis_load(const VisParams<double>& vp, const GridGeom& g, int s) {
+  DVis r{};
+  if (s < g.M) {
+    const double2 a = gld(vp.a + s), V = gld(vp.vis + s);
+    const int2 m = gld(vp.mxy + s);
+    r.ax = a.x;
+    r.ay = a.y;
+    r.e1s = gld(vp.e1 + s) * g.inv_MM;
+    r.w = gld(vp.w + s);
+    r.vx = V.x;
+    r.vy = V.y;
+    r.mx = m.x;
+    r.my = m.y;
+  }
+  return r;
+}
+
+// Tap gather of two spectra with the same taps (degrid_core on Gf0 and Gf1)
+// for visibility s with constants r.  The tap rows are loaded in groups of up
+// to 12 (both spectra, 4 VGPRs per row and spectrum).  For the 24-tap kernel
+// a scheduling barrier keeps each group's loads ahead of its FMAs: the gather
+// is latency-bound, and left to itself the scheduler interleaved loads and
+// FMAs, which cost that stage ~30 %.  The 12-tap kernel stays unpinned: there
+// the barrier's extra live registers cost the kernel its second workgroup per
+// CU (measured on the MI355X).
 template <int MSPC>
-__device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisParams<double>& vp,
+__device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
                                         const double2* __restrict__ Gf0, const double2* __restrict__ Gf1,
-                                        int& s, double2& F0, double2& F1) {
+                                        double2& F0, double2& F1) {
   constexpr int S = DegridShape<MSPC>::S, G = DegridShape<MSPC>::G;
-  const int gid = task * blockDim.x + threadIdx.x;
-  s = gid / G;
+  constexpr int RG = (S <= 12) ? S : 12;  // rows per load group
+  static_assert(S % RG == 0, "tap rows split into whole groups");
   const int lane = threadIdx.x & (G - 1);
   const double t3x = g.t3x, t3y = g.t3y;
   double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
-  const double2 a = (s < g.M) ? vp.a[s] : make_double2(0, 0);
   const double l = (double)(lane - MSPC + 1);
-  const double gyl = dev_exp(l * a.y - l * l * t3y);  // row weight of tap row `lane`, shared by shuffle
+  const double gyl = dev_exp(l * r.ay - l * l * t3y);  // row weight of tap row `lane`, shared by shuffle
   if (s < g.M && lane < S) {
-    const int2 m = vp.mxy[s];
-    const double gx = dev_exp(l * a.x - l * l * t3x);
-    const size_t off = (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
+    const double gx = dev_exp(l * r.ax - l * l * t3x);
+    const size_t off = (size_t)r.my * g.Lx + (r.mx + g.NX + lane - MSPC + 1);
     const double2* c0 = Gf0 + off;
     const double2* c1 = Gf1 + off;
+    if constexpr (MSPC > 8) {
 #pragma unroll
-    for (int b = 0; b < S; b++) {
-      const double2 u0 = c0[(size_t)b * g.Lx], u1 = c1[(size_t)b * g.Lx];
-      const double gy = __shfl(gyl, b, G);
-      a0.x += gy * u0.x;
-      a0.y += gy * u0.y;
-      a1.x += gy * u1.x;
-      a1.y += gy * u1.y;
+      for (int b0 = 0; b0 < S; b0 += RG) {
+        double2 u0[RG], u1[RG];
+#pragma unroll
+        for (int b = 0; b < RG; b++) {
+          u0[b] = gld(c0 + (size_t)(b0 + b) * g.Lx);
+          u1[b] = gld(c1 + (size_t)(b0 + b) * g.Lx);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int b = 0; b < RG; b++) {
+          const double gy = __shfl(gyl, b0 + b, G);
+          a0.x += gy * u0[b].x;
+          a0.y += gy * u0[b].y;
+          a1.x += gy * u1[b].x;
+          a1.y += gy * u1[b].y;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < S; b++) {
+        const double2 u0 = gld(c0 + (size_t)b * g.Lx), u1 = gld(c1 + (size_t)b * g.Lx);
+        const double gy = __shfl(gyl, b, G);
+        a0.x += gy * u0.x;
+        a0.y += gy * u0.y;
+        a1.x += gy * u1.x;
+        a1.y += gy * u1.y;
+      }
     }
     a0.x *= gx;
     a0.y *= gx;
@@ -431,9 +483,8 @@ __device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisPa
     a1.y += __shfl_xor(a1.y, off, 64);
   }
   if (s < g.M && lane == 0) {
-    const double e1s = vp.e1[s] * g.inv_MM;
-    F0 = make_double2(e1s * a0.x, -e1s * a0.y);
-    F1 = make_double2(e1s * a1.x, -e1s * a1.y);
+    F0 = make_double2(r.e1s * a0.x, -r.e1s * a0.y);
+    F1 = make_double2(r.e1s * a1.x, -r.e1s * a1.y);
     return true;
   }
   return false;
@@ -441,31 +492,37 @@ __device__ inline bool degrid_core_pair(int task, const GridGeom& g, const VisPa
 
 // Step mode of st_degrid_model for the two steps of a paired round:
 // out_k = mz + W A d_k; v[2k] += |W V' - mz - W A d_k|^2, v[2k+1] += |W A d_k|^2.
+// dv: the task's visibility records in LDS, or nullptr to load them.  The
+// group leader's model load is issued before the tap gather.
 template <int MSPC>
 __device__ inline void st_degrid_model_pair(int task, const GridGeom& g, const VisParams<double>& vp,
-                                            const double2* __restrict__ Gf0, const double2* __restrict__ Gf1,
-                                            const double2* __restrict__ mz_in, double2* __restrict__ out0,
-                                            double2* __restrict__ out1, double v[4]) {
-  int s;
+                                            const DVis* dv, const double2* __restrict__ Gf0,
+                                            const double2* __restrict__ Gf1, const double2* __restrict__ mz_in,
+                                            double2* __restrict__ out0, double2* __restrict__ out1, double v[4]) {
+  constexpr int G = DegridShape<MSPC>::G;
+  const int j = threadIdx.x / G;  // visibility of this thread within the task
+  const int s = task * (MF_BLOCK / G) + j;
+  const bool lead = s < g.M && (threadIdx.x & (G - 1)) == 0;
+  double2 mz = make_double2(0, 0);
+  if (lead) mz = gld(mz_in + s);
+  const DVis r = dv ? dv[j] : dvis_load(vp, g, s);
   double2 F0, F1;
-  if (degrid_core_pair<MSPC>(task, g, vp, Gf0, Gf1, s, F0, F1)) {
-    const double w = vp.w[s];
-    const double2 V = vp.vis[s];
-    const double2 mz = mz_in[s];
-    const double bx = V.x * w - mz.x, by = V.y * w - mz.y;
+  if (degrid_core_pair<MSPC>(s, r, g, Gf0, Gf1, F0, F1)) {
+    const double w = r.w;
+    const double bx = r.vx * w - mz.x, by = r.vy * w - mz.y;
     {
       const double tx = F0.x * w, ty = F0.y * w;
       const double rx = bx - tx, ry = by - ty;
       v[0] += rx * rx + ry * ry;
       v[1] += tx * tx + ty * ty;
-      out0[s] = make_double2(mz.x + tx, mz.y + ty);
+      gst(out0 + s, make_double2(mz.x + tx, mz.y + ty));
     }
     {
       const double tx = F1.x * w, ty = F1.y * w;
       const double rx = bx - tx, ry = by - ty;
       v[2] += rx * rx + ry * ry;
       v[3] += tx * tx + ty * ty;
-      out1[s] = make_double2(mz.x + tx, mz.y + ty);
+      gst(out1 + s, make_double2(mz.x + tx, mz.y + ty));
     }
   }
 }
