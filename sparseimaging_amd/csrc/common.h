@@ -100,6 +100,16 @@ __device__ __forceinline__ T gld(const T* p) {
     return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) int*)p);
   }
 }
+// A pointer field of an argument struct, read as a global-address-space
+// pointer: every access through the result (also inside inlined helpers that
+// take plain pointers) compiles to GLOBAL instructions (see gld).  The field
+// must hold a global-memory address or null.
+template <typename T>
+__device__ __forceinline__ T* GP(T* const& field) {
+  __attribute__((address_space(1))) T* p;
+  __builtin_memcpy(&p, &field, sizeof(p));
+  return (T*)p;
+}
 template <typename T>
 __device__ __forceinline__ void gst(T* p, const T& v) {
   if constexpr (sizeof(T) == 16) {

@@ -157,7 +157,7 @@ __device__ inline bool flat_sync(unsigned long long* bar, int ci, unsigned nblk,
 // takes an acquire fence.  One L2 write-back per XCD instead of one per
 // workgroup, and 8-way instead of nblk-way contention on the global counter.
 __device__ inline bool xcd_sync(const BarCtx& B, unsigned long long epoch) {
-  unsigned long long* bar = B.bar;
+  unsigned long long* bar = GP(B.bar);
   const int xb = kXBase + kXStride * B.xcc;
   __hip_atomic_fetch_add(&bar[xb], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (B.leader) {
@@ -184,7 +184,7 @@ __device__ inline bool grid_sync(const BarCtx& B, unsigned long long& epoch, int
   epoch += 1;
   if (threadIdx.x == 0) {
     const unsigned long long ta = wall_clock64();
-    fail = (B.hier ? xcd_sync(B, epoch) : flat_sync(B.bar, 0, B.nblk, epoch)) ? 0 : 1;
+    fail = (B.hier ? xcd_sync(B, epoch) : flat_sync(GP(B.bar), 0, B.nblk, epoch)) ? 0 : 1;
     if (B.trace && epoch <= (unsigned long long)B.tcap) {
       const unsigned long long tr = wall_clock64();
       unsigned long long* t = B.trace + ((epoch - 1) * (unsigned long long)B.tnrow + (unsigned long long)B.trow) * 2;
@@ -365,7 +365,8 @@ __device__ inline float lpq_f(const float* p, const float* q, int i, int j, int 
 // the sums stay in registers across consecutive chunks of one tile.  The
 // entries of up to kGB chunks are loaded at once, one thread per entry, into
 // LDS records (tap offsets relative to the tile, tap slopes, deposit value
-// E1 w conj(r)); each record is then read by all lanes as a broadcast.  A tile is flushed when it changes: with plain stores
+// E1 w conj(r)); each record is then read by all lanes as a broadcast.  A tile
+// is flushed when it changes: with plain stores
 // when this workgroup processed all of the tile's chunks (Hg is zero there),
 // otherwise with one global atomic per touched cell.  Deposits as st_grid
 // (stages.h): visibility s at taps (l, l') with weight
@@ -431,9 +432,9 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
           const unsigned en = ent[e];
           const int sv = (int)(en & 0x7fffffffu);
           const bool mir = (en >> 31) != 0;
-          const int2 m = vp.mxy[sv];
-          const float2 a = vp.a[sv];
-          const float ew = vp.e1[sv] * vp.w[sv];
+          const int2 m = GP(vp.mxy)[sv];
+          const float2 a = GP(vp.a)[sv];
+          const float ew = GP(vp.e1)[sv] * GP(vp.w)[sv];
           const float2 r = resid(sv);
           const int tr = ch.x / g.ntx, tc = ch.x - tr * g.ntx;
           const int trb = tr * TY, tcb = tc * TX;
@@ -508,7 +509,10 @@ __host__ __device__ inline size_t table_lds_bytes(int Lx, int Ly, int NX, int NY
 
 // The solve of one problem by workgroups lb = 0 .. nb-1 of a cooperative
 // launch (all of the launch for a single solve, a share of it in a batch).
-template <int MSPC>
+// TAB: the twiddle and E4 tables are staged in LDS.  A compile-time choice, so
+// that every table read is an LDS or a GLOBAL access; a pointer that may hold
+// either compiles to FLAT accesses (see gld in common.h).
+template <int MSPC, bool TAB>
 __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const unsigned nb) {
   extern __shared__ double4 smem_d4[];
   __shared__ double red[16 * kPW];
@@ -523,10 +527,10 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   const bool movie = A.ngb > 0;
   const unsigned ngrp = movie ? (unsigned)A.ngb : nb;  // workgroups of the barrier and the reductions
   // this workgroup's column of the slot-major partial table
-  double* prow = A.part + (movie ? A.gb0 + (int)lb : (int)lb);
+  double* prow = GP(A.part) + (movie ? A.gb0 + (int)lb : (int)lb);
   const size_t pstr = ngrp;
 
-  if (!bar_setup(B, A.bar, ngrp, A.hier_barrier)) return;
+  if (!bar_setup(B, GP(A.bar), ngrp, A.hier_barrier)) return;
   if (threadIdx.x == 0) {
     B.trace = A.trace;
     B.ttag = A.ttag;
@@ -538,11 +542,11 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
 
   // LDS: [tables][stage scratch]
   char* lds = reinterpret_cast<char*>(smem_d4);
-  const double2 *twxd = A.twxd, *twyd = A.twyd;
-  const float2 *twxf = A.twxf, *twyf = A.twyf;
-  const double *e4xd = A.e4xd, *e4yd = A.e4yd;
-  const float *e4xf = A.e4xf, *e4yf = A.e4yf;
-  if (A.lds_tables) {
+  const double2 *twxd = GP(A.twxd), *twyd = GP(A.twyd);
+  const float2 *twxf = GP(A.twxf), *twyf = GP(A.twyf);
+  const double *e4xd = GP(A.e4xd), *e4yd = GP(A.e4yd);
+  const float *e4xf = GP(A.e4xf), *e4yf = GP(A.e4yf);
+  if constexpr (TAB) {
     double2* t64x = reinterpret_cast<double2*>(lds);
     double2* t64y = t64x + g.Lx;
     float2* t32x = reinterpret_cast<float2*>(t64y + g.Ly);
@@ -552,20 +556,20 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     float* q32x = reinterpret_cast<float*>(q64y + NY);
     float* q32y = q32x + NX;
     for (int i = threadIdx.x; i < g.Lx; i += blockDim.x) {
-      t64x[i] = A.twxd[i];
-      t32x[i] = A.twxf[i];
+      t64x[i] = GP(A.twxd)[i];
+      t32x[i] = GP(A.twxf)[i];
     }
     for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) {
-      t64y[i] = A.twyd[i];
-      t32y[i] = A.twyf[i];
+      t64y[i] = GP(A.twyd)[i];
+      t32y[i] = GP(A.twyf)[i];
     }
     for (int i = threadIdx.x; i < NX; i += blockDim.x) {
-      q64x[i] = A.e4xd[i];
-      q32x[i] = A.e4xf[i];
+      q64x[i] = GP(A.e4xd)[i];
+      q32x[i] = GP(A.e4xf)[i];
     }
     for (int i = threadIdx.x; i < NY; i += blockDim.x) {
-      q64y[i] = A.e4yd[i];
-      q32y[i] = A.e4yf[i];
+      q64y[i] = GP(A.e4yd)[i];
+      q32y[i] = GP(A.e4yf)[i];
     }
     twxd = t64x;
     twyd = t64y;
@@ -598,17 +602,17 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     for (int t = lb; t < A.n_row; t += nb) {
       auto load = [&](int i, int j) -> double {
         const int f = i * NY + j;
-        const double xc = A.x[f];
-        A.z[f] = xc;
+        const double xc = GP(A.x)[f];
+        GP(A.z)[f] = xc;
         st[0] += fabs(xc);
-        if (A.x_next) {
-          const double e = xc - A.x_next[f];
+        if (GP(A.x_next)) {
+          const double e = xc - GP(A.x_next)[f];
           sdx[0] += A.wt_next * e * e;
         }
         const int c = f % W, r = f / W;
         double dh = 0, dv = 0;
-        if (c < W - 1) dh = xc - A.x[f + 1];
-        if (r < H - 1) dv = xc - A.x[f + W];
+        if (c < W - 1) dh = xc - GP(A.x)[f + 1];
+        if (r < H - 1) dv = xc - GP(A.x)[f + W];
         st[1] += dh * dh + dv * dv;
         if (c < W - 1 && r < H - 1)
           st[2] += sqrt(dh * dh + dv * dv);
@@ -618,7 +622,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
           st[2] += fabs(dv);
         return xc * e4xd[i] * e4yd[j];
       };
-      st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, sm64, load);
+      st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), sm64, load);
     }
     block_sum<4>(st, red);
     block_sum<1>(sdx, red);
@@ -631,11 +635,11 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   }
   if (!grid_sync(B, epoch, 1)) return;
   for (int t = lb; t < A.n_ky; t += nb)
-    st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
+    st_colfft_fwd<double>(t, 1, GP(A.T64), GP(A.Gf64), NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
   if (!grid_sync(B, epoch, 2)) return;
   {
     double v[2] = {0, 0};
-    for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, nullptr, A.vm[0], A.vm[2], v);
+    for (int t = lb; t < A.n_deg; t += nb) st_degrid_model<MSPC>(t, g, A.pd, GP(A.Gf64), nullptr, GP(A.vm[0]), GP(A.vm[2]), v);
     block_sum<2>(v, red);
     if (threadIdx.x == 0) prow[(kPIchi) * pstr] = v[0];
   }
@@ -643,7 +647,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
   {
     const int slot[5] = {kPIchi, kPIl1, kPItsv, kPItv, kPIdx};
     double sum[5];
-    part_reduce<5>(A.part, ngrp, slot, sum, red);
+    part_reduce<5>(GP(A.part), ngrp, slot, sum, red);
     if (threadIdx.x == 0) {
       double c0 = sum[0] / 2 + A.lambda_l1 * sum[1];
       if (tv)
@@ -684,15 +688,15 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const double ma = bs.mom_a, mb = bs.mom_b;
       const bool acc = bs.accept != 0;
       const int ka = bs.kacc;  // the accepted trial's xnew and model
-      const double* xnw = A.xn[ka];
-      const double2* mx = A.vm[bs.ix];
-      const double2* mn = A.vm[ka ? bs.in1 : bs.in];
-      double2* mz = A.vm[bs.iz];
+      const double* xnw = GP(A.xn[ka]);
+      const double2* mx = GP(A.vm[bs.ix]);
+      const double2* mn = GP(A.vm[ka ? bs.in1 : bs.in]);
+      double2* mz = GP(A.vm[bs.iz]);
       if (mom) {
         for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-          const double xf = A.x[f], xn = xnw[f];
-          A.z[f] = ma * xf + mb * xn;
-          if (acc) A.x[f] = xn;
+          const double xf = GP(A.x)[f], xn = xnw[f];
+          GP(A.z)[f] = ma * xf + mb * xn;
+          if (acc) GP(A.x)[f] = xn;
         }
         for (int s = lb * blockDim.x + threadIdx.x; s < g.M; s += nb * blockDim.x) {
           const double2 a = mx[s], b = mn[s];
@@ -700,8 +704,8 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         }
       }
       auto resid = [&](int s) -> float2 {
-        const double w = A.pd.w[s];
-        const double2 V = A.pd.vis[s];
+        const double w = GP(A.pd.w)[s];
+        const double2 V = GP(A.pd.vis)[s];
         double2 m;
         if (mom) {
           const double2 a = mx[s], b = mn[s];
@@ -711,8 +715,9 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         }
         return make_float2((float)(V.x * w - m.x), (float)(V.y * w - m.y));
       };
-      grid_chunks<(MSPC <= 8) ? 1 : 4>(g, A.pf, resid, A.chunks, A.granges[lb], A.granges[lb + 1], A.ent,
-                                         A.Hg, lds);
+      grid_chunks<(MSPC <= 8) ? 1 : 4>(g, A.pf, resid, GP(A.chunks), GP(A.granges)[lb],
+                                              GP(A.granges)[lb + 1], GP(A.ent),
+                                              GP(A.Hg), lds);
       if (!grid_sync(B, epoch, 4)) return;
       if (threadIdx.x == 0 && mom && acc) {  // W A x <- W A xnew: swap roles
         int& rn = ka ? bs.in1 : bs.in;
@@ -722,7 +727,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
       // P2: inverse column pass (clears Hg behind it)
       for (int t = lb; t < A.n_ky; t += nb)
-        st_colfft_inv<float>(t, 1, A.Hg, A.U, NX, NY + 1, g.Lx, NX / 2, 0, A.Px, twxf, sm32);
+        st_colfft_inv<float>(t, 1, GP(A.Hg), GP(A.U), NX, NY + 1, g.Lx, NX / 2, 0, A.Px, twxf, sm32);
       if (!grid_sync(B, epoch, 5)) return;
     }
     const int trial_first = prologue;
@@ -738,43 +743,43 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       float gv = v * sc * e4xf[i] * e4yf[j];
       if (tsv) {
         const int cc = f % W, r = f / W;
-        const double zc = A.z[f];
+        const double zc = GP(A.z)[f];
         double d = 0.0;
-        if (cc < W - 1) d += zc - A.z[f + 1];
-        if (cc > 0) d += zc - A.z[f - 1];
-        if (r < H - 1) d += zc - A.z[f + W];
-        if (r > 0) d += zc - A.z[f - W];
+        if (cc < W - 1) d += zc - GP(A.z)[f + 1];
+        if (cc > 0) d += zc - GP(A.z)[f - 1];
+        if (r < H - 1) d += zc - GP(A.z)[f + W];
+        if (r > 0) d += zc - GP(A.z)[f - W];
         gv = gv - lt * (float)(2.0 * d);
       }
       if (movie) {  // - lambda_t dt_X(z) (matlab/lib/dt_X.m)
-        const double zc = A.z[f];
+        const double zc = GP(A.z)[f];
         double d = 0.0;
-        if (A.z_next) d += A.wt_next * (zc - A.z_next[f]);
-        if (A.z_prev) d += A.wt_prev * (zc - A.z_prev[f]);
+        if (GP(A.z_next)) d += A.wt_next * (zc - GP(A.z_next)[f]);
+        if (GP(A.z_prev)) d += A.wt_prev * (zc - GP(A.z_prev)[f]);
         gv = gv - (float)(A.lambda_t * 2.0 * d);
       }
-      A.gr[f] = gv;
+      GP(A.gr)[f] = gv;
     };
     if (tv) {
       if (trial_first) {
         for (int t = lb; t < A.n_row; t += nb)
-          st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+          st_rowfft_c2r<float>(t, 1, GP(A.U), NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
         if (!grid_sync(B, epoch, 7)) return;
       }
       // FGP iterations (c/mfista_TV_lib.c:189-384), b = z + g/c (nonneg: -l1/c + g/c + z)
       const float bshift = (A.nonneg == 1) ? (float)(-A.lambda_l1 / c) : 0.f;
       const float a3 = 1.f / (8.f * lam);
-      float *ri = A.rA, *si = A.sA, *ro = A.rB, *so = A.sB;
+      float *ri = GP(A.rA), *si = GP(A.sA), *ro = GP(A.rB), *so = GP(A.sB);
       for (int k = 0; k < A.fgp_iter; k++) {
-        const float c1 = (float)A.fgp_c[2 * k], c2 = (float)A.fgp_c[2 * k + 1];
+        const float c1 = (float)GP(A.fgp_c)[2 * k], c2 = (float)GP(A.fgp_c)[2 * k + 1];
         const int first = (k == 0);
         for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
           const int i = f % W, j = f / W;
           auto xat = [&](int ii, int jj) -> float {
             const int ff = jj * W + ii;
-            const float bb = bshift + cinv * A.gr[ff] + (float)A.z[ff];
+            const float bb = bshift + cinv * GP(A.gr)[ff] + (float)GP(A.z)[ff];
             const float L = first ? 0.f : lpq_f(ri, si, ii, jj, W, H);
-            return fgp_proj(bb - lam * L, eta, A.nonneg == 1, A.box && A.box[ff] == 0.0f);
+            return fgp_proj(bb - lam * L, eta, A.nonneg == 1, GP(A.box) && GP(A.box)[ff] == 0.0f);
           };
           const float xc = xat(i, j);
           float rr = 0.f, ss = 0.f;
@@ -798,15 +803,15 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
           }
           if (hp) {
             const int kk = (W - 1) * j + i;
-            const float pold = first ? 0.f : A.pp[kk];
+            const float pold = first ? 0.f : GP(A.pp)[kk];
             ro[kk] = np_ * c1 + c2 * pold;
-            A.pp[kk] = np_;
+            GP(A.pp)[kk] = np_;
           }
           if (hq) {
             const int kk = W * j + i;
-            const float qold = first ? 0.f : A.qq[kk];
+            const float qold = first ? 0.f : GP(A.qq)[kk];
             so[kk] = nq_ * c1 + c2 * qold;
-            A.qq[kk] = nq_;
+            GP(A.qq)[kk] = nq_;
           }
         }
         if (!grid_sync(B, epoch, 6)) return;
@@ -828,15 +833,15 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const double ci0 = 1.0 / c, e0 = A.lambda_l1 / c, ci1 = 1.0 / c1, e1 = A.lambda_l1 / c1;
       double v[4] = {0, 0, 0, 0};
       for (int t = lb; t < A.n_row; t += nb) {
-        if (trial_first) st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+        if (trial_first) st_rowfft_c2r<float>(t, 1, GP(A.U), NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
         auto load2 = [&](int i, int j, double& o0, double& o1) {
           const int f = i * NY + j;
-          const double zf = A.z[f], gf = (double)A.gr[f];
-          const bool boxed = A.box && A.box[f] == 0.0f;
+          const double zf = GP(A.z)[f], gf = (double)GP(A.gr)[f];
+          const bool boxed = GP(A.box) && GP(A.box)[f] == 0.0f;
           const double xa = prox_p<double>(gf * ci0 + zf, e0, A.nonneg, boxed);
           const double xb = prox_p<double>(gf * ci1 + zf, e1, A.nonneg, boxed);
-          A.xn[0][f] = xa;
-          A.xn[1][f] = xb;
+          GP(A.xn[0])[f] = xa;
+          GP(A.xn[1])[f] = xb;
           const double da = xa - zf, db = xb - zf;
           v[0] += da * da;
           v[1] += fabs(xa);
@@ -845,7 +850,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
           o0 = da * e4xd[i] * e4yd[j];
           o1 = db * e4xd[i] * e4yd[j];
         };
-        st_rowfft_fwd_pair<double>(t, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, A.T64b, sm64, load2);
+        st_rowfft_fwd_pair<double>(t, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), GP(A.T64b), sm64, load2);
       }
       block_sum<4>(v, red);
       if (threadIdx.x == 0) {
@@ -861,27 +866,27 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const bool have_dual = tv && A.fgp_iter > 0;
       for (int t = lb; t < A.n_row; t += nb) {
         if (!tv && trial_first)
-          st_rowfft_c2r<float>(t, 1, A.U, NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
+          st_rowfft_c2r<float>(t, 1, GP(A.U), NX, NY, g.Ly, NY / 2, A.Py, twyf, sm32, gstore);
         auto load = [&](int i, int j) -> double {
           const int f = i * NY + j;
-          const double zf = A.z[f];
-          const bool boxed = A.box && A.box[f] == 0.0f;
+          const double zf = GP(A.z)[f];
+          const bool boxed = GP(A.box) && GP(A.box)[f] == 0.0f;
           double xn;
           if (!tv) {
-            xn = prox_p<double>((double)A.gr[f] * cinv_d + zf, eta_d, A.nonneg, boxed);
+            xn = prox_p<double>((double)GP(A.gr)[f] * cinv_d + zf, eta_d, A.nonneg, boxed);
           } else {
             const int ii = f % W, jj = f / W;
-            const double bb = bshift + cinv_d * (double)A.gr[f] + zf;
-            const double L = have_dual ? (double)lpq_f(A.pp, A.qq, ii, jj, W, H) : 0.0;
+            const double bb = bshift + cinv_d * (double)GP(A.gr)[f] + zf;
+            const double L = have_dual ? (double)lpq_f(GP(A.pp), GP(A.qq), ii, jj, W, H) : 0.0;
             xn = fgp_proj_d(bb - lam_d * L, eta_d, A.nonneg == 1, boxed);
           }
-          A.xn[0][f] = xn;
+          GP(A.xn[0])[f] = xn;
           const double d = xn - zf;
           v[0] += d * d;
           v[1] += fabs(xn);
           return d * e4xd[i] * e4yd[j];
         };
-        st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, A.T64, sm64, load);
+        st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), sm64, load);
       }
       block_sum<2>(v, red);
       if (threadIdx.x == 0) {
@@ -893,13 +898,13 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     // T2: FP64 column pass of d; regulariser partials of xnew and d
     if (pr) {
       for (int t = lb; t < A.n_ky; t += nb)
-        st_colfft_fwd_pair<double>(t, A.T64, A.T64b, A.Gf64, A.Gf64b, NX, NY + 1, g.Lx, NX / 2, g.MSP, A.Px,
+        st_colfft_fwd_pair<double>(t, GP(A.T64), GP(A.T64b), GP(A.Gf64), GP(A.Gf64b), NX, NY + 1, g.Lx, NX / 2, g.MSP, A.Px,
                                    twxd, sm64);
       double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-        reg_terms(A.xn[0], A.z, f, W, H, false, r);
-        reg_terms(A.xn[1], A.z, f, W, H, false, r + 2);
-        if (A.x_next) {
+        reg_terms(GP(A.xn[0]), GP(A.z), f, W, H, false, r);
+        reg_terms(GP(A.xn[1]), GP(A.z), f, W, H, false, r + 2);
+        if (GP(A.x_next)) {
 #pragma unroll
           for (int k = 0; k < 2; k++) temporal_terms(A, k, f, r + 4 + 2 * k);
         }
@@ -917,11 +922,11 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
     } else {
       for (int t = lb; t < A.n_ky; t += nb)
-        st_colfft_fwd<double>(t, 1, A.T64, A.Gf64, NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
+        st_colfft_fwd<double>(t, 1, GP(A.T64), GP(A.Gf64), NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
       double r[4] = {0, 0, 0, 0};
       for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-        reg_terms(A.xn[0], A.z, f, W, H, tv, r);
-        if (A.x_next) temporal_terms(A, 0, f, r + 2);
+        reg_terms(GP(A.xn[0]), GP(A.z), f, W, H, tv, r);
+        if (GP(A.x_next)) temporal_terms(A, 0, f, r + 2);
       }
       block_sum<4>(r, red);
       if (threadIdx.x == 0) {
@@ -934,12 +939,12 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     if (!grid_sync(B, epoch, 9)) return;
     // T3: FP64 degrid of d: W A xnew = W A z + W A d, chi^2(xnew), |W A d|^2
     {
-      const double2* mz = A.vm[bs.iz];
+      const double2* mz = GP(A.vm[bs.iz]);
       if (pr) {
         double v[4] = {0, 0, 0, 0};
         for (int t = lb, k = 0; t < A.n_deg; t += nb, k++)
-          st_degrid_model_pair<MSPC>(t, g, A.pd, k < kTC ? dvc + k * kVPT : nullptr, A.Gf64, A.Gf64b, mz,
-                                     A.vm[bs.in], A.vm[bs.in1], v);
+          st_degrid_model_pair<MSPC>(t, g, A.pd, k < kTC ? dvc + k * kVPT : nullptr, GP(A.Gf64), GP(A.Gf64b), mz,
+                                     GP(A.vm[bs.in]), GP(A.vm[bs.in1]), v);
         block_sum<4>(v, red);
         if (threadIdx.x == 0) {
           prow[(kPChi) * pstr] = v[0];
@@ -950,7 +955,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       } else {
         double v[2] = {0, 0};
         for (int t = lb; t < A.n_deg; t += nb)
-          st_degrid_model<MSPC>(t, g, A.pd, A.Gf64, mz, A.vm[bs.in], nullptr, v);
+          st_degrid_model<MSPC>(t, g, A.pd, GP(A.Gf64), mz, GP(A.vm[bs.in]), nullptr, v);
         block_sum<2>(v, red);
         if (threadIdx.x == 0) {
           prow[(kPChi) * pstr] = v[0];
@@ -969,11 +974,11 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const int sl[2 * kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd,
                                    kPTrial + kPChi, kPTrial + kPLip, kPTrial + kPD2, kPTrial + kPL1,
                                    kPTrial + kPReg, kPTrial + kPTsvd, kPTrial + kPTx, kPTrial + kPTd};
-      part_reduce<2 * kPTrial>(A.part, ngrp, sl, sum, red);
+      part_reduce<2 * kPTrial>(GP(A.part), ngrp, sl, sum, red);
     } else {
       const int sl[kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd};
       double s1[kPTrial];
-      part_reduce<kPTrial>(A.part, ngrp, sl, s1, red);
+      part_reduce<kPTrial>(GP(A.part), ngrp, sl, s1, red);
 #pragma unroll
       for (int q = 0; q < kPTrial; q++) sum[q] = s1[q];
     }
@@ -981,7 +986,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const int it = bs.iter;
       if (prologue) {
         ring[it % (kTD + 1)] = bs.costtmp;
-        if (lb == 0) A.cost[it] = bs.costtmp;
+        if (lb == 0) GP(A.cost)[it] = bs.costtmp;
         bs.prologue = 0;
       }
       bs.rounds += 1;
@@ -1070,8 +1075,8 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
 
   // the accepted image of the last iteration (the reference's x = xnew)
   if (bs.mom_pending && bs.accept) {
-    const double* xnw = A.xn[bs.kacc];
-    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) A.x[f] = xnw[f];
+    const double* xnw = GP(A.xn[bs.kacc]);
+    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) GP(A.x)[f] = xnw[f];
   }
   if (lb == 0 && threadIdx.x == 0) {
     DevState* S = A.S;
@@ -1097,20 +1102,20 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
 template <int MSPC>
 constexpr int kMinWaves = (MSPC > 8) ? 2 : 1;
 
-template <int MSPC>
+template <int MSPC, bool TAB>
 __global__ void __launch_bounds__(MF_BLOCK, kMinWaves<MSPC>) k_mfista_persistent(const PArgs* __restrict__ A) {
-  mfista_body<MSPC>(*A, blockIdx.x, gridDim.x);
+  mfista_body<MSPC, TAB>(*A, blockIdx.x, gridDim.x);
 }
 
 // A batch of independent solves in one cooperative launch: blk[b] =
 // (problem, workgroup index within the problem) for workgroup b.
-template <int MSPC>
+template <int MSPC, bool TAB>
 __global__ void __launch_bounds__(MF_BLOCK, kMinWaves<MSPC>) k_mfista_batch(const PArgs* __restrict__ args,
-                                                           const int2* __restrict__ blk) {
+                                                                             const int2* __restrict__ blk) {
   const int2 pb = blk[blockIdx.x];
   if (pb.x < 0) return;  // a workgroup slot no problem was dealt
   const PArgs& A = args[pb.x];
-  mfista_body<MSPC>(A, pb.y, (unsigned)A.nblk);
+  mfista_body<MSPC, TAB>(A, pb.y, (unsigned)A.nblk);
 }
 
 // ---------------------------------------------------------------------------
@@ -1128,14 +1133,11 @@ struct PLaunch {
 static int persistent_config(Context& C, int MSP, const std::vector<const NufftState*>& sets, bool batch,
                              bool allow_pair, PLaunch* L) {
   if (MSP != 6 && MSP != 12) return MFISTA_E_ARG;
-  if (batch)
-    L->fn = (MSP == 6) ? (const void*)k_mfista_batch<6> : (const void*)k_mfista_batch<12>;
-  else
-    L->fn = (MSP == 6) ? (const void*)k_mfista_persistent<6> : (const void*)k_mfista_persistent<12>;
   // dynamic LDS: [twiddle + E4 tables, when they fit][degrid visibility
   // records][stage scratch: FP64 row/column FFT (two sequences per transform
-  // in a paired round) or the batched gridder's entries].  Paired rounds are used when their larger
-  // scratch still leaves the wanted number of workgroups per CU.
+  // in a paired round) or the batched gridder's entries].  Paired rounds are
+  // used when their larger scratch still leaves the wanted number of
+  // workgroups per CU.
   int dev = 0, ncu = 0, lds_max = 0;
   MF_CHECK(hipGetDevice(&dev));
   MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1157,6 +1159,15 @@ static int persistent_config(Context& C, int MSP, const std::vector<const NufftS
     L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
     if (L->lds_tables) sh += tables;
     L->sh = sh;
+    const bool tab = L->lds_tables != 0;
+    if (batch)
+      L->fn = (MSP == 6) ? (tab ? (const void*)k_mfista_batch<6, true> : (const void*)k_mfista_batch<6, false>)
+                         : (tab ? (const void*)k_mfista_batch<12, true> : (const void*)k_mfista_batch<12, false>);
+    else
+      L->fn = (MSP == 6) ? (tab ? (const void*)k_mfista_persistent<6, true>
+                                : (const void*)k_mfista_persistent<6, false>)
+                         : (tab ? (const void*)k_mfista_persistent<12, true>
+                                : (const void*)k_mfista_persistent<12, false>);
     int per_cu = 0;
     MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L->fn, MF_BLOCK, sh));
     if (per_cu < 1 && !pair) return MFISTA_E_HIP;

@@ -288,11 +288,11 @@ __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<
   // read it by shuffle (the weights depend on the row, not on the lane's
   // column).  Inside the branch every source lane b < S of an active lane's
   // group is active too (the group shares one visibility).
-  const C a = (s < g.M) ? vp.a[s] : mk<R>(0, 0);
+  const C a = (s < g.M) ? GP(vp.a)[s] : mk<R>(0, 0);
   const R l = (R)(lane - MSPC + 1);
   const R gyl = dev_exp(l * a.y - l * l * t3y);
   if (s < g.M && lane < S) {
-    const int2 m = vp.mxy[s];
+    const int2 m = GP(vp.mxy)[s];
     const R gx = dev_exp(l * a.x - l * l * t3x);
     const C* col = Gf + (size_t)m.y * g.Lx + (m.x + g.NX + lane - MSPC + 1);
     C gv[S];
@@ -313,7 +313,7 @@ __device__ inline bool degrid_core(int task, const GridGeom& g, const VisParams<
     acc.y += __shfl_xor(acc.y, off, 64);
   }
   if (s < g.M && lane == 0) {
-    const R e1s = vp.e1[s] * (R)g.inv_MM;
+    const R e1s = GP(vp.e1)[s] * (R)g.inv_MM;
     F = mk<R>(e1s * acc.x, -e1s * acc.y);
     return true;
   }
@@ -339,9 +339,9 @@ __device__ inline void st_degrid(int task, const GridGeom& g, const VisParams<R>
   v[0] = 0.0;
   v[1] = 0.0;
   if (lead) {
-    const R w = vp.w[s];
+    const R w = GP(vp.w)[s];
     if (mode == 0) {
-      const C V = vp.vis[s];
+      const C V = GP(vp.vis)[s];
       const C r = mk<R>((V.x - F.x) * w, (V.y - F.y) * w);
       v[0] = (double)r.x * (double)r.x + (double)r.y * (double)r.y;
       if (rz) rz[s] = r;
@@ -371,8 +371,8 @@ __device__ inline void st_degrid_model(int task, const GridGeom& g, const VisPar
   double2 F;
   const bool lead = degrid_core<double, MSPC>(task, g, vp, Gf, s, F);
   if (lead) {
-    const double w = vp.w[s];
-    const double2 V = vp.vis[s];
+    const double w = GP(vp.w)[s];
+    const double2 V = GP(vp.vis)[s];
     const double tx = F.x * w, ty = F.y * w;
     if (!mz_in) {
       const double rx = V.x * w - tx, ry = V.y * w - ty;
@@ -401,12 +401,12 @@ struct alignas(16) DVis {
 __device__ inline DVis dvis_load(const VisParams<double>& vp, const GridGeom& g, int s) {
   DVis r{};
   if (s < g.M) {
-    const double2 a = gld(vp.a + s), V = gld(vp.vis + s);
-    const int2 m = gld(vp.mxy + s);
+    const double2 a = gld(GP(vp.a) + s), V = gld(GP(vp.vis) + s);
+    const int2 m = gld(GP(vp.mxy) + s);
     r.ax = a.x;
     r.ay = a.y;
-    r.e1s = gld(vp.e1 + s) * g.inv_MM;
-    r.w = gld(vp.w + s);
+    r.e1s = gld(GP(vp.e1) + s) * g.inv_MM;
+    r.w = gld(GP(vp.w) + s);
     r.vx = V.x;
     r.vy = V.y;
     r.mx = m.x;
@@ -555,9 +555,9 @@ __device__ inline void st_grid(int chunk, const GridGeom& g, const VisParams<R>&
     const unsigned en = ent[e];
     const int s = (int)(en & 0x7fffffffu);
     const bool mir = (en >> 31) != 0;
-    const int2 m = vp.mxy[s];
-    const C a = vp.a[s];
-    const R ew = vp.e1[s] * vp.w[s];
+    const int2 m = GP(vp.mxy)[s];
+    const C a = GP(vp.a)[s];
+    const R ew = GP(vp.e1)[s] * GP(vp.w)[s];
     const C r = resid(s);
     C d = mk<R>(ew * r.x, -ew * r.y);
     int lp0, lp1, l0, l1;
