@@ -498,8 +498,8 @@ __host__ __device__ inline size_t grid_lds_bytes(int TX, int TY) {
 }
 
 // Visibility records of the paired degrid tasks cached in LDS per workgroup
-// (64 bytes each): all tasks of a workgroup while n_deg <= 4 nb (12 taps) or
-// 8 nb (24 taps); later tasks load their records.
+// (64 bytes each): the first 3 tasks of 20 visibilities (12 taps) or 8 tasks
+// of 8 (24 taps) of each workgroup; later tasks load their records.
 constexpr int kDVisCap = 64;
 
 // LDS bytes of the staged twiddle (FP32 and FP64, both axes) and E4 tables
@@ -583,13 +583,14 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     __syncthreads();
   }
   // the visibility records of this workgroup's degrid tasks
-  constexpr int kVPT = MF_BLOCK / DegridShape<MSPC>::G;  // visibilities per task
-  constexpr int kTC = kDVisCap / kVPT;                    // tasks cached
+  constexpr int kVPT = PairShape<MSPC>::VPT;  // visibilities per paired degrid task
+  constexpr int kTC = kDVisCap / kVPT;         // tasks cached
+  const int n_degp = (g.M + kVPT - 1) / kVPT;  // paired degrid tasks
   DVis* dvc = reinterpret_cast<DVis*>(lds);
   lds += kDVisCap * sizeof(DVis);
-  for (int q = threadIdx.x; q < kDVisCap; q += blockDim.x) {
+  for (int q = threadIdx.x; q < kTC * kVPT; q += blockDim.x) {
     const int k = q / kVPT, t = lb + k * (int)nb;
-    if (t < A.n_deg) dvc[q] = dvis_load(A.pd, g, t * kVPT + q % kVPT);
+    if (t < n_degp) dvc[q] = dvis_load(A.pd, g, t * kVPT + q % kVPT);
   }
   __syncthreads();
   float2* sm32 = reinterpret_cast<float2*>(lds);
@@ -942,7 +943,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       const double2* mz = GP(A.vm[bs.iz]);
       if (pr) {
         double v[4] = {0, 0, 0, 0};
-        for (int t = lb, k = 0; t < A.n_deg; t += nb, k++)
+        for (int t = lb, k = 0; t < n_degp; t += nb, k++)
           st_degrid_model_pair<MSPC>(t, g, A.pd, k < kTC ? dvc + k * kVPT : nullptr, GP(A.Gf64), GP(A.Gf64b), mz,
                                      GP(A.vm[bs.in]), GP(A.vm[bs.in1]), v);
         block_sum<4>(v, red);

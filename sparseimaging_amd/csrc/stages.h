@@ -415,29 +415,43 @@ __device__ inline DVis dvis_load(const VisParams<double>& vp, const GridGeom& g,
   return r;
 }
 
-// Tap gather of two spectra with the same taps (degrid_core on Gf0 and Gf1)
-// for visibility s with constants r.  The tap rows are loaded in groups of up
-// to 12 (both spectra, 4 VGPRs per row and spectrum).  For the 24-tap kernel
-// a scheduling barrier keeps each group's loads ahead of its FMAs: the gather
-// is latency-bound, and left to itself the scheduler interleaved loads and
-// FMAs, which cost that stage ~30 %.  The 12-tap kernel stays unpinned: there
-// the barrier's extra live registers cost the kernel its second workgroup per
-// CU (measured on the MI355X).
+// Lane layout of the paired degrid: a group of GL lanes per visibility (lane
+// li < 2 MSP owns tap column li), VPW groups per 64-wide wave, VPT
+// visibilities per 256-thread task.  12 taps use groups of 12 lanes, five to a
+// wave (60 of 64 lanes busy): a task then covers 20 visibilities instead of 16,
+// and no workgroup runs more than two of these latency-bound tasks at 20000
+// visibilities on 512 workgroups.
 template <int MSPC>
-__device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
+struct PairShape {
+  static constexpr int GL = (2 * MSPC <= 12) ? 12 : 32;
+  static constexpr int VPW = 64 / GL;
+  static constexpr int VPT = VPW * (MF_BLOCK / 64);
+};
+
+// Tap gather of two spectra with the same taps (degrid_core on Gf0 and Gf1)
+// for visibility s with constants r; lane li of group gi of its wave.  The
+// tap rows are loaded in groups of up to 12 (both spectra, 4 VGPRs per row
+// and spectrum).  For the 24-tap kernel a scheduling barrier keeps each
+// group's loads ahead of its FMAs: the gather is latency-bound, and left to
+// itself the scheduler interleaved loads and FMAs, which cost that stage
+// ~30 %.  The 12-tap kernel stays unpinned: there the barrier's extra live
+// registers cost the kernel its second workgroup per CU (measured on the
+// MI355X).
+template <int MSPC>
+__device__ inline bool degrid_core_pair(int s, bool valid, int gi, int li, const DVis& r, const GridGeom& g,
                                         const double2* __restrict__ Gf0, const double2* __restrict__ Gf1,
                                         double2& F0, double2& F1) {
-  constexpr int S = DegridShape<MSPC>::S, G = DegridShape<MSPC>::G;
+  constexpr int S = 2 * MSPC, GL = PairShape<MSPC>::GL;
   constexpr int RG = (S <= 12) ? S : 12;  // rows per load group
   static_assert(S % RG == 0, "tap rows split into whole groups");
-  const int lane = threadIdx.x & (G - 1);
+  const int base = gi * GL;  // wave lane of the group's lane 0
   const double t3x = g.t3x, t3y = g.t3y;
   double2 a0 = make_double2(0, 0), a1 = make_double2(0, 0);
-  const double l = (double)(lane - MSPC + 1);
-  const double gyl = dev_exp(l * r.ay - l * l * t3y);  // row weight of tap row `lane`, shared by shuffle
-  if (s < g.M && lane < S) {
+  const double l = (double)(li - MSPC + 1);
+  const double gyl = dev_exp(l * r.ay - l * l * t3y);  // row weight of tap row `li`, shared by shuffle
+  if (valid && li < S) {
     const double gx = dev_exp(l * r.ax - l * l * t3x);
-    const size_t off = (size_t)r.my * g.Lx + (r.mx + g.NX + lane - MSPC + 1);
+    const size_t off = (size_t)r.my * g.Lx + (r.mx + g.NX + li - MSPC + 1);
     const double2* c0 = Gf0 + off;
     const double2* c1 = Gf1 + off;
     if constexpr (MSPC > 8) {
@@ -452,7 +466,7 @@ __device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int b = 0; b < RG; b++) {
-          const double gy = __shfl(gyl, b0 + b, G);
+          const double gy = __shfl(gyl, base + b0 + b, 64);
           a0.x += gy * u0[b].x;
           a0.y += gy * u0[b].y;
           a1.x += gy * u1[b].x;
@@ -463,7 +477,7 @@ __device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
 #pragma unroll
       for (int b = 0; b < S; b++) {
         const double2 u0 = gld(c0 + (size_t)b * g.Lx), u1 = gld(c1 + (size_t)b * g.Lx);
-        const double gy = __shfl(gyl, b, G);
+        const double gy = __shfl(gyl, base + b, 64);
         a0.x += gy * u0.x;
         a0.y += gy * u0.y;
         a1.x += gy * u1.x;
@@ -475,14 +489,33 @@ __device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
     a1.x *= gx;
     a1.y *= gx;
   }
+  if constexpr ((GL & (GL - 1)) == 0) {
 #pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    a0.x += __shfl_xor(a0.x, off, 64);
-    a0.y += __shfl_xor(a0.y, off, 64);
-    a1.x += __shfl_xor(a1.x, off, 64);
-    a1.y += __shfl_xor(a1.y, off, 64);
+    for (int off = GL / 2; off > 0; off >>= 1) {
+      a0.x += __shfl_xor(a0.x, off, 64);
+      a0.y += __shfl_xor(a0.y, off, 64);
+      a1.x += __shfl_xor(a1.x, off, 64);
+      a1.y += __shfl_xor(a1.y, off, 64);
+    }
+  } else {
+    // shuffle-down tree over the GL lanes of the group: lane 0 ends with the
+    // sum (a lane adds its partner li + off only while that is in the group)
+    const int wl = base + li;
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) {
+      const bool in = li + off < GL;
+      const double p0 = __shfl(a0.x, wl + off, 64), p1 = __shfl(a0.y, wl + off, 64);
+      const double p2 = __shfl(a1.x, wl + off, 64), p3 = __shfl(a1.y, wl + off, 64);
+      if (in) {
+        a0.x += p0;
+        a0.y += p1;
+        a1.x += p2;
+        a1.y += p3;
+      }
+    }
+    static_assert(GL <= 16, "tree covers up to 16 lanes");
   }
-  if (s < g.M && lane == 0) {
+  if (valid && li == 0) {
     F0 = make_double2(r.e1s * a0.x, -r.e1s * a0.y);
     F1 = make_double2(r.e1s * a1.x, -r.e1s * a1.y);
     return true;
@@ -492,6 +525,7 @@ __device__ inline bool degrid_core_pair(int s, const DVis& r, const GridGeom& g,
 
 // Step mode of st_degrid_model for the two steps of a paired round:
 // out_k = mz + W A d_k; v[2k] += |W V' - mz - W A d_k|^2, v[2k+1] += |W A d_k|^2.
+// Task `task` covers visibilities task VPT .. task VPT + VPT - 1 (PairShape).
 // dv: the task's visibility records in LDS, or nullptr to load them.  The
 // group leader's model load is issued before the tap gather.
 template <int MSPC>
@@ -499,15 +533,19 @@ __device__ inline void st_degrid_model_pair(int task, const GridGeom& g, const V
                                             const DVis* dv, const double2* __restrict__ Gf0,
                                             const double2* __restrict__ Gf1, const double2* __restrict__ mz_in,
                                             double2* __restrict__ out0, double2* __restrict__ out1, double v[4]) {
-  constexpr int G = DegridShape<MSPC>::G;
-  const int j = threadIdx.x / G;  // visibility of this thread within the task
-  const int s = task * (MF_BLOCK / G) + j;
-  const bool lead = s < g.M && (threadIdx.x & (G - 1)) == 0;
+  using PS = PairShape<MSPC>;
+  const int wl = threadIdx.x & 63;
+  const int gi = wl / PS::GL, li = wl - gi * PS::GL;
+  const int j = (threadIdx.x >> 6) * PS::VPW + gi;  // visibility of this lane within the task
+  const int s = task * PS::VPT + j;
+  const bool valid = gi < PS::VPW && s < g.M;
+  const bool lead = valid && li == 0;
   double2 mz = make_double2(0, 0);
   if (lead) mz = gld(mz_in + s);
-  const DVis r = dv ? dv[j] : dvis_load(vp, g, s);
+  DVis r{};
+  if (gi < PS::VPW) r = dv ? dv[j] : dvis_load(vp, g, s);
   double2 F0, F1;
-  if (degrid_core_pair<MSPC>(s, r, g, Gf0, Gf1, F0, F1)) {
+  if (degrid_core_pair<MSPC>(s, valid, gi, li, r, g, Gf0, Gf1, F0, F1)) {
     const double w = r.w;
     const double bx = r.vx * w - mz.x, by = r.vy * w - mz.y;
     {
