@@ -32,9 +32,10 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM traffic of one launch of the bench workload, measured with rocprofv3
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a 256 MiB copy and
-# fill; profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md), for a
-# solve of 579 iterations and 1166 trials.
-MEASURED_TRAFFIC = {"bytes": 16854340953, "iterations": 579, "trials": 1166}
+# fill; profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md): the two
+# counters come from two runs, solves of 578 and 580 iterations in 756 and 766
+# paired trial rounds.
+MEASURED_TRAFFIC = {"bytes": 20484857931, "iterations": 579, "trial_rounds": 761}
 
 
 def log(*a):
@@ -90,8 +91,9 @@ def persistent_bytes(N, M, MSP, n_entries, iters, trials, rounds=None):
       T1                   read g, z once (12 B/px), write two xnew (16 B/px), two T64 (32 B/cell)
       T2                   read two T64 (32 B/cell), write two Gf64 (32 B/cell); read two
                            xnew and z (24 B/px)
-      T3                   read two Gf64 (32 B/cell), parameters and W A z once (72 B/vis),
-                           write two W A xnew (32 B/vis)
+      T3                   read two Gf64 (32 B/cell), W A z once (16 B/vis), write two
+                           W A xnew (32 B/vis); the visibility parameters (56 B/vis) are
+                           cached in LDS once per launch
     Returns (total, per iteration, per trial round).
     """
     P = N * N
@@ -102,8 +104,8 @@ def persistent_bytes(N, M, MSP, n_entries, iters, trials, rounds=None):
     per_trial = (20 * P + 16 * T) + (16 * T + 16 * Hf + 16 * P) + (16 * Hf + 88 * M)
     if rounds is None or rounds >= trials:
         return per_iter * iters + per_trial * trials, per_iter, per_trial
-    per_pair = (28 * P + 32 * T) + (32 * T + 32 * Hf + 24 * P) + (32 * Hf + 104 * M)
-    return per_iter * iters + per_pair * rounds, per_iter, per_pair
+    per_pair = (28 * P + 32 * T) + (32 * T + 32 * Hf + 24 * P) + (32 * Hf + 48 * M)
+    return 56 * M + per_iter * iters + per_pair * rounds, per_iter, per_pair
 
 
 def _batch_vs_single(sa, probs, preset, rel_tol):
