@@ -81,6 +81,22 @@ void mfista_imaging_core_dft(double *y, double *A, int *M, int *N, int NX, int N
                              int looe_flag, int box_flag, float *cl_box,
                              struct RESULT *mfista_result);
 
+/* replaces compute_LOOE_L1, c/looe_lib.c:195-216: the leave-one-out error
+ * estimate of an L1 DFT-mode solution xvec of yvec ~ Amat x (Amat M x N,
+ * column-major; yAx = yvec - Amat xvec).  Computed on the GPU in FP64.  Returns
+ * the LOOE and sets *looe_m (= LOOE) and *looe_std, or returns -1 with both
+ * set to 0 when the active-set Hessian is not positive definite (as the
+ * reference does when dposv fails).  lambda1 and yvec are unused there too. */
+double compute_LOOE_L1(int *M, int *N, double lambda1, double *yvec, double *Amat, double *xvec,
+                       double *yAx, double *looe_m, double *looe_std);
+
+/* replaces compute_LOOE_L1_TSV, c/looe_lib.c:276-309: as compute_LOOE_L1 with
+ * lambda_tsv d2_TSV (4, 6 or 8 per pixel, c/looe_lib.c:218-235) added to the
+ * Hessian's diagonal.  lambda_l1 and yvec are unused there too. */
+double compute_LOOE_L1_TSV(int *M, int *N, int NX, int NY, double lambda_l1, double lambda_tsv,
+                           double *yvec, double *Amat, double *xvec, double *yAx, double *looe_m,
+                           double *looe_std);
+
 /* replaces x2y_nufft, c++/mfista.hpp:136-137 (c++/mfista_nufft_lib.cpp:1076):
  * model visibilities y = A x with the C++ 12-tap preset, FP64. */
 void x2y_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *xin, double *y_r,
@@ -222,6 +238,14 @@ int mfista_imaging_core_dft_ex(const double *y, const double *A, int M, int N, i
                                double lambda_tsv, double cinit, const double *xinit, double *xout,
                                int nonneg_flag, int box_flag, const float *cl_box,
                                struct RESULT *mfista_result, mfista_opts *opts);
+
+/* the LOOE of compute_LOOE_L1_TSV (lambda_tsv = 0: compute_LOOE_L1) with a
+ * status return: *hessian_positive = 1 and *looe_m, *looe_std set, or 0 (both
+ * 0) when the Hessian is not positive definite; *n_active (may be NULL)
+ * receives the size of the active set.  M >= 2. */
+int mfista_looe_dft(int M, int N, int NX, int NY, double lambda_tsv, const double *Amat,
+                    const double *xvec, const double *yAx, double *looe_m, double *looe_std,
+                    int *hessian_positive, int *n_active);
 
 /* standalone NUFFT operators with a chosen preset and precision (0 = FP32,
  * 1 = FP64); host pointers; return a status */

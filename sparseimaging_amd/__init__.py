@@ -41,7 +41,7 @@ EXPORTED = [
     "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
     "mfista_amd_version", "mfista_profile_stages", "mfista_last_steps",
     "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
-    "mfista_movie_nufft",
+    "mfista_movie_nufft", "mfista_looe_dft", "compute_LOOE_L1", "compute_LOOE_L1_TSV",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
@@ -150,6 +150,12 @@ def _declare(L):
     L.mfista_imaging_core_dft_ex.restype = i
     L.mfista_imaging_core_dft_ex.argtypes = [_dp, _dp, i, i, i, i, i, d, d, d, d, d, _dp, _dp, i, i, _fp,
                                              C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_looe_dft.restype = i
+    L.mfista_looe_dft.argtypes = [i, i, i, i, d, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
+    L.compute_LOOE_L1.restype = d
+    L.compute_LOOE_L1.argtypes = [_ip, _ip, d, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.compute_LOOE_L1_TSV.restype = d
+    L.compute_LOOE_L1_TSV.argtypes = [_ip, _ip, i, i, d, d, _dp, _dp, _dp, _dp, _dp, _dp]
     L.mfista_profile_stages.restype = i
     L.mfista_profile_stages.argtypes = [i, _dp, i]
     L.mfista_last_steps.restype = C.c_long
@@ -369,6 +375,28 @@ def mfista_imaging_core_dft(y, A, NX, NY, maxiter=MAXITER, eps=EPS, lambda_l1=1.
     if res.residual:
         C.CDLL(None).free(C.cast(res.residual, C.c_void_p))
     return xout.reshape(NX, NY), res.as_dict(), _info(o), resid
+
+
+def compute_looe(A, x, yAx, NX=1, NY=1, lambda_tsv=0.0):
+    """Leave-one-out error estimate of a DFT-mode solution on the GPU, FP64:
+    compute_LOOE_L1 (lambda_tsv = 0, c/looe_lib.c:195) or compute_LOOE_L1_TSV
+    (c/looe_lib.c:276).  A: M x N; x: the N solution values in the order of A's
+    columns; yAx = y - A x.  Returns a dict: looe (-1 if the active-set Hessian
+    is not positive definite), looe_m, looe_std, hessian_positive, n_active."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    M, N = A.shape
+    xv, px = _d(np.asarray(x, dtype=np.float64).ravel())
+    r, pr = _d(np.asarray(yAx, dtype=np.float64).ravel())
+    if xv.size != N or r.size != M:
+        raise ValueError("x must have N values and yAx M values")
+    lm, ls = C.c_double(0.0), C.c_double(0.0)
+    pd, na = C.c_int(0), C.c_int(0)
+    rc = lib().mfista_looe_dft(M, N, NX, NY, lambda_tsv, A.ctypes.data_as(_dp), px, pr, C.byref(lm),
+                               C.byref(ls), C.byref(pd), C.byref(na))
+    if rc != 0:
+        raise MfistaError(rc)
+    return {"looe": lm.value if pd.value else -1.0, "looe_m": lm.value, "looe_std": ls.value,
+            "hessian_positive": pd.value, "n_active": na.value}
 
 
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,

@@ -18,7 +18,7 @@ LEARN_HEADER = os.path.join(ROOT, "include", "mfista_learn.h")
 def _declared_functions(header=HEADER):
     src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char|long)\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", src, flags=re.M)
+    names = re.findall(r"^\s*(?:const\s+)?(?:void|int|char|long|double)\s*\*?\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", src, flags=re.M)
     return sorted(set(names))
 
 
