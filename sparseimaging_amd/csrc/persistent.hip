@@ -971,11 +971,25 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     // backtracking loop (c/mfista_nufft_lib.c:692-723) takes them: the first
     // that passes is accepted; one that fails multiplies c by eta.
     double sum[2 * kPTrial];
-    if (pr) {
+    if (pr && (movie || MSPC > 8)) {
       const int sl[2 * kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd,
                                    kPTrial + kPChi, kPTrial + kPLip, kPTrial + kPD2, kPTrial + kPL1,
                                    kPTrial + kPReg, kPTrial + kPTsvd, kPTrial + kPTx, kPTrial + kPTd};
       part_reduce<2 * kPTrial>(GP(A.part), ngrp, sl, sum, red);
+    } else if (pr && MSPC <= 8) {  // outside a movie the temporal slots (kPTx, kPTd) are zero: not read
+      // (12 taps only: the 24-tap kernel spills more with a second reduction shape)
+      const int sl[12] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd,
+                          kPTrial + kPChi, kPTrial + kPLip, kPTrial + kPD2, kPTrial + kPL1,
+                          kPTrial + kPReg, kPTrial + kPTsvd};
+      double s1[12];
+      part_reduce<12>(GP(A.part), ngrp, sl, s1, red);
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) sum[k * kPTrial + q] = s1[k * 6 + q];
+        sum[k * kPTrial + kPTx] = 0.0;
+        sum[k * kPTrial + kPTd] = 0.0;
+      }
     } else {
       const int sl[kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd};
       double s1[kPTrial];
