@@ -896,13 +896,21 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       }
     }
     if (!grid_sync(B, epoch, 8)) return;
-    // T2: FP64 column pass of d; regulariser partials of xnew and d
+    // T2: FP64 column pass of d; regulariser partials of xnew and d.  When at
+    // least half as many workgroups as pixels / 256 have no column task, those
+    // workgroups take the partials and the column-task workgroups skip them:
+    // one dependent round trip fewer on the stage's critical path.
+    const int nreg = (int)nb - A.n_ky;
+    const bool split = nreg > 0 && 2 * (long)nreg * blockDim.x >= n;
+    const bool do_reg = !split || (int)lb >= A.n_ky;
+    const int rg0 = split ? (int)lb - A.n_ky : (int)lb;
+    const int rgn = split ? nreg : (int)nb;
     if (pr) {
       for (int t = lb; t < A.n_ky; t += nb)
         st_colfft_fwd_pair<double>(t, GP(A.T64), GP(A.T64b), GP(A.Gf64), GP(A.Gf64b), NX, NY + 1, g.Lx, NX / 2, g.MSP, A.Px,
                                    twxd, sm64);
       double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+      for (int f = rg0 * blockDim.x + threadIdx.x; do_reg && f < n; f += rgn * blockDim.x) {
         reg_terms(GP(A.xn[0]), GP(A.z), f, W, H, false, r);
         reg_terms(GP(A.xn[1]), GP(A.z), f, W, H, false, r + 2);
         if (GP(A.x_next)) {
@@ -925,7 +933,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       for (int t = lb; t < A.n_ky; t += nb)
         st_colfft_fwd<double>(t, 1, GP(A.T64), GP(A.Gf64), NX, NY + 1, g.Lx, NX / 2, 0, g.MSP, A.Px, twxd, sm64);
       double r[4] = {0, 0, 0, 0};
-      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+      for (int f = rg0 * blockDim.x + threadIdx.x; do_reg && f < n; f += rgn * blockDim.x) {
         reg_terms(GP(A.xn[0]), GP(A.z), f, W, H, tv, r);
         if (GP(A.x_next)) temporal_terms(A, 0, f, r + 2);
       }
