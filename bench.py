@@ -225,8 +225,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
-              "(independent solves shard trivially across GPUs, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; multi-GPU runs are not supported "
+              "(independent solves shard across GPUs without communication)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

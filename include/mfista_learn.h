@@ -3,7 +3,7 @@
  *
  * No counterpart in the reference: the reference has no learned component;
  * these serve the warm start of mfista_imaging_core_nufft (cold-start MFISTA
- * stays the fallback and the judge).  Every array is a device pointer; tensors
+ * stays the fallback and decides which start is kept).  Every array is a device pointer; tensors
  * are NHWC FP32, 3x3 weights are [ky][kx][ci][co] in BF16 (uint16_t), the FP32
  * master copy lives with the optimiser.  `stream` is a hipStream_t (NULL = the
  * default stream).  Functions return 0 or a negative status. */

@@ -1,9 +1,10 @@
 """Learned warm start for NUFFT-mode MFISTA.
 
-A U-Net maps the dirty image Re(A^H W^2 V) and a log-lambda plane to a
-starting image; MFISTA then refines it to the usual tolerance.  Cold-start
-MFISTA stays the fallback and the judge: with verify=True a warm start whose
-converged objective is higher than the cold start's is rejected.  The
+A U-Net maps the dirty image Re(A^H W^2 V), a log-lambda plane and (third
+channel) the dirty beam to a starting image; MFISTA then refines it to the
+usual tolerance.  Cold-start MFISTA stays the fallback and decides which start
+is kept: with verify=True a warm start whose converged objective is higher
+than the cold start's is rejected.  The
 reference has no learned component; this module serves its
 mfista_imaging_core_nufft (c/mfista_nufft_lib.c:909) calls.
 
