@@ -149,6 +149,22 @@ def bench_lambda_grid(sa, synth):
     return out
 
 
+def bench_lambda_grid_256(sa, synth):
+    """Batched workload (a) at the headline size: 8 values of lambda_tsv,
+    log-spaced in [10^1.5, 10^2.5], over the 256^2 bench visibility set
+    (lambda_l1 = 1), 12-tap preset, 1e-6 relative tolerance."""
+    import numpy as np
+
+    P = synth.make_nufft_problem()
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=256, NY=256,
+                maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, lambda_tsv=10.0 ** b) for b in np.linspace(1.5, 2.5, 8)]
+    out = _batch_vs_single(sa, probs, sa.PRESET_CXX12, 1e-6)
+    out["workload"] = "lambda_tsv grid x8, 256^2, M=20000, 12 taps"
+    log("bench: lambda grid 256 " + json.dumps(out))
+    return out
+
+
 def bench_movie(sa, synth):
     """Batched workload (b): 32 frames of a drifting ring, 128^2, M per frame
     uniform in 5000..40000 (seed 1004), lambda_l1 = 1, lambda_tsv = 100,
@@ -220,8 +236,8 @@ def main():
     ap.add_argument("--warm", type=int, default=8,
                     help="held-out 256^2 problems for the learned warm start (0 = skip)")
     ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
-                    help="also time the batched workloads (6x6 lambda grid; 32-frame movie) "
-                         "against the same problems solved one by one")
+                    help="also time the batched workloads (6x6 lambda grid at 128^2, 8 lambda_tsv values "
+                         "at 256^2, 32-frame movie) against the same problems solved one by one")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -306,6 +322,7 @@ def main():
     batch = {}
     if args.batch in ("lambda", "all"):
         batch["lambda_grid"] = bench_lambda_grid(sa, synth)
+        batch["lambda_grid_256"] = bench_lambda_grid_256(sa, synth)
     if args.batch in ("movie", "all"):
         batch["movie"] = bench_movie(sa, synth)
 
