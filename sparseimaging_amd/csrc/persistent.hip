@@ -1235,18 +1235,39 @@ static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P,
     }
     MF_CHECK(hipMemcpyAsync(W.fgpc.p, cc.data(), cc.size() * 8, hipMemcpyHostToDevice, st));
   }
-  {  // contiguous gridding chunk ranges with balanced entry counts
-    std::vector<int> rg(nblk + 1, 0);
+  // Gridding ranges: workgroup b takes entries [tot b / nblk, tot (b+1) / nblk)
+  // of the tile-ordered entry sequence.  The tile chunk list is re-cut at those
+  // boundaries, so every workgroup gathers the same number of entries (to one);
+  // a cut piece keeps the tile's "first" flag only on its first piece and the
+  // "last" flag only on its last, so a tile split between workgroups is
+  // flushed with atomics.
+  std::vector<int4> pc;
+  {
     const std::vector<int4>& ch = N.h_chunks;
+    std::vector<int> rg(nblk + 1, 0);
     long tot = 0;
     for (const int4& c4 : ch) tot += c4.z - c4.y;
-    long acc = 0;
-    int b = 1;
-    for (int c = 0; c < (int)ch.size() && b < nblk; c++) {
-      acc += ch[c].z - ch[c].y;
-      while (b < nblk && acc * nblk >= tot * (long)b) rg[b++] = c + 1;
+    auto bnd = [&](int k) -> long { return tot * k / nblk; };
+    pc.reserve(ch.size() + nblk);
+    long G = 0;  // entries before the current chunk
+    int k = 1;
+    for (const int4& c4 : ch) {
+      const int len = c4.z - c4.y;
+      int off = 0;
+      while (off < len) {
+        while (k < nblk && bnd(k) <= G + off) rg[k++] = (int)pc.size();
+        const long next = (k < nblk) ? bnd(k) : G + len;
+        const int end = (int)std::min<long>(len, next - G);
+        const int fl = ((off == 0) ? (c4.w & 1) : 0) | ((end == len) ? (c4.w & 2) : 0);
+        pc.push_back(make_int4(c4.x, c4.y + off, c4.y + end, fl));
+        off = end;
+      }
+      G += len;
     }
-    for (; b <= nblk; b++) rg[b] = (int)ch.size();
+    while (k <= nblk) rg[k++] = (int)pc.size();
+    if (W.pchunks.ensure(pc.size() * sizeof(int4) + 16)) return MFISTA_E_HIP;
+    if (!pc.empty())
+      MF_CHECK(hipMemcpy(W.pchunks.p, pc.data(), pc.size() * sizeof(int4), hipMemcpyHostToDevice));
     MF_CHECK(hipMemcpy(W.granges.p, rg.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice));
   }
   if (P.xinit != W.px.p)
@@ -1276,8 +1297,8 @@ static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P,
   A.Gf64b = pair ? (double2*)((char*)W.Gf64.p + gf64) : nullptr;
   A.pair = pair;
   for (int q = 0; q < 4; q++) A.vm[q] = (double2*)W.vmod.p + (size_t)q * N.M;
-  A.chunks = (const int4*)N.chunks.p;
-  A.nchunks = N.nchunks;
+  A.chunks = (const int4*)W.pchunks.p;
+  A.nchunks = (int)pc.size();
   A.granges = (const int*)W.granges.p;
   A.ent = (const unsigned*)N.ent.p;
   A.x = (double*)W.px.p;

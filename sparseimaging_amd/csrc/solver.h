@@ -47,6 +47,7 @@ struct PersistWork {
   DevMem T64, Gf64;        // FP64 forward transform of the step
   DevMem Hg, U;            // FP32 adjoint grid and column-pass output
   DevMem part, state, cost, bar, fgp, fgpc, granges;
+  DevMem pchunks;          // the gridding chunk list re-cut at the workgroups' range boundaries
   DevMem x0, xo, box;      // batched host-pointer solves: staged xinit, xout, cl_box
   DevMem pargs;            // the solver kernel's arguments (device copy)
 };
