@@ -373,12 +373,17 @@ __device__ inline float lpq_f(const float* p, const float* q, int i, int j, int 
 // exp(l a_x - l^2 t3x + l' a_y - l'^2 t3y), conjugated at -k for mirror entries.
 constexpr int kGB = MF_BLOCK / kGridChunk;
 
-// One gridding entry as the gather loop reads it: the tap offsets of the
-// tile's cell (0, 0), the direction (+1 direct, -1 mirror), the tap slopes and
-// the deposit; 32 bytes, read with two 16-byte LDS loads.
+// One gridding entry as the gather loop reads it (48 bytes, three 16-byte
+// LDS loads).  Cell (row, col) of the tile takes taps l = ox + sg col along x
+// and l' = oy + sg row along y (sg = +1 direct, -1 mirror).  The loop works
+// with u = l - 1/2 and u' = l' - 1/2: the cell is covered when |u| < MSP and
+// |u'| < MSP, and its weight exp(l a_x - l^2 t3x + l' a_y - l'^2 t3y) is
+// 2^((a2x u + a1x) u + (a2y u' + a1y) u' + a0), the coefficients holding
+// log2(e) and the half-tap shift.
 struct alignas(16) GridEnt {
-  int oy, ox, sg, pad;
-  float ax, ay, dx, dy;
+  float ux, uy, sg, a0;    // u and u' of cell (0, 0), direction, constant term
+  float a1x, a2x, a1y, a2y;
+  float dx, dy, pad0, pad1;  // deposit E1 w r (conjugated for mirror entries)
 };
 
 template <int CPT, typename Resid>
@@ -388,15 +393,20 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
   const int TX = g.TX, TY = g.TY;
   GridEnt* er = reinterpret_cast<GridEnt*>(lds);
   int4* cmeta = reinterpret_cast<int4*>(er + MF_BLOCK);  // kGB chunk records
-  const int MSP = g.MSP, NX = g.NX;
-  const float t3x = (float)g.t3x, t3y = (float)g.t3y;
+  const int NX = g.NX;
+  const float fmsp = (float)g.MSP;
+  const double t3x = g.t3x, t3y = g.t3y;
+  constexpr double kL2E = 1.4426950408889634074;  // log2(e)
   // the cells this thread owns: (rr[c], cc[c]) within the tile
   int rr[CPT], cc[CPT];
+  float frr[CPT], fcc[CPT];
 #pragma unroll
   for (int c = 0; c < CPT; c++) {
     const int cell = threadIdx.x + c * MF_BLOCK;
     rr[c] = cell / TX;
     cc[c] = cell - rr[c] * TX;
+    frr[c] = (float)rr[c];
+    fcc[c] = (float)cc[c];
   }
   float2 acc[CPT];
 #pragma unroll
@@ -440,15 +450,23 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
           const int trb = tr * TY, tcb = tc * TX;
           // tap offsets of cell (row, col): direct l' = row - my, l = col - mx - NX;
           // mirror l' = -row - my, l = NX - mx - col
+          const int oy = mir ? -(trb + m.y) : trb - m.y;
+          const int ox = mir ? NX - m.x - tcb : tcb - m.x - NX;
+          // l a - l^2 t3 with l = u + 1/2: -t3 u^2 + (a - t3) u + (a/2 - t3/4)
+          const double ax = a.x, ay = a.y;
           GridEnt ge;
-          ge.oy = mir ? -(trb + m.y) : trb - m.y;
-          ge.ox = mir ? NX - m.x - tcb : tcb - m.x - NX;
-          ge.sg = mir ? -1 : 1;
-          ge.pad = 0;
-          ge.ax = a.x;
-          ge.ay = a.y;
+          ge.ux = (float)ox - 0.5f;
+          ge.uy = (float)oy - 0.5f;
+          ge.sg = mir ? -1.f : 1.f;
+          ge.a0 = (float)(kL2E * (0.5 * ax - 0.25 * t3x + 0.5 * ay - 0.25 * t3y));
+          ge.a1x = (float)(kL2E * (ax - t3x));
+          ge.a2x = (float)(-kL2E * t3x);
+          ge.a1y = (float)(kL2E * (ay - t3y));
+          ge.a2y = (float)(-kL2E * t3y);
           ge.dx = ew * r.x;
           ge.dy = mir ? ew * r.y : -ew * r.y;
+          ge.pad0 = 0.f;
+          ge.pad1 = 0.f;
           er[q] = ge;
         }
       }
@@ -470,14 +488,12 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
         const GridEnt ge = er[q];
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-          const int lp = ge.oy + ge.sg * rr[c];
-          const int l = ge.ox + ge.sg * cc[c];
-          if (lp > -MSP && lp <= MSP && l > -MSP && l <= MSP) {
-            const float fl = (float)l, flp = (float)lp;
-            const float wgt = dev_exp(fl * ge.ax - fl * fl * t3x + flp * ge.ay - flp * flp * t3y);
-            acc[c].x += wgt * ge.dx;
-            acc[c].y += wgt * ge.dy;
-          }
+          const float u = fmaf(ge.sg, fcc[c], ge.ux), up = fmaf(ge.sg, frr[c], ge.uy);
+          const float ex = fmaf(fmaf(ge.a2x, u, ge.a1x), u, ge.a0);
+          const float e = fmaf(fmaf(ge.a2y, up, ge.a1y), up, ex);
+          const float wgt = (fabsf(u) < fmsp && fabsf(up) < fmsp) ? __builtin_amdgcn_exp2f(e) : 0.f;
+          acc[c].x = fmaf(wgt, ge.dx, acc[c].x);
+          acc[c].y = fmaf(wgt, ge.dy, acc[c].y);
         }
       }
       if (ch.w & 2) {  // last chunk of the tile
