@@ -76,8 +76,14 @@ void launch_vis_params(const int* perm, const double* u, const double* v, const 
                        const double* vi, const double* sd, int M, int NX, int NY, int preset,
                        int MSP, VisParams<float> pf, VisParams<double> pd, int* yneg,
                        hipStream_t st);
+// Tile-list slots per visibility: the 2 MSP x 2 MSP footprint touches at most
+// kx x ky tiles directly and kx x km through its conjugate mirror below ky = 0.
+inline int tile_slots(int MSP, int TX, int TY) {
+  const int kx = (2 * MSP - 2) / TX + 2, ky = (2 * MSP - 2) / TY + 2, km = MSP / TY + 1;
+  return kx * ky + kx * km;
+}
 void launch_tile_entries(const int2* mxy, int M, int NX, int NY, int MSP, int TX, int TY,
-                         int ntx, int nty, int* keys, unsigned* vals, hipStream_t st);
+                         int ntx, int nty, int slots, int* keys, unsigned* vals, hipStream_t st);
 void launch_tile_ranges(const int* keys_sorted, int n, int ntiles, int* start, int* end,
                         hipStream_t st);
 

@@ -186,21 +186,21 @@ __global__ void k_vis_params(const int* __restrict__ perm, const double* __restr
 // up to 6 gridding-tile entries per visibility: <= 4 for the direct footprint
 // (clipped to ky >= 0) and <= 2 for the mirrored part of taps at ky <= 0.
 __global__ void k_tile_entries(const int2* __restrict__ mxy, int M, int NX, int NY, int MSP,
-                               int TX, int TY, int ntx, int nty, int* __restrict__ keys,
+                               int TX, int TY, int ntx, int nty, int slots, int* __restrict__ keys,
                                unsigned* __restrict__ vals) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= M) return;
   const int2 m = mxy[s];
   const int sentinel = ntx * nty;
   int n = 0;
-  int* kk = keys + (size_t)6 * s;
-  unsigned* vv = vals + (size_t)6 * s;
+  int* kk = keys + (size_t)slots * s;
+  unsigned* vv = vals + (size_t)slots * s;
   {
     const int r0 = max(0, m.y - MSP + 1), r1 = m.y + MSP;
     const int c0 = m.x + NX - MSP + 1, c1 = m.x + NX + MSP;
     for (int tr = r0 / TY; tr <= r1 / TY && tr < nty; tr++)
       for (int tc = c0 / TX; tc <= c1 / TX && tc < ntx; tc++)
-        if (n < 6) {
+        if (n < slots) {
           kk[n] = tr * ntx + tc;
           vv[n] = (unsigned)s;
           n++;
@@ -211,13 +211,13 @@ __global__ void k_tile_entries(const int2* __restrict__ mxy, int M, int NX, int 
     const int c0 = NX - m.x - MSP, c1 = NX - m.x + MSP - 1;
     for (int tr = 0; tr <= r1 / TY && tr < nty; tr++)
       for (int tc = c0 / TX; tc <= c1 / TX && tc < ntx; tc++)
-        if (n < 6) {
+        if (n < slots) {
           kk[n] = tr * ntx + tc;
           vv[n] = (unsigned)s | 0x80000000u;
           n++;
         }
   }
-  for (; n < 6; n++) {
+  for (; n < slots; n++) {
     kk[n] = sentinel;
     vv[n] = 0;
   }
@@ -338,9 +338,9 @@ void launch_vis_params(const int* perm, const double* u, const double* v, const 
 }
 
 void launch_tile_entries(const int2* mxy, int M, int NX, int NY, int MSP, int TX, int TY,
-                         int ntx, int nty, int* keys, unsigned* vals, hipStream_t st) {
+                         int ntx, int nty, int slots, int* keys, unsigned* vals, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_entries, dim3((M + 255) / 256), dim3(256), 0, st, mxy, M, NX, NY, MSP,
-                     TX, TY, ntx, nty, keys, vals);
+                     TX, TY, ntx, nty, slots, keys, vals);
 }
 
 void launch_tile_ranges(const int* keys_sorted, int n, int ntiles, int* start, int* end,

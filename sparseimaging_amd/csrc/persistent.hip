@@ -732,7 +732,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
         }
         return make_float2((float)(V.x * w - m.x), (float)(V.y * w - m.y));
       };
-      grid_chunks<(MSPC <= 8) ? 1 : 4>(g, A.pf, resid, GP(A.chunks), GP(A.granges)[lb],
+      grid_chunks<1>(g, A.pf, resid, GP(A.chunks), GP(A.granges)[lb],
                                               GP(A.granges)[lb + 1], GP(A.ent),
                                               GP(A.Hg), lds);
       if (!grid_sync(B, epoch, 4)) return;

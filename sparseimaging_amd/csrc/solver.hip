@@ -92,10 +92,12 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   g.MSP = MSP;
   g.Lx = 2 * NX;
   g.Ly = 2 * NY;
-  // gridding tiles: 16 x 16 cells for the 12-tap footprint (one cell per
-  // thread; the footprint covers 56 % of the tile), 32 x 32 for 24 taps
-  g.TX = (MSP <= 8) ? 16 : 32;
-  g.TY = (MSP <= 8) ? 16 : 32;
+  // gridding tiles: 16 x 16 cells, one per thread of the persistent gatherer.
+  // Per visibility the gather evaluates (tiles touched) x (cells per tile):
+  // ~730 cells for the 12-tap footprint, ~1500 for 24 taps (3000 with 32 x 32
+  // tiles).
+  g.TX = 16;
+  g.TY = 16;
   g.ntx = (g.Lx + g.TX - 1) / g.TX;
   g.nty = (NY + 1 + g.TY - 1) / g.TY;
   const double pi = M_PI, Mrx = 2.0 * NX, Mry = 2.0 * NY;
@@ -140,7 +142,8 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   }
   const size_t Ms = (size_t)M;
   const int ntiles = g.ntx * g.nty;
-  const size_t nent = 6 * Ms;
+  const int slots = tile_slots(MSP, g.TX, g.TY);
+  const size_t nent = (size_t)slots * Ms;
   if (N.mxy.ensure(Ms * sizeof(int2)) || N.af.ensure(Ms * sizeof(float2)) ||
       N.e1f.ensure(Ms * 4) || N.visf.ensure(Ms * sizeof(float2)) || N.wf.ensure(Ms * 4) ||
       N.ad.ensure(Ms * sizeof(double2)) || N.e1d.ensure(Ms * 8) ||
@@ -188,7 +191,7 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   }
   launch_vis_params((const int*)N.perm.p, d_u, d_v, vr, vi, sd, M, NX, NY, preset, MSP, N.pf(),
                     N.pd(), (int*)N.yneg.p, st);
-  launch_tile_entries((const int2*)N.mxy.p, M, NX, NY, MSP, g.TX, g.TY, g.ntx, g.nty,
+  launch_tile_entries((const int2*)N.mxy.p, M, NX, NY, MSP, g.TX, g.TY, g.ntx, g.nty, slots,
                       (int*)N.keys_a.p, (unsigned*)N.vals_a.p, st);
   tb = N.sort_tmp.n;
   if (sort_pairs_u(N.sort_tmp.p, &tb, (int*)N.keys_a.p, (int*)N.keys_b.p, (unsigned*)N.vals_a.p,
