@@ -197,9 +197,13 @@ def bench_warm_start(sa, nprob):
     tc = tw = 0.0
     acc = 0
     it_c, it_w, rel, dist = [], [], [], []
+    probs, x0s, net_s = [], [], 0.0
     for k in range(nprob):
         P = learn.make_problem(256, 20000, rng)
         args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"])
+        probs.append(dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=256,
+                          NY=256, lambda_l1=1.0, lambda_tsv=100.0, maxiter=50000, eps=0.0, cinit=1e4,
+                          nonneg_flag=1))
         t0 = time.perf_counter()
         xc, rc, _ = sa.mfista_imaging_core_nufft(*args, 256, 256, lambda_l1=1.0, lambda_tsv=100.0,
                                                  preset=sa.PRESET_CXX12, **kw)
@@ -208,6 +212,8 @@ def bench_warm_start(sa, nprob):
         _, rw, info = learn.solve_warm(net, *args, 256, lambda_l1=1.0, lambda_tsv=100.0, **kw)
         tw += time.perf_counter() - t0
         dist.append(float(np.linalg.norm(info["x0"] - xc) / np.linalg.norm(xc)))
+        x0s.append(info["x0"])
+        net_s += info["net_ms"] * 1e-3
         ok = rw["finalcost"] <= rc["finalcost"]
         acc += int(ok)
         it_c.append(rc["ITER"])
@@ -218,6 +224,21 @@ def bench_warm_start(sa, nprob):
            "mean_iter_warm": float(np.mean(it_w)), "max_objective_rel_diff_warm_minus_cold": float(np.max(rel)),
            "mean_start_distance": round(float(np.mean(dist)), 4),  # |x0 - x_cold| / |x_cold|
            "training": {k: net.meta.get(k) for k in ("n", "epochs", "N", "M")}}
+    # batching and the warm start together: the same problems in one batch
+    # launch from the cold start and from the network's starts (the network's
+    # inference time is charged to the warm batch)
+    sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)  # warm-up
+    t0 = time.perf_counter()
+    _, rbc, _ = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+    tbc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, rbw, _ = sa.mfista_imaging_core_nufft_batch([dict(p, xinit=x0) for p, x0 in zip(probs, x0s)],
+                                                   preset=sa.PRESET_CXX12, rel_tol=1e-6)
+    tbw = time.perf_counter() - t0 + net_s
+    out["batched"] = {"cold_solves_per_s": round(nprob / tbc, 3), "warm_solves_per_s": round(nprob / tbw, 3),
+                      "warm_batch_vs_single_cold": round(tc / tbw, 3),
+                      "max_objective_rel_diff_warm_minus_cold": float(max(
+                          (w["finalcost"] - c["finalcost"]) / c["finalcost"] for w, c in zip(rbw, rbc)))}
     log("bench: learned warm start " + json.dumps(out))
     return out
 
