@@ -124,16 +124,43 @@ __device__ __forceinline__ void gst(T* p, const T& v) {
   }
 }
 
+// Cross-lane move of a double by DPP control CTRL (register-only, no LDS).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave, in a fixed order, returned to every lane.
+// Quad permutes [1,0,3,2] and [2,3,0,1] give every lane its quad's sum, the
+// half-row mirror (lane i of 8 reads lane 7 - i) the sum of its 8 lanes, the
+// row mirror (lane i of 16 reads lane 15 - i) the sum of its row of 16; the
+// four row sums are then read from lanes 0, 16, 32 and 48.  All lanes must be
+// active.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  return ((read_lane(v, 0) + read_lane(v, 16)) + read_lane(v, 32)) + read_lane(v, 48);
+}
+
 // Fixed-order block reduction of NV doubles per thread; thread 0 of the block
-// receives the sums.  blockDim.x must be a multiple of 64 and <= 1024.
+// receives the sums.  blockDim.x must be a multiple of 64 and <= 1024, and
+// every thread of the block must call it.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double* smem /* >= 16*NV */) {
 #pragma unroll
-  for (int q = 0; q < NV; q++) {
-    double s = v[q];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    v[q] = s;
-  }
+  for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (lane == 0)
     for (int q = 0; q < NV; q++) smem[wid * NV + q] = v[q];

@@ -261,7 +261,7 @@ __device__ inline void part_reduce(const double* part, int nrow, const int (&slo
     double s = 0.0;
 #pragma unroll 8
     for (int i = lane; i < nrow; i += 64) s += gld(col + i);
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    s = wave_sum(s);
     if (lane == 0) red[q] = s;
   }
   __syncthreads();
