@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # HBM traffic of one launch of the bench workload, measured with rocprofv3
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors read off a 256 MiB copy and
 # fill; profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md): the two
-# counters come from two runs, solves of 579 and 576 iterations in 759 and 751
+# counters come from two runs, solves of 578 and 580 iterations in 761 and 762
 # paired trial rounds.
-MEASURED_TRAFFIC = {"bytes": 20284390492, "iterations": 578, "trial_rounds": 755}
+MEASURED_TRAFFIC = {"bytes": 20429725389, "iterations": 579, "trial_rounds": 762}
 
 
 def log(*a):
