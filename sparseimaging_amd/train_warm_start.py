@@ -29,6 +29,7 @@ def main():
     ap.add_argument("--mse", action="store_true", help="plain MSE instead of the relative squared error")
     ap.add_argument("--in-ch", type=int, default=3, choices=(2, 3),
                     help="input channels: dirty image and log-lambda plane (2), plus the dirty beam (3)")
+    ap.add_argument("--init", default=None, help="start from these saved weights (fine-tuning)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
@@ -37,7 +38,11 @@ def main():
     def log(msg):
         print(msg, flush=True)
 
-    net = learn.UNet(base=args.base, levels=args.levels, in_ch=args.in_ch, seed=args.seed)
+    if args.init:
+        net = learn.UNet.load(args.init)
+        args.in_ch = net.in_ch
+    else:
+        net = learn.UNet(base=args.base, levels=args.levels, in_ch=args.in_ch, seed=args.seed)
     t0 = time.time()
     X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=args.solve_batch,
                                         log=log, channels=args.in_ch)
@@ -48,7 +53,11 @@ def main():
     t_train = time.time() - t1
     out = args.out or learn.WEIGHTS
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
-    net.save(out, meta={"n": args.n, "N": args.N, "M": args.M, "epochs": args.epochs, "seed": args.seed,
+    prev = getattr(net, "meta", None) or {}
+    n_total = args.n + (prev.get("n", 0) if args.init else 0)
+    ep_total = args.epochs + (prev.get("epochs", 0) if args.init else 0)
+    net.save(out, meta={"n": n_total, "N": args.N, "M": args.M, "epochs": ep_total, "seed": args.seed,
+                        "fine_tuned_from": os.path.basename(args.init) if args.init else None,
                         "loss_kind": "mse" if args.mse else "relative",
                         "loss": hist, "data_s": round(t_data, 1), "train_s": round(t_train, 1)})
     log(f"saved {out}: data {t_data:.1f} s, training {t_train:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
