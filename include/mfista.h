@@ -143,6 +143,12 @@ typedef struct mfista_opts {
                        evaluates the trials c and c*ETA together (MFISTA_PERSISTENT_PAIRED=0:
                        one trial per round) and takes them in order, as the reference does */
   int single_trials; /* in: nonzero = one backtracking trial per round (persistent engine) */
+  int engine_used;   /* out: NUFFT solves, the engine that ran (MFISTA_ENGINE_*): a single
+                        solve falls back to the stream engine when the runtime refuses the
+                        cooperative launch */
+  int deterministic; /* in: nonzero = reproducible NUFFT solves (persistent engine): every
+                        gridding tile is summed by one workgroup in a fixed order, with no
+                        floating-point atomics, so repeated solves are bit-identical */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);
@@ -158,8 +164,11 @@ int mfista_imaging_core_nufft_ex(const double *u_dx, const double *v_dy, const d
 
 /* Same as mfista_imaging_core_nufft_ex, but every array is a DEVICE pointer
  * (HBM-resident inputs, e.g. from hipMalloc or a torch tensor's data_ptr) and
- * the work is queued on `stream` (a hipStream_t, NULL = the library's own).
- * RESULT and opts stay host structures. */
+ * the work is queued on `stream` (a hipStream_t; NULL = the HIP null stream),
+ * so it is ordered after the caller's earlier work on that stream: inputs
+ * written there need no synchronisation before the call.  The call returns
+ * when the solve has finished and xout is written.  RESULT and opts stay host
+ * structures. */
 int mfista_nufft_solve_device(const double *d_u, const double *d_v, const double *d_vis_r,
                               const double *d_vis_i, const double *d_vis_std, int M, int NX,
                               int NY, int maxiter, double eps, double lambda_l1,
@@ -265,6 +274,14 @@ int mfista_profile_stages(int reps, double *ms, int nstage);
 long mfista_last_steps(void);
 
 const char *mfista_amd_version(void);
+
+/* Releases the library's per-device state: waits for its work, frees its
+ * device and pinned host memory, destroys its streams and events.  Optional;
+ * call it before the HIP runtime is torn down (e.g. at the end of main, or from
+ * an atexit handler registered after the first call), not from a static
+ * destructor.  A later call of any entry point starts afresh.  No counterpart
+ * in the reference, which keeps no state between calls. */
+void mfista_amd_finalize(void);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@ otherwise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 
@@ -39,7 +40,7 @@ EXPORTED = [
     "mfista_imaging_core_nufft", "mfista_imaging_core_fft", "mfista_imaging_core_dft",
     "x2y_nufft", "y2x_nufft", "mfista_default_opts", "mfista_imaging_core_nufft_ex",
     "mfista_nufft_solve_device", "mfista_nufft_forward", "mfista_nufft_adjoint",
-    "mfista_amd_version", "mfista_profile_stages", "mfista_last_steps",
+    "mfista_amd_version", "mfista_amd_finalize", "mfista_profile_stages", "mfista_last_steps",
     "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
     "mfista_movie_nufft", "mfista_looe_dft", "compute_LOOE_L1", "compute_LOOE_L1_TSV",
 ]
@@ -76,7 +77,7 @@ class Opts(C.Structure):
         ("preset", C.c_int), ("fgp_iter", C.c_int), ("rel_tol", C.c_double), ("verbose", C.c_int),
         ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
         ("engine", C.c_int), ("kernel_ms", C.c_double), ("trial_rounds", C.c_int),
-        ("single_trials", C.c_int),
+        ("single_trials", C.c_int), ("engine_used", C.c_int), ("deterministic", C.c_int),
     ]
 
 
@@ -109,6 +110,10 @@ def lib():
             pass
         _lib = C.CDLL(LIB_PATH)
         _declare(_lib)
+        # free the library's streams and memory at interpreter exit, while the
+        # HIP runtime is still up (Python's exit handlers run before the C
+        # runtime's and the shared libraries' destructors)
+        atexit.register(_lib.mfista_amd_finalize)
     return _lib
 
 
@@ -162,6 +167,8 @@ def _declare(L):
     L.mfista_last_steps.argtypes = []
     L.mfista_amd_version.restype = C.c_char_p
     L.mfista_amd_version.argtypes = []
+    L.mfista_amd_finalize.restype = None
+    L.mfista_amd_finalize.argtypes = []
 
 
 def _d(a):
@@ -184,27 +191,30 @@ def _box(cl_box, n):
 
 
 def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT,
-              single_trials=False):
+              single_trials=False, deterministic=False):
     o = Opts()
     lib().mfista_default_opts(C.byref(o))
     o.preset, o.fgp_iter, o.rel_tol, o.verbose, o.engine = preset, fgp_iter, rel_tol, verbose, engine
     o.single_trials = 1 if single_trials else 0
+    o.deterministic = 1 if deterministic else 0
     return o
 
 
 def _info(o):
     return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials,
-            "kernel_ms": o.kernel_ms, "trial_rounds": o.trial_rounds}
+            "kernel_ms": o.kernel_ms, "trial_rounds": o.trial_rounds, "engine_used": o.engine_used}
 
 
 # ---------------------------------------------------------------- solvers
 def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
                               nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
-                              rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False):
+                              rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False):
     """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts).
     single_trials=True evaluates one backtracking trial per round in the persistent
-    engine (default: L1+TSV rounds evaluate the trials c and c*ETA together)."""
+    engine (default: L1+TSV rounds evaluate the trials c and c*ETA together);
+    deterministic=True grids without floating-point atomics (bit-identical
+    repeated solves, persistent engine)."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
     vr, pvr = _d(vis_r)
@@ -216,7 +226,8 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     xout = np.zeros(NX * NY)
     bflag, b, pb = _box(cl_box, NX * NY)
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
+                  deterministic=deterministic)
     rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
                                             lambda_tv, lambda_tsv, cinit, px0,
                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
@@ -227,7 +238,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
 
 
 def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0,
-                                    single_trials=False):
+                                    single_trials=False, deterministic=False):
     """Independent NUFFT solves in one persistent launch (no reference counterpart:
     a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
     the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
@@ -274,7 +285,7 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
         keep.append(b)
         q.box_flag, q.cl_box = bflag, pb
     res = (RESULT * n)()
-    o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials)
+    o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials, deterministic=deterministic)
     rc = lib().mfista_imaging_core_nufft_batch(n, arr, res, C.byref(o))
     if rc != 0:
         raise MfistaError(rc)
@@ -284,7 +295,7 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
 
 def mfista_movie_nufft(vis, u, v, noise_std, time_idx, c_box, Nx, Ny, xinit, lambda_l1, lambda_tsv, lambda_t,
                        nonneg_flag=1, cinit=1e4, maxiter=MAXITER, eps=EPS, preset=PRESET_C24, rel_tol=0.0,
-                       single_trials=False):
+                       single_trials=False, deterministic=False):
     """Coupled movie solve on the GPU with the arguments of
     matlab/MFISTA_movie_eht.m:1-4 (vis complex; time_idx an (M,) array of
     1-based frame numbers or an (M, 2) array [frame, time]; c_box None or an
@@ -314,7 +325,7 @@ def mfista_movie_nufft(vis, u, v, noise_std, time_idx, c_box, Nx, Ny, xinit, lam
     terms = np.zeros(4)
     om_r, om_i = np.zeros(M), np.zeros(M)
     iters = C.c_int(0)
-    o = make_opts(preset, FGPITER, rel_tol, single_trials=single_trials)
+    o = make_opts(preset, FGPITER, rel_tol, single_trials=single_trials, deterministic=deterministic)
     rc = lib().mfista_movie_nufft(pvr, pvi, pu, pv, psd, fi.ctypes.data_as(_ip), ptv, M,
                                   None if box is None else box.ctypes.data_as(_fp), Nx, Ny,
                                   None if x0 is None else x0.ctypes.data_as(_dp), lambda_l1, lambda_tsv,
@@ -402,9 +413,12 @@ def compute_looe(A, x, yAx, NX=1, NY=1, lambda_tsv=0.0):
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
                        lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
                        t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None,
-                       engine=ENGINE_PERSISTENT, single_trials=False):
+                       engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False):
     """HBM-resident solve: every array argument is a device pointer (int) or a
-    torch CUDA tensor (float64; cl_box float32).  Returns (RESULT dict, opts dict)."""
+    torch CUDA tensor (float64; cl_box float32).  The solve is queued on
+    `stream` (a hipStream_t as int); by default on torch's current stream, so
+    that it is ordered after the writes of the inputs.  Returns (RESULT dict,
+    opts dict)."""
 
     def ptr(t):
         if t is None:
@@ -414,8 +428,13 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
     M = t_u.numel() if hasattr(t_u, "numel") else None
     if M is None:
         raise ValueError("pass torch tensors (M is taken from t_u)")
+    if stream is None:
+        import torch
+
+        stream = torch.cuda.current_stream().cuda_stream
     res = RESULT()
-    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials)
+    o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
+                  deterministic=deterministic)
     rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
                                          maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
                                          ptr(t_xinit), ptr(t_xout), nonneg_flag,

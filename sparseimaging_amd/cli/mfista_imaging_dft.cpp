@@ -69,6 +69,7 @@ int main(int argc, char** argv) {
   const int st = mfista_imaging_core_dft_ex(y.data(), A.data(), M, N, NX, NY, o.maxiter, o.eps, l1, ltv, ltsv,
                                             cinit, xinit.data(), x.data(), o.nonneg_flag, o.box_flag, box.data(),
                                             &res, &o.opts);
+  mfista_amd_finalize();  // release the GPU state while the HIP runtime is up
   if (st != MFISTA_OK && st != MFISTA_E_NONNEG) {
     std::fprintf(stderr, "%s: solve failed with status %d\n", argv[0], st);
     return 1;

@@ -52,6 +52,7 @@ int main(int argc, char** argv) {
   const int st = mfista_imaging_core_fft_ex(iu, iv, yr, yi, sd, M, NX, NY, o.maxiter, o.eps, l1, ltv, ltsv, cinit,
                                             xinit.data(), x.data(), o.nonneg_flag, o.box_flag, box.data(), &res,
                                             &o.opts);
+  mfista_amd_finalize();  // release the GPU state while the HIP runtime is up
   mfista_io_free(iu);
   mfista_io_free(iv);
   mfista_io_free(yr);

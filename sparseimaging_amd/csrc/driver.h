@@ -26,6 +26,7 @@ struct RunParams {
   double eps_abs = 0, rel_tol = 0, lambda_l1 = 0, lambda_tv = 0, lambda_tsv = 0, cinit = 0;
   double eta_first = 1.1;
   int single_trials = 0;          // persistent engine: one backtracking trial per round
+  int deterministic = 0;          // persistent engine: gridding without atomics (tile-exact ranges)
   const double* xinit = nullptr;  // device, FP64
   double* xout = nullptr;         // device, FP64
   const float* box = nullptr;     // device

@@ -344,6 +344,9 @@ struct FftState {
 };
 static FftState g_fft;
 
+// mfista_amd_finalize has freed every allocation: forget the FFT-mode state
+void reset_fft_state() { g_fft = FftState(); }
+
 struct FftOp : Operator<double> {
   Context& C;
   FftState& F;

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "deal.h"
 #include "driver.h"
 #include "stages.h"
 
@@ -1257,8 +1258,32 @@ static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P,
   // a cut piece keeps the tile's "first" flag only on its first piece and the
   // "last" flag only on its last, so a tile split between workgroups is
   // flushed with atomics.
+  //
+  // Deterministic mode (P.deterministic): ranges are cut at tile boundaries
+  // only, so every tile is gathered by one workgroup, in entry order, and
+  // stored without atomics.  The adjoint grid is then the same bit for bit on
+  // every run and every launch size; the cost is a less even split of the
+  // gridding work (a dense tile is not shared).
   std::vector<int4> pc;
-  {
+  if (P.deterministic) {
+    const std::vector<int4>& ch = N.h_chunks;
+    std::vector<int> rg(nblk + 1, 0);
+    long tot = 0;
+    for (const int4& c4 : ch) tot += c4.z - c4.y;
+    pc = ch;
+    long G = 0;  // entries before chunk j
+    int k = 1;
+    for (int j = 0; j < (int)ch.size(); j++) {
+      if (ch[j].w & 1)  // a tile starts here: workgroups whose share begins by now start at it
+        while (k < nblk && tot * k / nblk <= G) rg[k++] = j;
+      G += ch[j].z - ch[j].y;
+    }
+    while (k <= nblk) rg[k++] = (int)ch.size();
+    if (W.pchunks.ensure(pc.size() * sizeof(int4) + 16)) return MFISTA_E_HIP;
+    if (!pc.empty())
+      MF_CHECK(hipMemcpy(W.pchunks.p, pc.data(), pc.size() * sizeof(int4), hipMemcpyHostToDevice));
+    MF_CHECK(hipMemcpy(W.granges.p, rg.data(), (size_t)(nblk + 1) * 4, hipMemcpyHostToDevice));
+  } else {
     const std::vector<int4>& ch = N.h_chunks;
     std::vector<int> rg(nblk + 1, 0);
     long tot = 0;
@@ -1387,6 +1412,17 @@ static int persistent_collect(PersistWork& W, const RunParams& P, hipStream_t st
 // launch, which it does in a process while another process holds the device's
 // cooperative-launch resources.
 static int timed_launch(const PLaunch& L, void** args, int nblk, hipStream_t st, float* ms) {
+  if (!ms) {  // untimed (the allocation-free entry point creates no events)
+    const hipError_t le = hipLaunchCooperativeKernel(L.fn, dim3(nblk), dim3(MF_BLOCK), args, (unsigned)L.sh, st);
+    if (le != hipSuccess) {
+      (void)hipGetLastError();
+      fprintf(stderr, "[mfista_amd] cooperative launch refused (%s)\n", hipGetErrorName(le));
+      return kCoopRefused;
+    }
+    MF_CHECK(hipStreamSynchronize(st));
+    MF_CHECK(hipGetLastError());
+    return 0;
+  }
   hipEvent_t e0, e1;
   MF_CHECK(hipEventCreate(&e0));
   MF_CHECK(hipEventCreate(&e1));
@@ -1472,7 +1508,7 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   const PArgs* dA = (const PArgs*)C.pw.pargs.p;
   void* args[] = {&dA};
   float kms = 0;
-  if ((rc = timed_launch(L, args, L.nblk, st, &kms))) return rc;
+  if ((rc = timed_launch(L, args, L.nblk, st, C.no_alloc ? nullptr : &kms))) return rc;
   out->kernel_ms = kms;
   C.persist_blocks = L.nblk;
   if ((rc = persistent_collect(C.pw, P, st, out))) return rc;
@@ -1493,81 +1529,19 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   return 0;
 }
 
-// Deals the launch's workgroups to problems p0 .. p0+pn-1: blk[b] = (problem,
-// index within the problem), nb[i] = workgroups of problem i.  Problems go
-// XCD by XCD (workgroup b runs on XCD b mod 8): each problem goes whole to the
-// XCD with the least work so far, and each XCD's workgroups are split among its
-// problems in proportion to their work per iteration, so a problem's data
-// stays in one L2.  With fewer problems than XCDs each spans several XCDs.
-static void deal_workgroups(const std::vector<NufftState*>& sets, const std::vector<RunParams>& P, int p0, int pn,
-                            int nblk, std::vector<int2>& blk, std::vector<int>& nb) {
-  const int kX = 8;
-  const int per_x = nblk / kX;
+// Deals the launch's workgroups to problems p0 .. p0+pn-1 (deal.h): blk[b] =
+// (problem, index within the problem), nb[i] = workgroups of problem i; the
+// work per iteration of a problem is its pixels plus its gridding taps.
+static int deal_workgroups(const std::vector<NufftState*>& sets, const std::vector<RunParams>& P, int p0, int pn,
+                           int nblk, std::vector<int2>& blk, std::vector<int>& nb) {
   std::vector<double> w(pn);
   for (int i = 0; i < pn; i++)
     w[i] = (double)P[p0 + i].NX * P[p0 + i].NY + 4.0 * sets[p0 + i]->M * sets[p0 + i]->MSP;
-  std::vector<int> order(pn), xcd(pn, 0);
-  nb.assign(pn, 1);
-  for (int i = 0; i < pn; i++) order[i] = i;
-  std::sort(order.begin(), order.end(), [&](int a, int b) { return w[a] > w[b]; });
-  std::vector<double> load(kX, 0.0);
-  blk.assign(nblk, make_int2(-1, 0));
-  if (pn >= kX) {
-    std::vector<int> cnt(kX, 0);
-    for (int i : order) {
-      int best = -1;
-      for (int x = 0; x < kX; x++)
-        if (cnt[x] < per_x && (best < 0 || load[x] < load[best])) best = x;
-      xcd[i] = best;
-      load[best] += w[i];
-      cnt[best]++;
-    }
-    // split each XCD's workgroups among its problems (at least one each)
-    std::vector<std::vector<int>> slots(kX);
-    for (int b = 0; b < nblk; b++) slots[b % kX].push_back(b);
-    for (int x = 0; x < kX; x++) {
-      std::vector<int> mine;
-      for (int i = 0; i < pn; i++)
-        if (xcd[i] == x) mine.push_back(i);
-      if (mine.empty()) continue;
-      const int avail = (int)slots[x].size();
-      const int left = avail - (int)mine.size();
-      int used = 0;
-      for (int i : mine) {
-        nb[i] = 1 + (int)((double)left * w[i] / std::max(load[x], 1e-300));
-        used += nb[i];
-      }
-      for (size_t k = 0; used < avail; k = (k + 1) % mine.size(), used++) nb[mine[k]]++;
-      int s0 = 0;
-      for (int i : mine) {
-        for (int j = 0; j < nb[i]; j++) blk[slots[x][s0 + j]] = make_int2(i, j);
-        s0 += nb[i];
-      }
-    }
-  } else {
-    // fewer problems than XCDs: each spans several XCDs, workgroups in
-    // proportion to work, interleaved
-    double wt = 0;
-    for (int i = 0; i < pn; i++) wt += w[i];
-    int used = 0;
-    for (int i = 0; i < pn; i++) {
-      nb[i] = std::max(1, (int)((double)nblk * w[i] / wt));
-      used += nb[i];
-    }
-    while (used > nblk)
-      for (int i = 0; i < pn && used > nblk; i++)
-        if (nb[i] > 1) nb[i]--, used--;
-    std::vector<int> next(pn, 0);
-    int b = 0;
-    for (bool any = true; any;) {
-      any = false;
-      for (int i = 0; i < pn; i++)
-        if (next[i] < nb[i]) {
-          blk[b++] = make_int2(i, next[i]++);
-          any = true;
-        }
-    }
-  }
+  std::vector<std::pair<int, int>> d;
+  if (!deal_workgroups(w, nblk, d, nb)) return MFISTA_E_ARG;
+  blk.resize(d.size());
+  for (size_t b = 0; b < d.size(); b++) blk[b] = make_int2(d[b].first, d[b].second);
+  return 0;
 }
 
 // Launches problems p0 .. p0+pn-1 (already dealt) and collects their outcomes.
@@ -1621,7 +1595,7 @@ int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const st
     const int pn = std::min(np - p0, L.nblk);
     std::vector<int2> blk;
     std::vector<int> nb;
-    deal_workgroups(sets, P, p0, pn, L.nblk, blk, nb);
+    if ((rc = deal_workgroups(sets, P, p0, pn, L.nblk, blk, nb))) return rc;
     std::vector<PArgs> A(pn);
     for (int i = 0; i < pn; i++)
       if ((rc = persistent_prepare(*sets[p0 + i], C.bwork[p0 + i], P[p0 + i], nb[i], L.lds_tables,
@@ -1656,7 +1630,7 @@ int solve_nufft_movie(Context& C, const std::vector<NufftState*>& sets, const st
   *kernel_ms = 0;
   std::vector<int2> blk;
   std::vector<int> nb;
-  deal_workgroups(sets, P, 0, nf, L.nblk, blk, nb);
+  if ((rc = deal_workgroups(sets, P, 0, nf, L.nblk, blk, nb))) return rc;
   std::vector<PArgs> A(nf);
   for (int i = 0; i < nf; i++)
     if ((rc = persistent_prepare(*sets[i], C.bwork[i], P[i], nb[i], L.lds_tables, C.persist_hier_barrier, L.pair,

@@ -12,13 +12,31 @@
 
 namespace mf {
 
-// grow-only device allocation
+// grow-only device allocation; inside an ArenaScope it is carved from the
+// scope's preallocated device buffer instead (no hipMalloc)
 struct DevMem {
   void* p = nullptr;
   size_t n = 0;
+  bool in_arena = false;  // carved from an arena: release() does not free it
   int ensure(size_t bytes);
   void release();
 };
+
+// A bump allocator over preallocated device memory.  While an ArenaScope is
+// alive (library mutex held), every DevMem::ensure carves from it; a request
+// that does not fit fails (and sets `exhausted`) instead of allocating.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0, peak = 0;
+  bool exhausted = false;
+};
+struct ArenaScope {
+  explicit ArenaScope(Arena* a);
+  ~ArenaScope();
+};
+
+// device allocations made by DevMem::ensure since the library was loaded
+long device_allocations();
 
 struct NufftState {
   bool ready = false, have64 = false;
@@ -61,6 +79,7 @@ struct BatchSet {
 
 struct Context {
   int device = -1;
+  bool no_alloc = false;  // the allocation-free entry point: workspaces from its arena, no events
   hipStream_t stream = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
   int* hflags = nullptr;  // pinned

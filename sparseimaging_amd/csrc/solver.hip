@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <vector>
 
 #include "../../include/mfista.h"
@@ -28,24 +29,55 @@ int sort_pairs(void* temp, size_t* temp_bytes, const int* kin, int* kout, const 
 int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, const unsigned* vin,
                  unsigned* vout, int n, int end_bit, hipStream_t st);
 
+// Every live workspace allocation, so that mfista_amd_finalize can free them
+// all (DevMem objects live in contexts, in vectors that move them, and in
+// file-scope state).  Guarded by the library mutex, which every entry point
+// holds while it allocates.
+static std::set<void*> g_live;
+static long g_nalloc = 0;
+static Arena* g_arena = nullptr;
+
+ArenaScope::ArenaScope(Arena* a) { g_arena = a; }
+ArenaScope::~ArenaScope() { g_arena = nullptr; }
+
+long device_allocations() { return g_nalloc; }
+
 int DevMem::ensure(size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (bytes <= n) return 0;
-  if (p) (void)hipFree(p);
-  p = nullptr;
-  n = 0;
+  release();
+  if (g_arena) {
+    Arena& a = *g_arena;
+    const size_t off = (a.off + 255) & ~(size_t)255;
+    if (off > a.cap || bytes > a.cap - off) {
+      a.exhausted = true;
+      return -1;
+    }
+    p = a.base + off;
+    n = bytes;
+    in_arena = true;
+    a.off = off + bytes;
+    if (a.off > a.peak) a.peak = a.off;
+    return 0;
+  }
   if (hipMalloc(&p, bytes) != hipSuccess) {
     p = nullptr;
     return -1;
   }
+  g_nalloc++;
+  g_live.insert(p);
   n = bytes;
   return 0;
 }
 
 void DevMem::release() {
-  if (p) (void)hipFree(p);
+  if (p && !in_arena) {
+    g_live.erase(p);
+    (void)hipFree(p);
+  }
   p = nullptr;
   n = 0;
+  in_arena = false;
 }
 
 int upload_twiddles_pub(const FFTPlan& P, DevMem& tf, DevMem& td, hipStream_t st) {
@@ -468,8 +500,9 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     printf("nonneg_flag must be chosen properly.\n");
     rc = 0;
   } else {
-    if (nufft_ensure64(C) || C.x64.ensure((size_t)A.NX * A.NY * 8) ||
-        C.g32.ensure((size_t)A.NX * A.NY * 4))
+    if (nufft_ensure64(C)) return MFISTA_E_HIP;
+    // the stream engine's buffers (the allocation-free entry point has no stream engine)
+    if (!C.no_alloc && (C.x64.ensure((size_t)A.NX * A.NY * 8) || C.g32.ensure((size_t)A.NX * A.NY * 4)))
       return MFISTA_E_HIP;
     NufftOp op(C);
     RunParams P;
@@ -488,6 +521,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     P.cinit = A.cinit;
     P.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
     P.single_trials = (opts && opts->single_trials) ? 1 : 0;
+    P.deterministic = (opts && opts->deterministic) ? 1 : 0;
     P.xinit = A.xinit;
     P.xout = A.xout;
     P.box = A.box;
@@ -496,13 +530,16 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
     const int engine = opts ? opts->engine : MFISTA_ENGINE_PERSISTENT;
     rc = kCoopRefused;
+    int used = MFISTA_ENGINE_PERSISTENT;
     if (engine == MFISTA_ENGINE_PERSISTENT && coop) rc = solve_nufft_persistent(C, P, &rs);
-    if (rc == kCoopRefused) {  // the stream engine: the same loop, one launch per stage
+    if (rc == kCoopRefused && !C.no_alloc) {  // the stream engine: the same loop, one launch per stage
       if (engine == MFISTA_ENGINE_PERSISTENT && coop)
         fprintf(stderr, "[mfista_amd] solving with the stream engine instead\n");
       rs = RunStats();
+      used = MFISTA_ENGINE_STREAM;
       rc = mfista_run<double>(C, op, P, &rs);
     }
+    if (opts) opts->engine_used = used;
     C.last_steps = rs.steps;
   }
   res->ITER = rs.ITER;
@@ -567,6 +604,29 @@ Context* get_context() {
 
 std::mutex& global_mutex() { return g_mu; }
 
+void reset_fft_state();  // driver.hip
+
+// Releases every context and workspace (library mutex held): waits for the
+// library's work, frees device and pinned memory, destroys the streams and
+// events.  The next call creates a fresh context.
+static void finalize_all() {
+  for (Context*& C : g_ctx) {
+    if (!C) continue;
+    (void)hipSetDevice(C->device);
+    if (C->stream) (void)hipStreamSynchronize(C->stream);
+    for (hipEvent_t& e : C->ev)
+      if (e) (void)hipEventDestroy(e);
+    if (C->hflags) (void)hipHostFree(C->hflags);
+    if (C->stream) (void)hipStreamDestroy(C->stream);
+    delete C;
+    C = nullptr;
+  }
+  g_ctx.clear();
+  for (void* p : g_live) (void)hipFree(p);
+  g_live.clear();
+  reset_fft_state();
+}
+
 // upload helper for the host-pointer entry points
 int upload(Context& C, DevMem& m, const void* h, size_t bytes) {
   if (m.ensure(bytes)) return MFISTA_E_HIP;
@@ -580,7 +640,12 @@ using namespace mf;
 
 extern "C" {
 
-const char* mfista_amd_version(void) { return "mfista_amd 0.1 (gfx950)"; }
+const char* mfista_amd_version(void) { return "mfista_amd 0.2 (gfx950)"; }
+
+void mfista_amd_finalize(void) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  finalize_all();
+}
 
 void mfista_default_opts(mfista_opts* o) {
   memset(o, 0, sizeof(*o));
@@ -603,8 +668,12 @@ int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double
   if (!d_u || !d_v || !d_vis_r || !d_vis_i || !d_vis_std || !d_xinit || !d_xout || !res)
     return MFISTA_E_ARG;
   if (box_flag && !d_cl_box) return MFISTA_E_ARG;
+  // the work is queued on the caller's stream, NULL being the HIP null stream:
+  // either way it is ordered after the caller's earlier work on that stream
+  // (the library's own stream is a non-blocking one, unordered with respect
+  // to the caller's writes of the inputs)
   hipStream_t saved = C->stream;
-  if (stream) C->stream = (hipStream_t)stream;
+  C->stream = (hipStream_t)stream;
   SolveArgs A;
   A.u = d_u;
   A.v = d_v;
@@ -734,6 +803,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     R.cinit = q.cinit;
     R.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
     R.single_trials = (opts && opts->single_trials) ? 1 : 0;
+    R.deterministic = (opts && opts->deterministic) ? 1 : 0;
     R.xinit = (const double*)W.x0.p;
     R.xout = (double*)W.xo.p;
     R.box = q.box_flag ? (const float*)W.box.p : nullptr;
@@ -773,6 +843,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     R->mean_sq_error = R->sq_error / (double)q.M;
   }
   if (opts) {
+    opts->engine_used = MFISTA_ENGINE_PERSISTENT;
     opts->solve_ms = secs * 1e3;
     opts->kernel_ms = kms;
     opts->total_trials = total_trials;
@@ -881,6 +952,7 @@ int mfista_movie_nufft(const double* vis_r, const double* vis_i, const double* u
     R.cinit = cinit;
     R.eta_first = kEta;
     R.single_trials = (opts && opts->single_trials) ? 1 : 0;
+    R.deterministic = (opts && opts->deterministic) ? 1 : 0;
     R.xinit = (const double*)W.x0.p;
     R.xout = (double*)W.xo.p;
     R.box = c_box ? (const float*)W.box.p : nullptr;
@@ -952,6 +1024,7 @@ int mfista_movie_nufft(const double* vis_r, const double* vis_i, const double* u
     int tt = 0, mt = 0;
     for (const RunStats& r : st) mt = std::max(mt, r.max_trials);
     tt = st[0].total_trials;
+    opts->engine_used = MFISTA_ENGINE_PERSISTENT;
     opts->solve_ms = secs * 1e3;
     opts->kernel_ms = kms;
     opts->total_trials = tt;
