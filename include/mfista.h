@@ -275,6 +275,33 @@ long mfista_last_steps(void);
 
 const char *mfista_amd_version(void);
 
+/* Allocation-free variant of mfista_nufft_solve_device: the same arguments,
+ * the same solve and results, but every workspace comes from a scratch array
+ * of MFISTA_NOALLOC_SCRATCH_BYTES that the library's code object holds on each
+ * device (from the first call on that device until the process ends), so the
+ * call makes no hipMalloc/hipFree and creates no stream, event or pinned
+ * buffer.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid allocation).
+ * A problem whose workspaces do not fit returns MFISTA_E_SIZE with a message
+ * on stderr (a 256x256 solve with M = 20000 uses well under half of it).  Only the persistent
+ * engine runs here (opts->engine must be MFISTA_ENGINE_PERSISTENT; a refused
+ * cooperative launch returns MFISTA_E_COOP), and opts->kernel_ms is not
+ * measured (0).  All calls on one device share its scratch array: the library
+ * serialises them and each returns when its solve has finished; calls that
+ * overlapped on one device would race. */
+#define MFISTA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mfista_nufft_solve_device_noalloc(const double *d_u, const double *d_v, const double *d_vis_r,
+                                      const double *d_vis_i, const double *d_vis_std, int M, int NX,
+                                      int NY, int maxiter, double eps, double lambda_l1,
+                                      double lambda_tv, double lambda_tsv, double cinit,
+                                      const double *d_xinit, double *d_xout, int nonneg_flag,
+                                      int box_flag, const float *d_cl_box,
+                                      struct RESULT *mfista_result, mfista_opts *opts, void *stream);
+
+/* Diagnostic: the number of device allocations (hipMalloc) the library has
+ * made since it was loaded. */
+long mfista_device_allocations(void);
+
 /* Releases the library's per-device state: waits for its work, frees its
  * device and pinned host memory, destroys its streams and events.  Optional;
  * call it before the HIP runtime is torn down (e.g. at the end of main, or from

@@ -43,6 +43,7 @@ EXPORTED = [
     "mfista_amd_version", "mfista_amd_finalize", "mfista_profile_stages", "mfista_last_steps",
     "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
     "mfista_movie_nufft", "mfista_looe_dft", "compute_LOOE_L1", "compute_LOOE_L1_TSV",
+    "mfista_nufft_solve_device_noalloc", "mfista_device_allocations",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
@@ -142,6 +143,10 @@ def _declare(L):
     L.mfista_nufft_solve_device.restype = i
     L.mfista_nufft_solve_device.argtypes = [C.c_void_p] * 5 + [i, i, i, i, d, d, d, d, d] + [
         C.c_void_p, C.c_void_p, i, i, C.c_void_p, C.POINTER(RESULT), C.POINTER(Opts), C.c_void_p]
+    L.mfista_nufft_solve_device_noalloc.restype = i
+    L.mfista_nufft_solve_device_noalloc.argtypes = L.mfista_nufft_solve_device.argtypes
+    L.mfista_device_allocations.restype = C.c_long
+    L.mfista_device_allocations.argtypes = []
     L.mfista_nufft_forward.restype = i
     L.mfista_movie_nufft.argtypes = [_dp, _dp, _dp, _dp, _dp, _ip, _dp, i, _fp, i, i, _dp, d, d, d, i, d, i, d,
                                      _dp, _dp, _dp, _dp, _dp, _ip, C.POINTER(Opts)]
@@ -443,6 +448,39 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
     if rc != 0:
         raise MfistaError(rc)
     return res.as_dict(), _info(o)
+
+
+def solve_nufft_device_noalloc(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
+                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1, t_box=None,
+                               preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None, deterministic=False):
+    """solve_nufft_device through the allocation-free entry point: the
+    library's per-device scratch array (MFISTA_NOALLOC_SCRATCH_BYTES) holds
+    every workspace; no device memory is allocated.  Returns (RESULT dict,
+    opts dict)."""
+
+    def ptr(t):
+        if t is None:
+            return None
+        return t if isinstance(t, int) else t.data_ptr()
+
+    if stream is None:
+        import torch
+
+        stream = torch.cuda.current_stream().cuda_stream
+    res = RESULT()
+    o = make_opts(preset, fgp_iter, rel_tol, deterministic=deterministic)
+    rc = lib().mfista_nufft_solve_device_noalloc(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), t_u.numel(),
+                                                 NX, NY, maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
+                                                 ptr(t_xinit), ptr(t_xout), nonneg_flag, 0 if t_box is None else 1,
+                                                 ptr(t_box), C.byref(res), C.byref(o), stream)
+    if rc != 0:
+        raise MfistaError(rc)
+    return res.as_dict(), _info(o)
+
+
+def device_allocations():
+    """Device allocations the library has made since it was loaded (diagnostic)."""
+    return int(lib().mfista_device_allocations())
 
 
 def nufft_forward(u, v, x, preset=PRESET_C24, fp64=False):
