@@ -19,6 +19,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_mfista.so")
+# the same source built with OpenMP: the all-cores CPU baseline (timing only;
+# the checker is the serial build)
+_LIB_OMP_PATH = os.path.join(_HERE, "liboracle_mfista_omp.so")
 
 PRESET_C24 = 0
 PRESET_CXX12 = 1
@@ -52,16 +55,25 @@ class Opts(C.Structure):
 
 
 _lib = None
+_lib_omp = None
 
 
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(_LIB_PATH):
+    if force or not (os.path.exists(_LIB_PATH) and os.path.exists(_LIB_OMP_PATH)):
         subprocess.run(["make", "-s"] + (["-B"] if force else []) + ["-C", _HERE], check=True)
     return _LIB_PATH
 
 
-def lib():
-    global _lib
+def lib(omp: bool = False):
+    """The serial oracle (the checker), or with omp=True its OpenMP build."""
+    global _lib, _lib_omp
+    if omp:
+        if _lib_omp is None:
+            if not os.path.exists(_LIB_OMP_PATH):
+                build()
+            _lib_omp = C.CDLL(_LIB_OMP_PATH)
+            _declare(_lib_omp)
+        return _lib_omp
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
@@ -305,7 +317,7 @@ def _finish(rc, res, xout, opts, buf):
 
 def imaging_core_nufft(u, v, vis_r, vis_i, vis_std, NX, NY, maxiter, eps, lambda_l1, lambda_tv,
                        lambda_tsv, cinit, xinit=None, nonneg=1, cl_box=None, preset=PRESET_C24,
-                       fgp_iter=FGPITER, rel_tol=0.0, trace=False):
+                       fgp_iter=FGPITER, rel_tol=0.0, trace=False, omp=False):
     u, pu = _d(u)
     v, pv = _d(v)
     vr, pvr = _d(vis_r)
@@ -318,7 +330,7 @@ def imaging_core_nufft(u, v, vis_r, vis_i, vis_std, NX, NY, maxiter, eps, lambda
     bx, pbx = _f(cl_box, NX * NY)
     res = Result()
     o, buf = _opts(preset, fgp_iter, rel_tol, maxiter, trace)
-    rc = lib().or_imaging_core_nufft(pu, pv, pvr, pvi, pvs, u.size, NX, NY, maxiter, eps, lambda_l1,
+    rc = lib(omp).or_imaging_core_nufft(pu, pv, pvr, pvi, pvs, u.size, NX, NY, maxiter, eps, lambda_l1,
                                      lambda_tv, lambda_tsv, cinit, px0, xout.ctypes.data_as(_dp),
                                      nonneg, bflag, pbx, C.byref(res), C.byref(o))
     return _finish(rc, res, xout.reshape(NX, NY), o, buf)

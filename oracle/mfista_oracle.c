@@ -12,6 +12,12 @@
  * unnormalised mixed-radix DFT with FFTW's sign and half-plane conventions,
  * and plain loops for dcopy/daxpy/ddot/dasum/dgemv.
  *
+ * Built twice: serially (liboracle_mfista.so, the checker) and with OpenMP
+ * (liboracle_mfista_omp.so, the all-cores CPU baseline, as the C++ reference
+ * is built with -fopenmp, c++/makefile:23-24): the 2-D FFT passes, the
+ * degridding and the 12-tap gridding (per-thread grids summed afterwards) then
+ * run in parallel.  The #pragma lines are inert in the serial build.
+ *
  * Deliberate deviations (documented): a tap that would leave the 2N x 2N grid
  * is rejected up front (the C library writes out of bounds, the C++ one warns);
  * the C++ soft_threshold_box bug (threshold ignored when box_flag=1) is not
@@ -24,6 +30,9 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -173,7 +182,12 @@ void or_fft_c2c(int n, double *data, int sign) {
  * k in [0, n1/2], i in [0, n0).  Rows are transformed two at a time. */
 void or_fft2_r2c(int n0, int n1, const double *in, double *out) {
   const int h = n1 / 2 + 1;
+  (void)fft_plan(n0); /* plans are made here, before any parallel loop reads them */
+  (void)fft_plan(n1);
+#pragma omp parallel
+  {
   double *z = dvec(2 * (size_t)n1);
+#pragma omp for schedule(static)
   for (int i = 0; i < n0; i += 2) {
     const int two = (i + 1 < n0);
     for (int j = 0; j < n1; j++) {
@@ -195,6 +209,7 @@ void or_fft2_r2c(int n0, int n1, const double *in, double *out) {
   }
   free(z);
   double *col = dvec(2 * (size_t)n0);
+#pragma omp for schedule(static)
   for (int k = 0; k < h; k++) {
     for (int i = 0; i < n0; i++) {
       col[2 * i] = out[2 * ((size_t)i * h + k)];
@@ -207,6 +222,7 @@ void or_fft2_r2c(int n0, int n1, const double *in, double *out) {
     }
   }
   free(col);
+  }
 }
 
 /* 2-D half-complex-to-real, FFTW c2r semantics: backward c2c along dimension 0
@@ -216,7 +232,12 @@ void or_fft2_c2r(int n0, int n1, const double *in, double *out) {
   const int h = n1 / 2 + 1;
   double *t = dvec(2 * (size_t)n0 * h);
   memcpy(t, in, sizeof(double) * 2 * (size_t)n0 * h);
+  (void)fft_plan(n0);
+  (void)fft_plan(n1);
+#pragma omp parallel
+  {
   double *col = dvec(2 * (size_t)n0);
+#pragma omp for schedule(static)
   for (int k = 0; k < h; k++) {
     for (int i = 0; i < n0; i++) {
       col[2 * i] = t[2 * ((size_t)i * h + k)];
@@ -230,6 +251,7 @@ void or_fft2_c2r(int n0, int n1, const double *in, double *out) {
   }
   free(col);
   double *z = dvec(2 * (size_t)n1);
+#pragma omp for schedule(static)
   for (int i = 0; i < n0; i += 2) {
     const int two = (i + 1 < n0);
     const double *a = t + 2 * (size_t)i * h;
@@ -258,6 +280,7 @@ void or_fft2_c2r(int n0, int n1, const double *in, double *out) {
     }
   }
   free(z);
+  }
   free(t);
 }
 
@@ -471,6 +494,8 @@ typedef struct or_nufft {
   double *rgrid; /* 2Nx x 2Ny real */
   double *cgrid; /* 2Nx x (Ny+1) complex */
   double *mbuf;  /* C++ preset: padded spectrum buffer */
+  double *priv;  /* OpenMP build: per-thread gridding buffers (npriv of them) */
+  int npriv;
 } or_nufft;
 
 /* c/mfista_nufft_lib.c:32-40 */
@@ -486,6 +511,7 @@ static int m2mr(int id, int N) { return (id < N / 2) ? id + 3 * N / 2 : id - N /
 static void nufft_free(or_nufft *P) {
   free(P->E1); free(P->E2x); free(P->E2y); free(P->E3x); free(P->E3y); free(P->E4);
   free(P->mx); free(P->my); free(P->yneg); free(P->rgrid); free(P->cgrid); free(P->mbuf);
+  free(P->priv);
   memset(P, 0, sizeof(*P));
 }
 
@@ -594,6 +620,7 @@ static void nufft_fwd(or_nufft *P, const double *x, double *yr, double *yi) {
             Mh = Ny + 1, S = 2 * MSP;
   const double MM = (double)Mrx * Mry;
   memset(P->rgrid, 0, sizeof(double) * (size_t)Mrx * Mry);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < Nx; ++i) {
     int ix = m2mr(i, Nx);
     for (int j = 0; j < Ny; ++j)
@@ -602,6 +629,7 @@ static void nufft_fwd(or_nufft *P, const double *x, double *yr, double *yi) {
   or_fft2_r2c(Mrx, Mry, P->rgrid, P->cgrid);
   const double *G = P->cgrid;
   if (P->preset == OR_PRESET_C24) {
+#pragma omp parallel for schedule(static)
     for (int k = 0; k < M; ++k) {
       double fr = 0, fi = 0;
       for (int ly = -MSP + 1; ly <= MSP; ++ly) {
@@ -636,6 +664,7 @@ static void nufft_fwd(or_nufft *P, const double *x, double *yr, double *yi) {
   double *hb = P->mbuf;
   memset(hb, 0, sizeof(double) * 2 * (size_t)R * C);
 #define HB(r, c) (hb + 2 * ((size_t)(r) * C + (c)))
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < Nx; ++i)
     for (int r = 0; r < Mh; ++r) {
       HB(MSP + r, i + Nx + MSP)[0] = G[2 * ((size_t)i * Mh + r)];
@@ -656,6 +685,7 @@ static void nufft_fwd(or_nufft *P, const double *x, double *yr, double *yi) {
     }
   }
   const double MMinv = 1 / (4.0 * (double)Nx * Ny);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < M; ++k) {
     double sr = 0, si = 0;
     for (int a = 0; a < S; a++) {
@@ -723,20 +753,49 @@ static void nufft_adj(or_nufft *P, const double *Fr, const double *Fi, double *x
   memset(lb, 0, sizeof(double) * 2 * (size_t)R * C);
 #define LB(r, c) (lb + 2 * ((size_t)(r) * C + (c)))
   const double MMinv = 1 / (2.0 * (4.0 * (double)Nx * Ny));
-  for (int k = 0; k < M; ++k) {
-    double vr = Fr[k], vi = -Fi[k]; /* NU_SIGN = -1: conj */
-    if (P->yneg[k]) vi = -vi;
-    double v0r = MMinv * P->E1[k] * vr, v0i = MMinv * P->E1[k] * vi;
-    for (int a = 0; a < S; a++) {
-      double gx = P->E2x[(size_t)S * k + a];
-      for (int b = 0; b < S; b++) {
-        double g = gx * P->E2y[(size_t)S * k + b];
-        double *d = LB(P->mx[k] + 1 + Nx + a, P->my[k] + 1 + b);
-        d[0] += v0r * g;
-        d[1] += v0i * g;
+#define SCATTER(buf, k)                                                       \
+  do {                                                                        \
+    double vr = Fr[k], vi = -Fi[k]; /* NU_SIGN = -1: conj */                  \
+    if (P->yneg[k]) vi = -vi;                                                 \
+    double v0r = MMinv * P->E1[k] * vr, v0i = MMinv * P->E1[k] * vi;          \
+    for (int a = 0; a < S; a++) {                                             \
+      double gx = P->E2x[(size_t)S * (k) + a];                                \
+      for (int b = 0; b < S; b++) {                                           \
+        double g = gx * P->E2y[(size_t)S * (k) + b];                          \
+        double *d = (buf) + 2 * ((size_t)(P->mx[k] + 1 + Nx + a) * C + (P->my[k] + 1 + b)); \
+        d[0] += v0r * g;                                                      \
+        d[1] += v0i * g;                                                      \
+      }                                                                       \
+    }                                                                         \
+  } while (0)
+#ifdef _OPENMP
+  const int nthr = omp_get_max_threads();
+  if (nthr > 1) { /* one grid per thread, summed in thread order afterwards */
+    const size_t nb = 2 * (size_t)R * C;
+    if (!P->priv || P->npriv < nthr) {
+      free(P->priv);
+      P->priv = dvec(nb * (size_t)nthr);
+      P->npriv = nthr;
+    }
+#pragma omp parallel num_threads(nthr)
+    {
+      double *mine = P->priv + nb * (size_t)omp_get_thread_num();
+      memset(mine, 0, sizeof(double) * nb);
+#pragma omp for schedule(static)
+      for (int k = 0; k < M; ++k) SCATTER(mine, k);
+#pragma omp for schedule(static)
+      for (size_t e = 0; e < nb; e++) {
+        double acc = 0;
+        for (int q = 0; q < nthr; q++) acc += P->priv[nb * (size_t)q + e];
+        lb[e] = acc;
       }
     }
+  } else
+#endif
+  {
+    for (int k = 0; k < M; ++k) SCATTER(lb, k);
   }
+#undef SCATTER
   /* fold the conjugate (ky < 0) half onto ky > 0 */
   for (int i = 0; i < Nx; ++i)
     for (int j = 0; j < MSP; ++j) {
@@ -778,6 +837,7 @@ static void nufft_adj(or_nufft *P, const double *Fr, const double *Fi, double *x
   free(ax);
   LB(MSP + Nx, MSP)[0] *= 2.0;
   LB(MSP + Nx, MSP)[1] *= 2.0;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < Nx; ++i)
     for (int r = 0; r < Mh; ++r) {
       H[2 * ((size_t)i * Mh + r)] = LB(i + MSP + Nx, MSP + r)[0];
@@ -787,6 +847,7 @@ static void nufft_adj(or_nufft *P, const double *Fr, const double *Fi, double *x
     }
 #undef LB
   or_fft2_c2r(Mrx, Mry, H, P->rgrid);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < Nx; ++k) {
     int idx = m2mr(k, Nx);
     for (int j = 0; j < Ny; ++j)
