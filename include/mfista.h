@@ -248,6 +248,32 @@ int mfista_imaging_core_dft_ex(const double *y, const double *A, int M, int N, i
                                int nonneg_flag, int box_flag, const float *cl_box,
                                struct RESULT *mfista_result, mfista_opts *opts);
 
+/* DFT-mode products for K right-hand sides that share A, on the FP64 matrix
+ * cores: R_k = y - A x_k (y NULL: zero) and, if G is not NULL, G_k = A^T R_k.
+ * The reference computes one dgemv per vector (c/mfista_dft_lib.c:30-48,
+ * calc_yAz / dL_dx); with many vectors the products are GEMMs.  A is M x N,
+ * column-major; X holds K vectors of N (X[k N + n]), R K vectors of M, G K
+ * vectors of N; host pointers; 1 <= K <= 64.  Returns a status. */
+int mfista_dft_products(const double *A, int M, int N, int K, const double *X, const double *y,
+                        double *R, double *G);
+
+/* nprob DFT-mode problems that share y and A and differ in lambda_l1,
+ * lambda_tsv and cinit (a regularisation grid): each is solved exactly as
+ * mfista_imaging_core_dft_ex would solve it (the loop of
+ * c/mfista_dft_lib.c:282-436, L1 or L1+TSV, eps unscaled, opts->rel_tol
+ * honoured), but up to 64 problems advance together and their products with A
+ * are GEMMs on the FP64 matrix cores (mfista_dft_products).  xinit: NULL
+ * (zeros) or nprob images of N = NX NY values (xinit[k N + n]); xout: nprob
+ * images; results[k] as mfista_imaging_core_dft fills it (residual malloc'ed
+ * with M doubles, owned by the caller).  One cl_box (box_flag = 1) applies to
+ * every problem.  TV is not offered here (lambda_tv is not an argument).
+ * No counterpart in the reference, which solves one problem per call. */
+int mfista_imaging_core_dft_batch(const double *y, const double *A, int M, int N, int NX, int NY,
+                                  int nprob, const double *lambda_l1, const double *lambda_tsv,
+                                  const double *cinit, int maxiter, double eps, const double *xinit,
+                                  double *xout, int nonneg_flag, int box_flag, const float *cl_box,
+                                  struct RESULT *results, mfista_opts *opts);
+
 /* the LOOE of compute_LOOE_L1_TSV (lambda_tsv = 0: compute_LOOE_L1) with a
  * status return: *hessian_positive = 1 and *looe_m, *looe_std set, or 0 (both
  * 0) when the Hessian is not positive definite; *n_active (may be NULL)

@@ -44,6 +44,7 @@ EXPORTED = [
     "mfista_imaging_core_fft_ex", "mfista_imaging_core_dft_ex", "mfista_imaging_core_nufft_batch",
     "mfista_movie_nufft", "mfista_looe_dft", "compute_LOOE_L1", "compute_LOOE_L1_TSV",
     "mfista_nufft_solve_device_noalloc", "mfista_device_allocations",
+    "mfista_dft_products", "mfista_imaging_core_dft_batch",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
@@ -160,6 +161,11 @@ def _declare(L):
     L.mfista_imaging_core_dft_ex.restype = i
     L.mfista_imaging_core_dft_ex.argtypes = [_dp, _dp, i, i, i, i, i, d, d, d, d, d, _dp, _dp, i, i, _fp,
                                              C.POINTER(RESULT), C.POINTER(Opts)]
+    L.mfista_dft_products.restype = i
+    L.mfista_dft_products.argtypes = [_dp, i, i, i, _dp, _dp, _dp, _dp]
+    L.mfista_imaging_core_dft_batch.restype = i
+    L.mfista_imaging_core_dft_batch.argtypes = [_dp, _dp, i, i, i, i, i, _dp, _dp, _dp, i, d, _dp, _dp, i, i,
+                                                _fp, C.POINTER(RESULT), C.POINTER(Opts)]
     L.mfista_looe_dft.restype = i
     L.mfista_looe_dft.argtypes = [i, i, i, i, d, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
     L.compute_LOOE_L1.restype = d
@@ -391,6 +397,58 @@ def mfista_imaging_core_dft(y, A, NX, NY, maxiter=MAXITER, eps=EPS, lambda_l1=1.
     if res.residual:
         C.CDLL(None).free(C.cast(res.residual, C.c_void_p))
     return xout.reshape(NX, NY), res.as_dict(), _info(o), resid
+
+
+def dft_products(A, X, y=None, adjoint=True):
+    """R_k = y - A x_k and G_k = A^T R_k for the K columns of X (N x K) on the
+    FP64 matrix cores.  Returns (R: M x K, G: N x K or None)."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    M, N = A.shape
+    X = np.asarray(X, dtype=np.float64)
+    K = X.shape[1]
+    Xk = np.ascontiguousarray(X.T)
+    yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    R = np.zeros((K, M))
+    G = np.zeros((K, N)) if adjoint else None
+    rc = lib().mfista_dft_products(A.ctypes.data_as(_dp), M, N, K, Xk.ctypes.data_as(_dp),
+                                   None if yy is None else yy.ctypes.data_as(_dp), R.ctypes.data_as(_dp),
+                                   None if G is None else G.ctypes.data_as(_dp))
+    if rc != 0:
+        raise MfistaError(rc)
+    return R.T.copy(), (None if G is None else G.T.copy())
+
+
+def mfista_imaging_core_dft_batch(y, A, NX, NY, lambda_l1, lambda_tsv, cinit=1.0, maxiter=MAXITER, eps=EPS,
+                                  xinit=None, nonneg_flag=1, cl_box=None, rel_tol=0.0):
+    """Many DFT-mode L1(+TSV) problems sharing y and A (one per entry of
+    lambda_l1 / lambda_tsv), solved in lockstep with the products on the FP64
+    matrix cores.  Returns (list of x[NX,NY], list of RESULT dicts, list of
+    residuals, info)."""
+    A = np.asfortranarray(np.asarray(A, dtype=np.float64))
+    M, N = A.shape
+    l1 = np.ascontiguousarray(np.atleast_1d(lambda_l1), dtype=np.float64)
+    ltsv = np.ascontiguousarray(np.broadcast_to(lambda_tsv, l1.shape), dtype=np.float64)
+    cin = np.ascontiguousarray(np.broadcast_to(cinit, l1.shape), dtype=np.float64)
+    n = l1.size
+    y, py = _d(y)
+    x0 = None if xinit is None else np.ascontiguousarray(np.broadcast_to(np.asarray(xinit, np.float64).reshape(-1, N),
+                                                                         (n, N)))
+    xout = np.zeros((n, N))
+    bflag, b, pb = _box(cl_box, N)
+    res = (RESULT * n)()
+    o = make_opts(PRESET_C24, FGPITER, rel_tol)
+    rc = lib().mfista_imaging_core_dft_batch(py, A.ctypes.data_as(_dp), M, N, NX, NY, n, l1.ctypes.data_as(_dp),
+                                             ltsv.ctypes.data_as(_dp), cin.ctypes.data_as(_dp), int(maxiter), eps,
+                                             None if x0 is None else x0.ctypes.data_as(_dp),
+                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb, res, C.byref(o))
+    if rc != 0:
+        raise MfistaError(rc)
+    resid = []
+    for r in res:
+        resid.append(np.ctypeslib.as_array(r.residual, shape=(M,)).copy() if r.residual else None)
+        if r.residual:
+            C.CDLL(None).free(C.cast(r.residual, C.c_void_p))
+    return [x.reshape(NX, NY) for x in xout], [r.as_dict() for r in res], resid, _info(o)
 
 
 def compute_looe(A, x, yAx, NX=1, NY=1, lambda_tsv=0.0):

@@ -77,6 +77,11 @@ struct BatchSet {
   double vis_power = 0;  // sum |V|^2 (eps scaling)
 };
 
+// Device workspace of the DFT-mode multi-problem solver (dft_batch.hip).
+struct DftBatchBufs {
+  DevMem A, y, X, Z, XN, D, G, Rx, Rz, Rn, part, psum, psq_n, psq_z, S, cost, box;
+};
+
 struct Context {
   int device = -1;
   bool no_alloc = false;  // the allocation-free entry point: workspaces from its arena, no events
@@ -98,6 +103,7 @@ struct Context {
   DevMem movie_part;             // partial-sum table shared by the frames of a coupled movie
   DevMem trace, trace_tag;       // persistent-engine barrier trace (MFISTA_PERSISTENT_TRACE)
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
+  DftBatchBufs dbb;    // DFT mode, many problems sharing one matrix
 
   int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
