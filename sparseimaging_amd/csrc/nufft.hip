@@ -16,8 +16,6 @@
 //                 add per touched cell
 //   k_colfft_inv  c2c backward along x per ky -> U[i][ky]
 //   k_rowfft_c2r  c2r along y (FFTW semantics), crop, * E4 / (8 NX NY)
-#include <hipcub/hipcub.hpp>
-
 #include "nufft.h"
 #include "stages.h"
 
@@ -347,20 +345,6 @@ void launch_tile_ranges(const int* keys_sorted, int n, int ntiles, int* start, i
                         hipStream_t st) {
   hipLaunchKernelGGL(k_tile_ranges, dim3((n + 255) / 256), dim3(256), 0, st, keys_sorted, n,
                      ntiles, start, end);
-}
-
-// radix sort of (key, value) pairs; temp storage sized by the first call
-int sort_pairs(void* temp, size_t* temp_bytes, const int* kin, int* kout, const int* vin, int* vout,
-               int n, int end_bit, hipStream_t st) {
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, n, 0,
-                                                    end_bit, st);
-  return e == hipSuccess ? 0 : -1;
-}
-int sort_pairs_u(void* temp, size_t* temp_bytes, const int* kin, int* kout, const unsigned* vin,
-                 unsigned* vout, int n, int end_bit, hipStream_t st) {
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, kin, kout, vin, vout, n, 0,
-                                                    end_bit, st);
-  return e == hipSuccess ? 0 : -1;
 }
 
 #define MF_INST(R)                                                                               \

@@ -4,7 +4,7 @@
 // per-process context owns one HIP stream and grow-only device workspaces,
 // so repeated solves of the same shape allocate nothing.  The NUFFT set-up
 // (reference preNUFFT, c/mfista_nufft_lib.c:50-121) runs on the GPU: cell
-// indices and tap slopes per visibility, a radix sort by gridding tile for
+// indices and tap slopes per visibility, a stable sort by gridding tile for
 // locality, and the per-tile visibility lists of the LDS gridding kernel.
 #include <algorithm>
 #include <chrono>

@@ -262,6 +262,47 @@ __device__ inline void st_rowfft_c2r(int task, int nseq, const cplx<R>* __restri
   __syncthreads();
 }
 
+// st_rowfft_c2r (one sequence) with two hooks around its input loads:
+// issue() runs before them and commit() after them, before the barrier that
+// publishes the LDS input, so a caller's own independent loads (issued into
+// registers, stored to LDS in commit()) are in flight together with these.
+template <typename R, typename Store, typename Issue, typename Commit>
+__device__ inline void st_rowfft_c2r_pf(int task, const cplx<R>* __restrict__ U, int NX, int NY, int Ly, int shift,
+                                        const FFTPlan& P, const cplx<R>* __restrict__ tw, cplx<R>* sm, Store store,
+                                        Issue issue, Commit commit) {
+  using C = cplx<R>;
+  C* b0 = sm;
+  C* b1 = sm + Ly;
+  const int row0 = task * 2;
+  const int Lh = Ly / 2, nky = Ly / 2 + 1;
+  issue();
+  for (int t = threadIdx.x; t < Ly; t += blockDim.x) {
+    const int b = t;
+    const int i0 = row0, i1 = i0 + 1;
+    const int k = (b <= Lh) ? b : Ly - b;
+    C A = (i0 < NX) ? U[(size_t)i0 * nky + k] : mk<R>(0, 0);
+    C B = (i1 < NX) ? U[(size_t)i1 * nky + k] : mk<R>(0, 0);
+    if (k == 0 || k == Lh) {
+      A.y = 0;
+      B.y = 0;
+    }
+    b0[t] = (b <= Lh) ? mk<R>(A.x - B.y, A.y + B.x) : mk<R>(A.x + B.y, -A.y + B.x);
+  }
+  commit();
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 1, P, tw, +1);
+  for (int t = threadIdx.x; t < 2 * NY; t += blockDim.x) {
+    const int r = t / NY, j = t - r * NY;
+    const int i = row0 + r;
+    if (i >= NX) continue;
+    int b = j - shift;
+    if (b < 0) b += Ly;
+    const C z = res[b];
+    store(i, j, (r & 1) ? z.y : z.x);
+  }
+  __syncthreads();
+}
+
 // Tap gather of the visibilities of one task, G = 16 (MSP 6) or 32 (MSP 12)
 // lanes per visibility and blockDim / G visibilities per task; lane l owns tap
 // column kx = mx + l.  The tap count 2 MSP is a compile-time constant so the
