@@ -67,7 +67,15 @@ def test_nufft_signed_l1_tsv(preset, engine):
 @pytest.mark.parametrize("case", ["fgp_l1", "box"])
 def test_nufft_tv_signed_and_box(case, engine):
     """L1+TV (24-tap preset, FGP 50 iterations): nonneg_flag = 0 runs FGP_L1;
-    nonneg_flag = 1 with a CLEAN box runs FGP_nonneg_box."""
+    nonneg_flag = 1 with a CLEAN box runs FGP_nonneg_box.  The signed TV
+    problem converges slowly (a stopping rule ends the GPU and the oracle
+    solves at different points whose images differ by ~2e-3), so both run a
+    fixed 20000 iterations; the oracle's solutions are the fixtures of
+    tests/golden/tv_oracle.npz (make_tv_reference.py, same inputs)."""
+    import hashlib
+    import os
+
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "tv_oracle.npz"))
     if case == "fgp_l1":
         P = _signed_problem(64, 2500, 72)
         nonneg, box = 0, None
@@ -77,17 +85,18 @@ def test_nufft_tv_signed_and_box(case, engine):
         box = np.ones((64, 64), np.float32)
         box[:, :12] = 0.0
         box[50:, :] = 0.0
-    kw = dict(maxiter=3000, eps=0.0, lambda_l1=0.5, lambda_tv=2.0, lambda_tsv=0.0, cinit=1e4, rel_tol=1e-8)
-    print(f"case nufft L1+TV {case} engine={engine} 64x64 M=2500 fgp=50", flush=True)
+    h = hashlib.sha256()
+    for k in ("u", "v", "vis_r", "vis_i", "std"):
+        h.update(np.ascontiguousarray(P[k], dtype=np.float64).tobytes())
+    assert h.hexdigest() == str(ref[f"{case}_sha256"])
+    kw = dict(maxiter=20000, eps=0.0, lambda_l1=0.5, lambda_tv=2.0, lambda_tsv=0.0, cinit=1e4, rel_tol=0.0)
+    print(f"case nufft L1+TV {case} engine={engine} 64x64 M=2500 fgp=50, 20000 iterations", flush=True)
     xg, rg, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
                                              nonneg_flag=nonneg, cl_box=box, preset=sa.PRESET_C24, fgp_iter=50,
                                              engine=engine, **kw)
-    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"], 0.0,
-                                  kw["lambda_l1"], kw["lambda_tv"], 0.0, kw["cinit"], nonneg=nonneg,
-                                  cl_box=None if box is None else box.ravel(), preset=oracle.PRESET_C24,
-                                  fgp_iter=50, rel_tol=kw["rel_tol"])
+    o = {"x": ref[f"{case}_x"], "result": {"finalcost": float(ref[f"{case}_F"]), "ITER": 20000}}
     if box is not None:
-        assert np.all(xg[box == 0] == 0) and np.all(o["x"][box == 0] == 0)
+        assert np.all(xg[box == 0] == 0)
     else:
         assert (o["x"] < 0).any() and (xg < 0).any()
     _cmp(f"TV {case}", xg, rg, o, 1e-3, 1e-6)
