@@ -7,13 +7,19 @@ Checked against the FP64 oracle's solutions in tests/golden/cfg2_oracle.npz
 c/mfista_nufft_lib.c:557-787 with the C++ preset of
 c++/mfista_nufft_lib.cpp:85-184).  Parity bar: image relative L2 <= 1e-3 and
 objective relative error <= 1e-6, at three end points:
-  * the benchmark's rule, cost[t-50] - cost[t] <= 1e-6 cost[t]: the GPU's and
-    the oracle's stopping points;
+  * the benchmark's rule, cost[t-50] - cost[t] <= 1e-6 cost[t];
   * 4000 iterations;
   * 20000 iterations (converged).
-At the rule's stop both are still about 4e-5 above the converged objective
-(the rule measures progress, not distance to the optimum); the test states
-that gap too.
+At the rule's stop both solvers are still about 4e-5 above the converged
+objective: the rule measures progress, not distance to the optimum, and near
+the stop the iterate still moves by about 3e-4 (relative L2) per iteration.
+The GPU and the oracle can therefore stop a few iterations apart (575 against
+580 seen), and their images then differ by that motion, not by arithmetic:
+the oracle's own iterates 575 and 580 differ by 1.5e-3.  So at the rule the
+image bar is applied to the oracle's iterate at the GPU's stopping count (the
+serial oracle, run here for that many iterations), and the objective bar to
+both that iterate and the oracle's own rule stop.  The image distance between
+the two rule stops is printed and held to 1e-2 (about 30 iterations apart).
 """
 import hashlib
 import os
@@ -21,6 +27,7 @@ import os
 import numpy as np
 import pytest
 
+import oracle
 import sparseimaging_amd as sa
 from sparseimaging_amd import synth
 
@@ -56,5 +63,18 @@ def test_headline_parity(problem, end):
     print(f"  ITER gpu {r['ITER']} oracle {int(REF[f'ITER_{end}'])}; image relL2 {l2:.3e}; objective rel "
           f"{dF:.3e} (gpu {r['finalcost']:.10e} oracle {Fo:.10e}); gap to the converged objective {gap:.2e}; "
           f"{info['kernel_ms']:.1f} ms", flush=True)
-    assert l2 <= 1e-3
     assert dF <= 1e-6
+    if end != "rule":
+        assert l2 <= 1e-3
+        return
+    assert l2 <= 1e-2
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 256, 256, r["ITER"], 0.0,
+                                  KW["lambda_l1"], 0.0, KW["lambda_tsv"], KW["cinit"], nonneg=1,
+                                  preset=oracle.PRESET_CXX12, rel_tol=0.0)
+    xs, Fs = o["x"], o["result"]["finalcost"]
+    l2s = float(np.linalg.norm(x - xs) / np.linalg.norm(xs))
+    dFs = abs(r["finalcost"] - Fs) / Fs
+    print(f"  oracle at the GPU's {r['ITER']} iterations: image relL2 {l2s:.3e}; objective rel {dFs:.3e}",
+          flush=True)
+    assert l2s <= 1e-3
+    assert dFs <= 1e-6
