@@ -151,10 +151,20 @@ def _batch_vs_single(sa, probs, preset, rel_tol):
         worst = max(worst, abs(rb["finalcost"] - r1["finalcost"]) / abs(r1["finalcost"]))
         iters += rb["ITER"]
     ts = time.perf_counter() - t0
+    # the same batch on the wide engine (every stage one launch for all problems)
+    sa.mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol, engine=sa.ENGINE_STREAM)  # warm-up
+    t0 = time.perf_counter()
+    _, rw, iw = sa.mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol, engine=sa.ENGINE_STREAM)
+    tw = time.perf_counter() - t0
+    worst_w = max(abs(a["finalcost"] - b["finalcost"]) / abs(b["finalcost"]) for a, b in zip(rw, rs))
     n = len(probs)
     return {"solves": n, "batch_solves_per_s": round(n / tb, 3), "single_solves_per_s": round(n / ts, 3),
             "speedup": round(ts / tb, 2), "batch_ms": round(tb * 1e3, 2), "kernel_ms": round(info["kernel_ms"], 2),
-            "iterations": iters, "max_objective_rel_diff": worst}
+            "iterations": iters, "max_objective_rel_diff": worst,
+            "wide": {"solves_per_s": round(n / tw, 3), "speedup_vs_single": round(ts / tw, 2),
+                     "batch_ms": round(tw * 1e3, 2), "kernel_ms": round(iw["kernel_ms"], 2),
+                     "engine_used": iw["engine_used"], "iterations": int(sum(r["ITER"] for r in rw)),
+                     "max_objective_rel_diff_vs_persistent_batch": worst_w}}
 
 
 def bench_lambda_grid(sa, synth):
@@ -352,6 +362,61 @@ def cpu_baseline(P, N, cfg, runs, gpu_iter):
                               "sample": "one solve of the serial build for the GPU solve's iteration count"}}
 
 
+def stage_table(sa, N, M, MSP, reps):
+    """Per-launch duration and achieved bandwidth of each stream-engine kernel
+    on the NUFFT set-up of the last solve (HIP events, `reps` launches each)."""
+    prof = sa.profile_stages(reps)
+    nbytes = stage_bytes(N, M, MSP, int(1.6 * M))
+    return {k: {"us": round(prof[k] * 1e3, 2), "bytes": nbytes[k],
+                "GBps": round(nbytes[k] / (prof[k] * 1e-3) / 1e9, 1),
+                "frac_of_hbm_peak": round(nbytes[k] / (prof[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k in prof}
+
+
+def workload_tv512(sa, synth, args):
+    """The 512^2 L1+TV workload: 20 Gaussians + 2 rings (seed 1003), M = 200000
+    points uniform in a disc of radius 0.48 cycles/px with density ~ 1/r, 24-tap
+    preset, FGP 50 iterations, lambda_l1 = 0.5, lambda_tv = 5, nonneg, 500
+    MFISTA iterations (no stopping rule), cinit = 1e4."""
+    import torch
+
+    P = synth.make_nufft_problem(N=512, M=200000, seed=1003, kind="tv", coverage="disc", amax=0.48)
+    kw = dict(maxiter=500, eps=0.0, lambda_l1=0.5, lambda_tv=5.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
+              preset=sa.PRESET_C24, fgp_iter=50)
+    for _ in range(args.warmup):
+        sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x, r, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512, **kw)
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"workload": "nufft_l1tv_512_m200000_24tap_fgp50", "ms_per_solve": round(dt * 1e3, 2),
+            "kernel_ms": round(info["kernel_ms"], 3), "iterations": r["ITER"], "trials": info["total_trials"],
+            "us_per_iteration": round(info["kernel_ms"] * 1e3 / r["ITER"], 2), "finalcost": r["finalcost"],
+            "stream_engine_kernels": stage_table(sa, 512, 200000, 12, args.profile_reps)}
+
+
+def workload_lambda128(sa, synth, args):
+    """The 6 x 6 lambda grid at 128^2 (M = 8000 arcs, seed 1004) as one batch
+    launch, 12-tap preset, 1e-6 relative tolerance (bench_lambda_grid's
+    problems)."""
+    import numpy as np
+
+    P = synth.make_nufft_problem(N=128, M=8000, seed=1004)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=128, NY=128,
+                maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0)
+    probs = [dict(base, lambda_l1=10.0 ** a, lambda_tsv=10.0 ** b)
+             for a in np.linspace(-2, 3, 6) for b in np.linspace(0, 5, 6)]
+    for _ in range(args.warmup):
+        sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"workload": "lambda_grid_6x6_128_m8000_12tap", "ms_per_batch": round(dt * 1e3, 2),
+            "solves_per_s": round(len(probs) / dt, 3), "kernel_ms": round(info["kernel_ms"], 3),
+            "iterations": [r["ITER"] for r in rs]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +431,9 @@ def main():
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--warm", type=int, default=8,
                     help="held-out 256^2 problems for the learned warm start (0 = skip)")
+    ap.add_argument("--workload", choices=["headline", "tv512", "lambda128"], default="headline",
+                    help="headline: the bench line; tv512 / lambda128: time only the 512^2 L1+TV solve or the "
+                         "6x6 lambda-grid batch (profiling runs) and print their own JSON line")
     ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
                     help="also time the batched workloads (6x6 lambda grid at 128^2, 8 lambda_tsv values "
                          "at 256^2, 32-frame movie) against the same problems solved one by one")
@@ -381,6 +449,11 @@ def main():
 
     import sparseimaging_amd as sa
     from sparseimaging_amd import synth
+
+    if args.workload != "headline":
+        fn = workload_tv512 if args.workload == "tv512" else workload_lambda128
+        print(json.dumps(fn(sa, synth, args)), flush=True)
+        return
 
     N, M, seed = 256, 20000, 1002
     cfg = dict(maxiter=50000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=1e4,

@@ -149,6 +149,10 @@ typedef struct mfista_opts {
   int deterministic; /* in: nonzero = reproducible NUFFT solves (persistent engine): every
                         gridding tile is summed by one workgroup in a fixed order, with no
                         floating-point atomics, so repeated solves are bit-identical */
+  int autotune;      /* in: nonzero = NUFFT solves and batches (persistent engine) use the
+                        launch configuration the autotuner chose for their shape, tuning
+                        first (on the problem itself) when the table has none */
+  int tune_iters;    /* in: iterations per autotuner candidate (default 30) */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);
@@ -199,7 +203,12 @@ typedef struct {
  * or the status of the first invalid problem (then nothing is solved); a
  * problem with nonneg_flag outside {0, 1} gets ITER = 0, as in the reference
  * (c/mfista_nufft_lib.c:634-637).  opts->solve_ms and opts->kernel_ms report
- * the whole batch. */
+ * the whole batch.  Engines (opts->engine): MFISTA_ENGINE_PERSISTENT (default)
+ * runs the batch in one cooperative launch, workgroups dealt to problems;
+ * MFISTA_ENGINE_STREAM runs the wide engine when every problem is L1 or
+ * L1+TSV of one image size (otherwise the persistent launch): the stream
+ * engine's loop with every kernel launched once for all problems, one step
+ * captured as a hipGraph; opts->engine_used reports which ran. */
 int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem *probs,
                                     struct RESULT *results, mfista_opts *opts);
 
@@ -327,6 +336,59 @@ int mfista_nufft_solve_device_noalloc(const double *d_u, const double *d_v, cons
 /* Diagnostic: the number of device allocations (hipMalloc) the library has
  * made since it was loaded. */
 long mfista_device_allocations(void);
+
+/* ---- launch-configuration autotuner --------------------------------------
+ * No counterpart in the reference, whose schedule is fixed (its nearest
+ * mechanism is configure_tile, c++/mfista_nufft_lib.cpp:643-678, which
+ * balances gridding taps over OpenMP threads by rule).  The persistent
+ * solver's gridding tile (16x16, 32x16 or 32x32 cells), workgroups per CU,
+ * paired backtracking rounds and, for batches, the engine are timed on the
+ * problem itself: every candidate
+ * solves it for `iters` iterations from its xinit (no stopping rule), twice,
+ * and the fastest per iteration is kept in a process-wide table under the
+ * shape key (NX, NY, preset, regulariser class, floor(log2 M), floor(log2
+ * problems)).  Solves and batches with opts->autotune set use the table's
+ * entry for their shape and tune on a miss.  MFISTA_TUNE_CACHE=<file> loads
+ * a table saved earlier when the library starts and rewrites the file after
+ * every new entry. */
+typedef struct mfista_tune {
+  int NX, NY, preset; /* key: image size and gridding preset */
+  int reg;            /* key: 0 L1, 1 L1+TSV, 2 L1+TV */
+  int m_log2;         /* key: floor(log2 M) (the largest M of a batch) */
+  int nprob_log2;     /* key: floor(log2 problems): 0 for a single solve */
+  int tile_x, tile_y; /* gridding tile, cells along kx and ky */
+  int blocks_per_cu;  /* workgroups per CU of the cooperative launch */
+  int paired;         /* paired backtracking rounds (L1 and L1+TSV) */
+  int engine;         /* batches: MFISTA_ENGINE_PERSISTENT (one cooperative launch) or
+                         MFISTA_ENGINE_STREAM (the wide engine: every stage one launch for
+                         all problems, a step replayed as a graph) */
+  double us_per_iter; /* measured microseconds per iteration */
+} mfista_tune;
+
+/* Tunes the shape of this problem (arguments as mfista_imaging_core_nufft_ex,
+ * without maxiter, eps, xout and the box; opts: preset, fgp_iter,
+ * deterministic; NULL = defaults) and stores the choice.  iters <= 0: 30.
+ * *best receives the choice; all (may be NULL) up to max_all candidates with
+ * their times, *n_all (may be NULL) their number.  Returns a status. */
+int mfista_autotune_nufft(const double *u_dx, const double *v_dy, const double *vis_r,
+                          const double *vis_i, const double *vis_std, int M, int NX, int NY,
+                          double lambda_l1, double lambda_tv, double lambda_tsv, double cinit,
+                          const double *xinit, int nonneg_flag, const mfista_opts *opts, int iters,
+                          mfista_tune *best, mfista_tune *all, int max_all, int *n_all);
+/* The same for a batch (mfista_imaging_core_nufft_batch's problems; every
+ * problem runs `iters` iterations; xout is not written). */
+int mfista_autotune_nufft_batch(int nprob, const mfista_nufft_problem *probs,
+                                const mfista_opts *opts, int iters, mfista_tune *best,
+                                mfista_tune *all, int max_all, int *n_all);
+/* the table: entries copied to out (up to cap); returns their number */
+int mfista_tune_table(mfista_tune *out, int cap);
+/* writes the table to a text file / merges a file into it (returns the number
+ * of entries read); status < 0 on error */
+int mfista_tune_save(const char *path);
+int mfista_tune_load(const char *path);
+void mfista_tune_clear(void);
+/* candidate solves the autotuner has timed in this process */
+long mfista_tune_runs(void);
 
 /* Releases the library's per-device state: waits for its work, frees its
  * device and pinned host memory, destroys its streams and events.  Optional;

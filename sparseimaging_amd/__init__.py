@@ -45,6 +45,8 @@ EXPORTED = [
     "mfista_movie_nufft", "mfista_looe_dft", "compute_LOOE_L1", "compute_LOOE_L1_TSV",
     "mfista_nufft_solve_device_noalloc", "mfista_device_allocations",
     "mfista_dft_products", "mfista_imaging_core_dft_batch",
+    "mfista_autotune_nufft", "mfista_autotune_nufft_batch", "mfista_tune_table", "mfista_tune_save",
+    "mfista_tune_load", "mfista_tune_clear", "mfista_tune_runs",
 ]
 STAGES = ["rowfft_fwd", "colfft_fwd", "degrid", "grid", "colfft_inv", "rowfft_c2r", "tsv_grad", "prox"]
 
@@ -80,7 +82,20 @@ class Opts(C.Structure):
         ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
         ("engine", C.c_int), ("kernel_ms", C.c_double), ("trial_rounds", C.c_int),
         ("single_trials", C.c_int), ("engine_used", C.c_int), ("deterministic", C.c_int),
+        ("autotune", C.c_int), ("tune_iters", C.c_int),
     ]
+
+
+class Tune(C.Structure):
+    """mfista_tune: one entry of the autotuner's table (shape key and choice)."""
+    _fields_ = [
+        ("NX", C.c_int), ("NY", C.c_int), ("preset", C.c_int), ("reg", C.c_int), ("m_log2", C.c_int),
+        ("nprob_log2", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("blocks_per_cu", C.c_int),
+        ("paired", C.c_int), ("engine", C.c_int), ("us_per_iter", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class MfistaError(RuntimeError):
@@ -180,6 +195,22 @@ def _declare(L):
     L.mfista_amd_version.argtypes = []
     L.mfista_amd_finalize.restype = None
     L.mfista_amd_finalize.argtypes = []
+    L.mfista_autotune_nufft.restype = i
+    L.mfista_autotune_nufft.argtypes = [_dp, _dp, _dp, _dp, _dp, i, i, i, d, d, d, d, _dp, i, C.POINTER(Opts), i,
+                                        C.POINTER(Tune), C.POINTER(Tune), i, _ip]
+    L.mfista_autotune_nufft_batch.restype = i
+    L.mfista_autotune_nufft_batch.argtypes = [i, C.POINTER(NufftProblem), C.POINTER(Opts), i, C.POINTER(Tune),
+                                              C.POINTER(Tune), i, _ip]
+    L.mfista_tune_table.restype = i
+    L.mfista_tune_table.argtypes = [C.POINTER(Tune), i]
+    L.mfista_tune_save.restype = i
+    L.mfista_tune_save.argtypes = [C.c_char_p]
+    L.mfista_tune_load.restype = i
+    L.mfista_tune_load.argtypes = [C.c_char_p]
+    L.mfista_tune_clear.restype = None
+    L.mfista_tune_clear.argtypes = []
+    L.mfista_tune_runs.restype = C.c_long
+    L.mfista_tune_runs.argtypes = []
 
 
 def _d(a):
@@ -202,12 +233,14 @@ def _box(cl_box, n):
 
 
 def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT,
-              single_trials=False, deterministic=False):
+              single_trials=False, deterministic=False, autotune=False, tune_iters=30):
     o = Opts()
     lib().mfista_default_opts(C.byref(o))
     o.preset, o.fgp_iter, o.rel_tol, o.verbose, o.engine = preset, fgp_iter, rel_tol, verbose, engine
     o.single_trials = 1 if single_trials else 0
     o.deterministic = 1 if deterministic else 0
+    o.autotune = 1 if autotune else 0
+    o.tune_iters = int(tune_iters)
     return o
 
 
@@ -220,12 +253,15 @@ def _info(o):
 def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter=MAXITER, eps=EPS,
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
                               nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
-                              rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False):
+                              rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False,
+                              autotune=False, tune_iters=30):
     """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts).
     single_trials=True evaluates one backtracking trial per round in the persistent
     engine (default: L1+TSV rounds evaluate the trials c and c*ETA together);
     deterministic=True grids without floating-point atomics (bit-identical
-    repeated solves, persistent engine)."""
+    repeated solves, persistent engine); autotune=True uses the autotuner's
+    launch configuration for this shape (tuned first, with tune_iters
+    iterations per candidate, when the table has none)."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
     vr, pvr = _d(vis_r)
@@ -238,7 +274,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     bflag, b, pb = _box(cl_box, NX * NY)
     res = RESULT()
     o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
-                  deterministic=deterministic)
+                  deterministic=deterministic, autotune=autotune, tune_iters=tune_iters)
     rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
                                             lambda_tv, lambda_tsv, cinit, px0,
                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
@@ -248,15 +284,9 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     return xout.reshape(NX, NY), res.as_dict(), _info(o)
 
 
-def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0,
-                                    single_trials=False, deterministic=False):
-    """Independent NUFFT solves in one persistent launch (no reference counterpart:
-    a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
-    the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
-    vis_std, NX, NY and optionally maxiter, eps, lambda_l1, lambda_tv, lambda_tsv,
-    cinit, xinit, nonneg_flag, cl_box).  Problems given the same numpy arrays for
-    u_dx, v_dy, vis_r, vis_i, vis_std share one NUFFT set-up.  Returns
-    (list of x[NX,NY], list of RESULT dicts, opts dict for the whole batch)."""
+def _pack_problems(problems):
+    """mfista_nufft_problem array of a list of problem dicts; returns (array,
+    buffers to keep alive, output arrays)."""
     n = len(problems)
     arr = (NufftProblem * n)()
     keep = []
@@ -295,13 +325,99 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
         bflag, b, pb = _box(p.get("cl_box"), NX * NY)
         keep.append(b)
         q.box_flag, q.cl_box = bflag, pb
+    return arr, keep, outs
+
+
+def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0,
+                                    single_trials=False, deterministic=False, autotune=False, tune_iters=30,
+                                    engine=ENGINE_PERSISTENT):
+    """Independent NUFFT solves in one persistent launch (no reference counterpart:
+    a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
+    the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
+    vis_std, NX, NY and optionally maxiter, eps, lambda_l1, lambda_tv, lambda_tsv,
+    cinit, xinit, nonneg_flag, cl_box).  Problems given the same numpy arrays for
+    u_dx, v_dy, vis_r, vis_i, vis_std share one NUFFT set-up.  autotune=True:
+    the autotuner's launch configuration for the batch's shape.
+    engine=ENGINE_STREAM: the wide engine (every stage one launch for all
+    problems; L1 and L1+TSV of one image size).  Returns
+    (list of x[NX,NY], list of RESULT dicts, opts dict for the whole batch)."""
+    n = len(problems)
+    arr, keep, outs = _pack_problems(problems)
     res = (RESULT * n)()
-    o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials, deterministic=deterministic)
+    o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials, deterministic=deterministic,
+                  autotune=autotune, tune_iters=tune_iters, engine=engine)
     rc = lib().mfista_imaging_core_nufft_batch(n, arr, res, C.byref(o))
     if rc != 0:
         raise MfistaError(rc)
     xs = [xo.reshape(int(p["NX"]), int(p["NY"])) for xo, p in zip(outs, problems)]
     return xs, [r.as_dict() for r in res], _info(o)
+
+
+# ---------------------------------------------------------------- autotuner
+def autotune_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0,
+                   cinit=1e4, xinit=None, nonneg_flag=1, preset=PRESET_C24, fgp_iter=FGPITER, iters=30,
+                   deterministic=False):
+    """Times the persistent solver's launch configurations on this problem
+    (`iters` iterations each) and stores the fastest for its shape.  Returns
+    (choice dict, list of every candidate's dict with its us_per_iter)."""
+    u, pu = _d(u_dx)
+    v, pv = _d(v_dy)
+    vr, pvr = _d(vis_r)
+    vi, pvi = _d(vis_i)
+    vs, pvs = _d(vis_std)
+    x0 = np.zeros(NX * NY) if xinit is None else np.asarray(xinit, dtype=np.float64).ravel()
+    x0, px0 = _d(x0)
+    o = make_opts(preset, fgp_iter, deterministic=deterministic)
+    best, allc, n = Tune(), (Tune * 32)(), C.c_int(0)
+    rc = lib().mfista_autotune_nufft(pu, pv, pvr, pvi, pvs, u.size, NX, NY, lambda_l1, lambda_tv, lambda_tsv,
+                                     cinit, px0, nonneg_flag, C.byref(o), iters, C.byref(best), allc, 32,
+                                     C.byref(n))
+    if rc != 0:
+        raise MfistaError(rc)
+    return best.as_dict(), [allc[k].as_dict() for k in range(min(n.value, 32))]
+
+
+def autotune_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, iters=30, deterministic=False):
+    """autotune_nufft for a batch (the problems of mfista_imaging_core_nufft_batch)."""
+    n = len(problems)
+    arr, keep, outs = _pack_problems(problems)
+    o = make_opts(preset, fgp_iter, deterministic=deterministic)
+    best, allc, na = Tune(), (Tune * 32)(), C.c_int(0)
+    rc = lib().mfista_autotune_nufft_batch(n, arr, C.byref(o), iters, C.byref(best), allc, 32, C.byref(na))
+    if rc != 0:
+        raise MfistaError(rc)
+    return best.as_dict(), [allc[k].as_dict() for k in range(min(na.value, 32))]
+
+
+def tune_table():
+    """The autotuner's table: a list of entry dicts."""
+    n = lib().mfista_tune_table(None, 0)
+    buf = (Tune * max(n, 1))()
+    n = lib().mfista_tune_table(buf, n)
+    return [buf[k].as_dict() for k in range(n)]
+
+
+def tune_save(path):
+    rc = lib().mfista_tune_save(os.fsencode(path))
+    if rc < 0:
+        raise MfistaError(rc)
+
+
+def tune_load(path):
+    """Merges a saved table; returns the number of entries read."""
+    rc = lib().mfista_tune_load(os.fsencode(path))
+    if rc < 0:
+        raise MfistaError(rc)
+    return rc
+
+
+def tune_clear():
+    lib().mfista_tune_clear()
+
+
+def tune_runs():
+    """Candidate solves the autotuner has timed in this process."""
+    return int(lib().mfista_tune_runs())
 
 
 def mfista_movie_nufft(vis, u, v, noise_std, time_idx, c_box, Nx, Ny, xinit, lambda_l1, lambda_tsv, lambda_t,
@@ -476,7 +592,7 @@ def compute_looe(A, x, yAx, NX=1, NY=1, lambda_tsv=0.0):
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
                        lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
                        t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None,
-                       engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False):
+                       engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False, autotune=False):
     """HBM-resident solve: every array argument is a device pointer (int) or a
     torch CUDA tensor (float64; cl_box float32).  The solve is queued on
     `stream` (a hipStream_t as int); by default on torch's current stream, so
@@ -497,7 +613,7 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
         stream = torch.cuda.current_stream().cuda_stream
     res = RESULT()
     o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
-                  deterministic=deterministic)
+                  deterministic=deterministic, autotune=autotune)
     rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
                                          maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
                                          ptr(t_xinit), ptr(t_xout), nonneg_flag,

@@ -49,6 +49,11 @@ constexpr int kCoopRefused = MFISTA_E_COOP;
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms);
+// independent solves of one image size and preset with every kernel launched
+// once for all problems, one step captured as a graph (wide.hip); L1 and
+// L1+TSV only
+int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                     std::vector<RunStats>& out, float* kernel_ms);
 // the frames of a coupled movie in one launch with one control step (persistent.hip)
 int solve_nufft_movie(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       double lambda_t, const std::vector<double>& w, std::vector<RunStats>& out,

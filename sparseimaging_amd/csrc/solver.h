@@ -77,6 +77,14 @@ struct BatchSet {
   double vis_power = 0;  // sum |V|^2 (eps scaling)
 };
 
+// Device workspace of one problem of the wide engine (wide.hip).
+struct WideWork {
+  DevMem x, z, xnew, gr, d;  // FP64 iterates, gradient, step
+  DevMem T64, Gf64, rz64;    // FP64 forward transform and residual
+  DevMem rz32, Hg, U;        // FP32 adjoint
+  DevMem part, stat, state, cost;
+};
+
 // Device workspace of the DFT-mode multi-problem solver (dft_batch.hip).
 struct DftBatchBufs {
   DevMem A, y, X, Z, XN, D, G, Rx, Rz, Rn, part, psum, psq_n, psq_z, S, cost, box;
@@ -104,11 +112,17 @@ struct Context {
   DevMem trace, trace_tag;       // persistent-engine barrier trace (MFISTA_PERSISTENT_TRACE)
   DevMem g32;          // stream engine: FP32 gradient before its FP64 copy
   DftBatchBufs dbb;    // DFT mode, many problems sharing one matrix
+  std::vector<WideWork> wwork;   // wide engine: one workspace per problem
+  DevMem wdesc, wdone;           // wide engine: problem descriptors, finished-problem counter
+  hipEvent_t ev_t[2] = {nullptr, nullptr};  // timing events (wide engine)
 
   int persist_blocks_per_cu = 2;  // MFISTA_PERSISTENT_BLOCKS_PER_CU overrides
   int persist_blocks = 0;  // workgroups of the last persistent launch
   int persist_hier_barrier = 1;  // MFISTA_PERSISTENT_FLAT_BARRIER=1 selects the flat barrier
   int persist_pair = 1;          // MFISTA_PERSISTENT_PAIRED=0: one backtracking trial per round
+  int grid_tile_x = 16, grid_tile_y = 16;  // gridding tile of the next NUFFT set-up (autotuner)
+  int def_blocks_per_cu = 2, def_pair = 1;  // the knobs' defaults (environment overrides included)
+  int tune_engine = -1;  // batches: the autotuner's engine (-1: the caller's opts->engine)
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

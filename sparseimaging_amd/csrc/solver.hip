@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/mfista.h"
+#include "autotune.h"
 #include "mfista_kernels.h"
 #include "nufft.h"
 #include "driver.h"
@@ -124,12 +125,14 @@ int nufft_setup(Context& C, int preset, const double* d_u, const double* d_v, co
   g.MSP = MSP;
   g.Lx = 2 * NX;
   g.Ly = 2 * NY;
-  // gridding tiles: 16 x 16 cells, one per thread of the persistent gatherer.
-  // Per visibility the gather evaluates (tiles touched) x (cells per tile):
-  // ~730 cells for the 12-tap footprint, ~1500 for 24 taps (3000 with 32 x 32
-  // tiles).
-  g.TX = 16;
-  g.TY = 16;
+  // gridding tiles: 16 x 16 cells by default, one per thread of the
+  // persistent gatherer; the autotuner may choose 32 x 16 or 32 x 32 (2 or 4
+  // cells per thread).  Per visibility the gather evaluates (tiles touched) x
+  // (cells per tile): ~730 cells for the 12-tap footprint and ~1500 for 24
+  // taps with 16 x 16 tiles, 3000 for 24 taps with 32 x 32 tiles; larger tiles
+  // have fewer shared edges to flush with atomics.
+  g.TX = C.grid_tile_x;
+  g.TY = C.grid_tile_y;
   g.ntx = (g.Lx + g.TX - 1) / g.TX;
   g.nty = (NY + 1 + g.TY - 1) / g.TY;
   const double pi = M_PI, Mrx = 2.0 * NX, Mry = 2.0 * NY;
@@ -460,6 +463,34 @@ struct NufftOp : Operator<double> {
   }
 };
 
+// Times the autotuner's candidates on this problem: `iters` iterations from
+// its xinit, no stopping rule, xout overwritten.
+static int tune_single(Context& C, const SolveArgs& A, const mfista_opts* opts, int iters, TuneChoice* best,
+                       std::vector<TuneChoice>* all) {
+  SolveArgs a = A;
+  a.maxiter = iters > 0 ? iters : 30;
+  a.eps = 0.0;
+  mfista_opts o;
+  if (opts)
+    o = *opts;
+  else
+    mfista_default_opts(&o);
+  o.autotune = 0;
+  o.rel_tol = 0.0;
+  o.engine = MFISTA_ENGINE_PERSISTENT;
+  o.verbose = 0;
+  auto run = [&](double* us) -> int {
+    RESULT r;
+    memset(&r, 0, sizeof(r));
+    const int rc = solve_nufft(C, a, &r, &o);
+    if (rc) return rc;
+    if (o.engine_used != MFISTA_ENGINE_PERSISTENT) return MFISTA_E_COOP;
+    *us = o.kernel_ms * 1e3 / (r.ITER > 0 ? r.ITER : 1);
+    return 0;
+  };
+  return tune_search(C, tune_reg(A.lambda_tv, A.lambda_tsv), false, run, best, all);
+}
+
 int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
   const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
   int tv_mode;
@@ -476,6 +507,32 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     return MFISTA_E_TV_PRESET;
   }
   if (A.maxiter <= 0) return MFISTA_E_ARG;
+  // the autotuner's launch configuration for this shape (tuned here on a
+  // miss; the allocation-free entry point only looks it up)
+  const int engine_req = opts ? opts->engine : MFISTA_ENGINE_PERSISTENT;
+  bool tuned = false;
+  if (opts && opts->autotune && engine_req == MFISTA_ENGINE_PERSISTENT) {
+    const TuneKey k = tune_key(A.NX, A.NY, preset, tune_reg(A.lambda_tv, A.lambda_tsv), A.M, 1);
+    TuneChoice ch;
+    bool have = tune_lookup(k, &ch);
+    if (!have && !C.no_alloc) {
+      const int trc = tune_single(C, A, opts, opts->tune_iters, &ch, nullptr);
+      if (trc) return trc;
+      tune_store(k, ch);
+      have = true;
+    }
+    if (have) {
+      tune_apply(C, ch);
+      tuned = true;
+    }
+  }
+  struct RestoreKnobs {
+    Context& C;
+    bool on;
+    ~RestoreKnobs() {
+      if (on) tune_restore(C);
+    }
+  } restore_knobs{C, tuned};
   auto t0 = std::chrono::steady_clock::now();
   int rc = nufft_setup(C, preset, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY);
   if (rc) {
@@ -597,6 +654,9 @@ Context* get_context() {
     }
     if (const char* e = getenv("MFISTA_PERSISTENT_FLAT_BARRIER")) C->persist_hier_barrier = atoi(e) ? 0 : 1;
     if (const char* e = getenv("MFISTA_PERSISTENT_PAIRED")) C->persist_pair = atoi(e) ? 1 : 0;
+    C->def_blocks_per_cu = C->persist_blocks_per_cu;
+    C->def_pair = C->persist_pair;
+    tune_load_env();
     g_ctx[dev] = C;
   }
   return g_ctx[dev];
@@ -615,6 +675,8 @@ static void finalize_all() {
     (void)hipSetDevice(C->device);
     if (C->stream) (void)hipStreamSynchronize(C->stream);
     for (hipEvent_t& e : C->ev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t& e : C->ev_t)
       if (e) (void)hipEventDestroy(e);
     if (C->hflags) (void)hipHostFree(C->hflags);
     if (C->stream) (void)hipStreamDestroy(C->stream);
@@ -653,6 +715,7 @@ void mfista_default_opts(mfista_opts* o) {
   o->fgp_iter = FGPITER;
   o->rel_tol = 0.0;
   o->engine = MFISTA_ENGINE_PERSISTENT;
+  o->tune_iters = 30;
 }
 
 int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double* d_vis_r,
@@ -703,35 +766,16 @@ int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double
 // the contract.  Each problem follows mfista_imaging_core_nufft
 // (c/mfista_nufft_lib.c:909-957): eps scaled by its mean |V|^2, the same
 // argument checks and messages, and the FP64 RESULT pass.
-int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, struct RESULT* res,
-                                    mfista_opts* opts) {
-  std::lock_guard<std::mutex> lock(global_mutex());
-  Context* C = get_context();
-  if (!C) return MFISTA_E_HIP;
-  if (nprob <= 0 || !pr || !res) return MFISTA_E_ARG;
+// One pass of a batch: the NUFFT set-ups (one per visibility set), each
+// problem's workspace, the launch, and (results = true) the FP64 RESULT pass.
+// iters > 0 runs every problem for exactly that many iterations with no
+// stopping rule and writes no results (the autotuner's candidates).
+static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, const std::vector<int>& tv,
+                     mfista_opts* opts, int iters, bool results, RESULT* res, double* us_per_iter) {
   const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
   const int fgp_iter = (opts && opts->fgp_iter > 0) ? opts->fgp_iter : FGPITER;
-  const double rel_tol = opts ? opts->rel_tol : 0.0;
+  const double rel_tol = (opts && iters <= 0) ? opts->rel_tol : 0.0;
   auto t0 = std::chrono::steady_clock::now();
-  std::vector<int> tv(nprob, 0);
-  for (int i = 0; i < nprob; i++) {
-    const mfista_nufft_problem& q = pr[i];
-    if (q.M <= 0 || q.NX <= 0 || q.NY <= 0 || !q.u_dx || !q.v_dy || !q.vis_r || !q.vis_i || !q.vis_std ||
-        !q.xinit || !q.xout || (q.box_flag && !q.cl_box) || q.maxiter <= 0)
-      return MFISTA_E_ARG;
-    if (q.lambda_tv == 0)
-      tv[i] = 0;
-    else if (q.lambda_tsv == 0)
-      tv[i] = 1;
-    else {
-      printf("You cannot set both of lambda_TV and lambda_TSV positive.\n");
-      return MFISTA_E_BOTH_REG;
-    }
-    if (tv[i] && preset == MFISTA_PRESET_CXX12) {
-      printf("We have not implemented TV option.\n");
-      return MFISTA_E_TV_PRESET;
-    }
-  }
   // visibility sets: problems with the same arrays share one NUFFT set-up
   std::vector<int> set_of(nprob, -1);
   std::vector<int> set_rep;  // a problem of each set
@@ -778,7 +822,7 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
   std::vector<int> which;  // problem index of each solve
   for (int i = 0; i < nprob; i++) {
     const mfista_nufft_problem& q = pr[i];
-    memset(&res[i], 0, sizeof(RESULT));
+    if (results) memset(&res[i], 0, sizeof(RESULT));
     if (!tv[i] && q.nonneg_flag != 0 && q.nonneg_flag != 1) {  // c/mfista_nufft_lib.c:634-637
       printf("nonneg_flag must be chosen properly.\n");
       continue;
@@ -790,12 +834,12 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     RunParams R;
     R.NX = q.NX;
     R.NY = q.NY;
-    R.maxiter = q.maxiter;
+    R.maxiter = iters > 0 ? iters : q.maxiter;
     R.nonneg = q.nonneg_flag;
     R.box_flag = q.box_flag;
     R.tv_mode = tv[i];
     R.fgp_iter = fgp_iter;
-    R.eps_abs = q.eps * C->bset[set_of[i]].vis_power / (double)q.M;
+    R.eps_abs = iters > 0 ? 0.0 : q.eps * C->bset[set_of[i]].vis_power / (double)q.M;
     R.rel_tol = rel_tol;
     R.lambda_l1 = q.lambda_l1;
     R.lambda_tv = q.lambda_tv;
@@ -813,10 +857,23 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
   }
   std::vector<RunStats> st;
   float kms = 0;
-  if ((rc = solve_nufft_batch(*C, sets, P, st, &kms))) return rc;
+  // the wide engine (every kernel launched once for all problems) takes
+  // batches of L1 / L1+TSV problems of one image size; others, and the
+  // default, run in one persistent launch
+  const int engine = C->tune_engine >= 0 ? C->tune_engine : (opts ? opts->engine : MFISTA_ENGINE_PERSISTENT);
+  bool wide = engine == MFISTA_ENGINE_STREAM && !P.empty();
+  for (size_t j = 0; wide && j < P.size(); j++)
+    if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide = false;
+  if (P.empty())
+    st.clear();
+  else if (wide)
+    rc = solve_nufft_wide(*C, sets, P, st, &kms);
+  else
+    rc = solve_nufft_batch(*C, sets, P, st, &kms);
+  if (rc) return rc;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   int total_trials = 0, max_trials = 0, rounds = 0;
-  for (size_t j = 0; j < which.size(); j++) {
+  for (size_t j = 0; results && j < which.size(); j++) {
     const int i = which[j];
     const mfista_nufft_problem& q = pr[i];
     PersistWork& W = C->bwork[i];
@@ -843,13 +900,105 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     R->mean_sq_error = R->sq_error / (double)q.M;
   }
   if (opts) {
-    opts->engine_used = MFISTA_ENGINE_PERSISTENT;
+    opts->engine_used = wide ? MFISTA_ENGINE_STREAM : MFISTA_ENGINE_PERSISTENT;
     opts->solve_ms = secs * 1e3;
     opts->kernel_ms = kms;
     opts->total_trials = total_trials;
     opts->max_trials = max_trials;
     opts->trial_rounds = rounds;
   }
+  if (us_per_iter) *us_per_iter = kms * 1e3 / (iters > 0 ? iters : 1);
+  return 0;
+}
+
+// Argument checks of a batch (each problem as mfista_imaging_core_nufft_ex
+// checks it); tv[i] = 1 for the L1+TV problems.
+static int batch_check(int nprob, const mfista_nufft_problem* pr, int preset, std::vector<int>* tvp) {
+  std::vector<int>& tv = *tvp;
+  tv.assign(nprob, 0);
+  for (int i = 0; i < nprob; i++) {
+    const mfista_nufft_problem& q = pr[i];
+    if (q.M <= 0 || q.NX <= 0 || q.NY <= 0 || !q.u_dx || !q.v_dy || !q.vis_r || !q.vis_i || !q.vis_std ||
+        !q.xinit || !q.xout || (q.box_flag && !q.cl_box) || q.maxiter <= 0)
+      return MFISTA_E_ARG;
+    if (q.lambda_tv == 0)
+      tv[i] = 0;
+    else if (q.lambda_tsv == 0)
+      tv[i] = 1;
+    else {
+      printf("You cannot set both of lambda_TV and lambda_TSV positive.\n");
+      return MFISTA_E_BOTH_REG;
+    }
+    if (tv[i] && preset == MFISTA_PRESET_CXX12) {
+      printf("We have not implemented TV option.\n");
+      return MFISTA_E_TV_PRESET;
+    }
+  }
+  return 0;
+}
+
+int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, struct RESULT* res,
+                                    mfista_opts* opts) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (nprob <= 0 || !pr || !res) return MFISTA_E_ARG;
+  const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
+  int rc = 0;
+  std::vector<int> tv;
+  if ((rc = batch_check(nprob, pr, preset, &tv))) return rc;
+  bool tuned = false;
+  if (opts && opts->autotune) {
+    int mmax = 0, reg = 0;
+    for (int i = 0; i < nprob; i++) {
+      mmax = std::max(mmax, pr[i].M);
+      reg = std::max(reg, tune_reg(pr[i].lambda_tv, pr[i].lambda_tsv));
+    }
+    const TuneKey k = tune_key(pr[0].NX, pr[0].NY, preset, reg, mmax, nprob);
+    TuneChoice ch;
+    if (!tune_lookup(k, &ch)) {
+      const int iters = opts->tune_iters > 0 ? opts->tune_iters : 30;
+      auto run = [&](double* us) { return run_batch(C, nprob, pr, tv, opts, iters, false, nullptr, us); };
+      if ((rc = tune_search(*C, reg, true, run, &ch, nullptr))) return rc;
+      tune_store(k, ch);
+    }
+    tune_apply(*C, ch);
+    tuned = true;
+  }
+  rc = run_batch(C, nprob, pr, tv, opts, 0, true, res, nullptr);
+  if (tuned) tune_restore(*C);
+  return rc;
+}
+
+int mfista_autotune_nufft_batch(int nprob, const mfista_nufft_problem* pr, const mfista_opts* opts, int iters,
+                                mfista_tune* best, mfista_tune* all, int max_all, int* n_all) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (nprob <= 0 || !pr || !best) return MFISTA_E_ARG;
+  mfista_opts o;
+  if (opts)
+    o = *opts;
+  else
+    mfista_default_opts(&o);
+  int rc = 0;
+  std::vector<int> tv;
+  if ((rc = batch_check(nprob, pr, o.preset, &tv))) return rc;
+  int mmax = 0, reg = 0;
+  for (int i = 0; i < nprob; i++) {
+    mmax = std::max(mmax, pr[i].M);
+    reg = std::max(reg, tune_reg(pr[i].lambda_tv, pr[i].lambda_tsv));
+  }
+  const int it = iters > 0 ? iters : 30;
+  auto run = [&](double* us) { return run_batch(C, nprob, pr, tv, &o, it, false, nullptr, us); };
+  TuneChoice ch;
+  std::vector<TuneChoice> cand;
+  if ((rc = tune_search(*C, reg, true, run, &ch, &cand))) return rc;
+  const TuneKey k = tune_key(pr[0].NX, pr[0].NY, o.preset, reg, mmax, nprob);
+  tune_store(k, ch);
+  to_public(k, ch, best);
+  for (int i = 0; all && i < (int)cand.size() && i < max_all; i++) to_public(k, cand[i], &all[i]);
+  if (n_all) *n_all = (int)cand.size();
   return 0;
 }
 
@@ -1083,6 +1232,54 @@ int mfista_imaging_core_nufft_ex(const double* u_dx, const double* v_dy, const d
   if (rc) return rc;
   if (hipMemcpy(xout, C->h_x1.p, nb, hipMemcpyDeviceToHost) != hipSuccess) return MFISTA_E_HIP;
   *res = tmp;
+  return 0;
+}
+
+int mfista_autotune_nufft(const double* u_dx, const double* v_dy, const double* vis_r,
+                          const double* vis_i, const double* vis_std, int M, int NX, int NY,
+                          double lambda_l1, double lambda_tv, double lambda_tsv, double cinit,
+                          const double* xinit, int nonneg_flag, const mfista_opts* opts, int iters,
+                          mfista_tune* best, mfista_tune* all, int max_all, int* n_all) {
+  std::lock_guard<std::mutex> lock(global_mutex());
+  Context* C = get_context();
+  if (!C) return MFISTA_E_HIP;
+  if (M <= 0 || NX <= 0 || NY <= 0 || !u_dx || !v_dy || !vis_r || !vis_i || !vis_std || !best)
+    return MFISTA_E_ARG;
+  const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
+  if (lambda_tv > 0 && lambda_tsv > 0) return MFISTA_E_BOTH_REG;
+  if (lambda_tv > 0 && preset == MFISTA_PRESET_CXX12) return MFISTA_E_TV_PRESET;
+  const size_t Mb = (size_t)M * 8, nb = (size_t)NX * NY * 8;
+  int rc;
+  if ((rc = upload(*C, C->h_u, u_dx, Mb)) || (rc = upload(*C, C->h_v, v_dy, Mb)) ||
+      (rc = upload(*C, C->h_vr, vis_r, Mb)) || (rc = upload(*C, C->h_vi, vis_i, Mb)) ||
+      (rc = upload(*C, C->h_sd, vis_std, Mb)) || (rc = upload(*C, C->h_x0, xinit, nb)) ||
+      (rc = upload(*C, C->h_x1, nullptr, nb)))
+    return rc;
+  if (!xinit) MF_CHECK(hipMemsetAsync(C->h_x0.p, 0, nb, C->stream));
+  SolveArgs A;
+  A.u = (const double*)C->h_u.p;
+  A.v = (const double*)C->h_v.p;
+  A.vr = (const double*)C->h_vr.p;
+  A.vi = (const double*)C->h_vi.p;
+  A.sd = (const double*)C->h_sd.p;
+  A.M = M;
+  A.NX = NX;
+  A.NY = NY;
+  A.lambda_l1 = lambda_l1;
+  A.lambda_tv = lambda_tv;
+  A.lambda_tsv = lambda_tsv;
+  A.cinit = cinit;
+  A.xinit = (const double*)C->h_x0.p;
+  A.xout = (double*)C->h_x1.p;
+  A.nonneg = nonneg_flag;
+  TuneChoice ch;
+  std::vector<TuneChoice> cand;
+  if ((rc = tune_single(*C, A, opts, iters, &ch, &cand))) return rc;
+  const TuneKey k = tune_key(NX, NY, preset, tune_reg(lambda_tv, lambda_tsv), M, 1);
+  tune_store(k, ch);
+  to_public(k, ch, best);
+  for (int i = 0; all && i < (int)cand.size() && i < max_all; i++) to_public(k, cand[i], &all[i]);
+  if (n_all) *n_all = (int)cand.size();
   return 0;
 }
 

@@ -1,0 +1,443 @@
+// wide.hip -- the batched-kernel ("wide") engine for many NUFFT solves at once.
+//
+// The loop is the stream engine's (driver.hip mfista_run, restating
+// c/mfista_nufft_lib.c:557-787): every step is one backtracking trial,
+// preceded by the iteration prologue (forward transform at z in FP64, FP32
+// adjoint of its residual, TSV gradient) when the previous trial was accepted,
+// and a one-workgroup FP64 control step per problem sets the gates of the next
+// step.  What changes is the launch shape: every kernel of a step covers all
+// problems of the batch at once (blockIdx.y = problem, blockIdx.x = the
+// stage's task within it), so a stage of 32 problems at 256^2 fills the GPU
+// with thousands of workgroups instead of a hundred, and the kernels keep
+// their own small register footprints (several workgroups per CU), where the
+// persistent batch shares one 255-VGPR kernel across all stages.  A problem
+// whose gates are closed (converged, or accepting in the middle of a trial
+// sequence) returns at once from the kernels that do not apply to it.
+//
+// The step's launch sequence is captured once per batch in a hipGraph and
+// replayed; the host reads one counter of finished problems every few steps.
+// Everything a kernel needs is in device memory (per-problem descriptors and
+// states), so replays need no host work.  No memset is captured: the adjoint
+// grids are cleared by the column pass that reads them.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "driver.h"
+#include "mfista_dev.h"
+#include "solver.h"
+#include "stages.h"
+
+namespace mf {
+
+// One problem of a wide batch (device copy; the kernels index it by blockIdx.y).
+struct WProb {
+  GridGeom g;  // of the problem's visibility set (M, tiles)
+  VisParams<double> pd;
+  VisParams<float> pf;
+  const int4* chunks;  // gridding chunk list of the set
+  const unsigned* ent;
+  int nchunks, n_degrid;
+  PartialLayout L;
+  double *x, *z, *xnew, *gr, *d;  // FP64 iterates, gradient and step
+  double2 *T64, *Gf64;            // FP64 forward transform (of z, d or the start)
+  double2* rz64;                  // FP64 weighted residual at z
+  float2* rz32;                   // its FP32 copy (gridding input)
+  float2 *Hg, *U;                 // FP32 adjoint grid (kept zero between adjoints) and column pass
+  double *part, *stat, *cost;
+  DevState* S;
+  const float* box;
+  double lambda_tsv;
+};
+
+// What the problems of a batch share: image size, preset, FFT plans, twiddle
+// and E4 tables (they depend only on the size and the preset).
+struct WShared {
+  FFTPlan Px, Py;
+  const double2 *twx64, *twy64;
+  const float2 *twx32, *twy32;
+  const double *e4x64, *e4y64;
+  const float *e4x32, *e4y32;
+  int NX, NY, Lx, Ly, MSP;
+  int nseq_r64, nseq_c64, nseq_ci32, nseq_c2r32;
+  float adj_scale;  // 1 / (8 NX NY)
+  int* ndone;       // problems finished
+};
+
+// which transform a forward kernel computes
+enum WFwd { kWInit = 0, kWProlog = 1, kWTrial = 2 };
+
+__device__ inline bool w_closed(const DevState* S, int which) {
+  if (which == kWProlog) return S->gate_prologue != 0;
+  if (which == kWTrial) return S->gate_trial != 0;
+  return false;
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_rowfft64(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if (w_closed(p.S, which)) return;
+  extern __shared__ double4 smem_d4[];
+  const double* x = (which == kWInit) ? p.x : (which == kWProlog ? p.z : p.d);
+  const int NY = sh.NY;
+  auto load = [&](int i, int j) -> double { return x[(size_t)i * NY + j] * sh.e4x64[i] * sh.e4y64[j]; };
+  st_rowfft_fwd<double>(blockIdx.x, sh.nseq_r64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64, p.T64,
+                        reinterpret_cast<double2*>(smem_d4), load);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_colfft64(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if (w_closed(p.S, which)) return;
+  extern __shared__ double4 smem_d4[];
+  st_colfft_fwd<double>(blockIdx.x, sh.nseq_c64, p.T64, p.Gf64, sh.NX, sh.NY + 1, sh.Lx, sh.NX / 2, 0, sh.MSP,
+                        sh.Px, sh.twx64, reinterpret_cast<double2*>(smem_d4));
+}
+
+// init: chi^2 partials at the start (L.chi_x); prologue: residual at z (FP64
+// and FP32 copies) and chi^2 partials (L.chi_z); trial: step mode, chi^2 at
+// xnew (L.chi_x) and |W A d|^2 (L.lip)
+template <int MSPC>
+__global__ void __launch_bounds__(MF_BLOCK) kw_degrid64(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if ((int)blockIdx.x >= p.n_degrid || w_closed(p.S, which)) return;
+  __shared__ double red[32];
+  double v[2];
+  if (which == kWTrial)
+    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, nullptr, nullptr, p.rz64, 1, nullptr, nullptr,
+                            nullptr, v);
+  else if (which == kWProlog)
+    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, p.rz64, p.rz32, nullptr, 0, nullptr, nullptr,
+                            nullptr, v);
+  else
+    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                            nullptr, v);
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    if (which == kWTrial) {
+      p.part[p.L.chi_x + blockIdx.x] = v[0];
+      p.part[p.L.lip + blockIdx.x] = v[1];
+    } else {
+      p.part[(which == kWProlog ? p.L.chi_z : p.L.chi_x) + blockIdx.x] = v[0];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_grid32(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if ((int)blockIdx.x >= p.nchunks || p.S->gate_prologue) return;
+  extern __shared__ double4 smem_d4[];
+  const float2* rz = p.rz32;
+  st_grid<float>(blockIdx.x, p.g, p.pf, [&](int s) { return rz[s]; }, p.chunks, p.ent, p.Hg,
+                 reinterpret_cast<float2*>(smem_d4));
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_colinv32(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if (p.S->gate_prologue) return;
+  extern __shared__ double4 smem_d4[];
+  st_colfft_inv<float>(blockIdx.x, sh.nseq_ci32, p.Hg, p.U, sh.NX, sh.NY + 1, sh.Lx, sh.NX / 2, 0, sh.Px,
+                       sh.twx32, reinterpret_cast<float2*>(smem_d4));
+}
+
+// c2r along ky, crop, E4 / (8 NX NY) in FP32 (as the stream engine's FP32
+// adjoint), stored as the FP64 gradient
+__global__ void __launch_bounds__(MF_BLOCK) kw_c2r32(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if (p.S->gate_prologue) return;
+  extern __shared__ double4 smem_d4[];
+  const int NY = sh.NY;
+  double* gr = p.gr;
+  auto store = [&](int i, int j, float v) {
+    v *= sh.adj_scale;
+    v *= sh.e4x32[i];
+    v *= sh.e4y32[j];
+    gr[(size_t)i * NY + j] = (double)v;
+  };
+  st_rowfft_c2r<float>(blockIdx.x, sh.nseq_c2r32, p.U, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy32,
+                       reinterpret_cast<float2*>(smem_d4), store);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_tsv_grad(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if (p.S->gate_prologue || !(p.lambda_tsv > 0.0)) return;
+  __shared__ double red[16];
+  dev_tsv_grad<double>(blockIdx.x, p.z, p.gr, sh.NX, sh.NY, p.lambda_tsv, p.part + p.L.tsv_z, red);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_prox(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if (p.S->gate_trial) return;
+  __shared__ double red[16 * kProxW];
+  dev_prox<double>(blockIdx.x, p.z, p.gr, p.xnew, p.d, p.box, sh.NX, sh.NY, p.S, p.part + p.L.prox, red);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_control(const WProb* __restrict__ W, int* ndone) {
+  const WProb& p = W[blockIdx.y];
+  __shared__ double red[16];
+  const int was_done = p.S->done;
+  dev_control(p.S, p.cost, p.part, p.L, red);
+  if (threadIdx.x == 0 && !was_done && p.S->done) atomicAdd(ndone, 1);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_momentum(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  dev_momentum<double>(blockIdx.x, p.x, p.xnew, p.z, sh.NX * sh.NY, p.S);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_stats(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  __shared__ double red[64];
+  dev_stats<double>(blockIdx.x, p.x, sh.NX, sh.NY, p.stat, red);
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_init_control(const WProb* __restrict__ W, int n_elem) {
+  const WProb& p = W[blockIdx.y];
+  __shared__ double red[16];
+  dev_init_control(p.S, p.part, p.L, p.stat, n_elem, red);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static int nseq_for(int L, size_t csize, int nseq_total) {
+  int n = (int)(65536 / (2 * (size_t)L * csize));
+  if (n < 1) n = 1;
+  int want = nseq_total / 256;
+  if (want < 1) want = 1;
+  if (n > want) n = want;
+  if (n > 8) n = 8;
+  return n;
+}
+
+struct WLaunch {
+  dim3 rowf, colf, deg, grid, colinv, c2r, elem, one;
+  size_t sh_rowf, sh_colf, sh_grid, sh_colinv, sh_c2r;
+};
+
+static void launch_forward(const WLaunch& L, const WProb* W, const WShared& sh, int which, int MSP,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(kw_rowfft64, L.rowf, dim3(MF_BLOCK), L.sh_rowf, st, W, sh, which);
+  hipLaunchKernelGGL(kw_colfft64, L.colf, dim3(MF_BLOCK), L.sh_colf, st, W, sh, which);
+  if (MSP == 6)
+    hipLaunchKernelGGL(kw_degrid64<6>, L.deg, dim3(MF_BLOCK), 0, st, W, sh, which);
+  else
+    hipLaunchKernelGGL(kw_degrid64<12>, L.deg, dim3(MF_BLOCK), 0, st, W, sh, which);
+}
+
+// one trial step of every problem (the stream engine's step)
+static void launch_step(const WLaunch& L, const WProb* W, const WShared& sh, int MSP, hipStream_t st) {
+  launch_forward(L, W, sh, kWProlog, MSP, st);
+  hipLaunchKernelGGL(kw_grid32, L.grid, dim3(MF_BLOCK), L.sh_grid, st, W, sh);
+  hipLaunchKernelGGL(kw_colinv32, L.colinv, dim3(MF_BLOCK), L.sh_colinv, st, W, sh);
+  hipLaunchKernelGGL(kw_c2r32, L.c2r, dim3(MF_BLOCK), L.sh_c2r, st, W, sh);
+  hipLaunchKernelGGL(kw_tsv_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_prox, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  launch_forward(L, W, sh, kWTrial, MSP, st);
+  hipLaunchKernelGGL(kw_control, L.one, dim3(MF_BLOCK), 0, st, W, sh.ndone);
+  hipLaunchKernelGGL(kw_momentum, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+}
+
+int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
+                     std::vector<RunStats>& out, float* kernel_ms) {
+  const int B = (int)P.size();
+  if (B <= 0 || (int)sets.size() != B) return MFISTA_E_ARG;
+  NufftState& N0 = *sets[0];
+  const int NX = P[0].NX, NY = P[0].NY, n = NX * NY, MSP = N0.MSP;
+  for (int b = 0; b < B; b++) {
+    const NufftState& N = *sets[b];
+    if (P[b].NX != NX || P[b].NY != NY || N.MSP != MSP || N.geom.TX != N0.geom.TX || N.geom.TY != N0.geom.TY ||
+        P[b].tv_mode)
+      return MFISTA_E_ARG;
+  }
+  hipStream_t st = C.stream;
+  const GridGeom& g0 = N0.geom;
+  const int nky = NY + 1;
+  WShared sh;
+  memset(&sh, 0, sizeof(sh));
+  sh.Px = N0.Px;
+  sh.Py = N0.Py;
+  sh.twx64 = (const double2*)N0.twxd.p;
+  sh.twy64 = (const double2*)N0.twyd.p;
+  sh.twx32 = (const float2*)N0.twxf.p;
+  sh.twy32 = (const float2*)N0.twyf.p;
+  sh.e4x64 = (const double*)N0.e4xd.p;
+  sh.e4y64 = (const double*)N0.e4yd.p;
+  sh.e4x32 = (const float*)N0.e4xf.p;
+  sh.e4y32 = (const float*)N0.e4yf.p;
+  sh.NX = NX;
+  sh.NY = NY;
+  sh.Lx = g0.Lx;
+  sh.Ly = g0.Ly;
+  sh.MSP = MSP;
+  sh.nseq_r64 = nseq_for(g0.Ly, sizeof(double2), (NX + 1) / 2);
+  sh.nseq_c64 = nseq_for(g0.Lx, sizeof(double2), nky);
+  sh.nseq_ci32 = nseq_for(g0.Lx, sizeof(float2), nky);
+  sh.nseq_c2r32 = nseq_for(g0.Ly, sizeof(float2), (NX + 1) / 2);
+  sh.adj_scale = (float)(1.0 / (8.0 * (double)NX * NY));
+  // per-problem workspaces and descriptors
+  if ((int)C.wwork.size() < B) C.wwork.resize(B);
+  std::vector<WProb> hw(B);
+  const int n_elem = elem_blocks(n);
+  const size_t rb = (size_t)n * 8;
+  int max_deg = 1, max_chunks = 1;
+  for (int b = 0; b < B; b++) {
+    NufftState& N = *sets[b];
+    WideWork& Wk = C.wwork[b];
+    const RunParams& R = P[b];
+    const int ndeg = degrid_blocks(N.M, MSP);
+    PartialLayout L;
+    L.n_degrid = ndeg;
+    L.n_elem = n_elem;
+    L.chi_z = 0;
+    L.chi_x = ndeg;
+    L.lip = 2 * ndeg;
+    L.tsv_z = 3 * ndeg;
+    L.prox = L.tsv_z + n_elem;
+    L.total = L.prox + kProxW * n_elem;
+    if (Wk.x.ensure(rb) || Wk.z.ensure(rb) || Wk.xnew.ensure(rb) || Wk.gr.ensure(rb) || Wk.d.ensure(rb) ||
+        Wk.T64.ensure((size_t)nky * NX * sizeof(double2)) ||
+        Wk.Gf64.ensure(((size_t)NY + MSP) * g0.Lx * sizeof(double2)) ||
+        Wk.rz64.ensure((size_t)N.M * sizeof(double2)) || Wk.rz32.ensure((size_t)N.M * sizeof(float2)) ||
+        Wk.Hg.ensure((size_t)nky * g0.Lx * sizeof(float2)) || Wk.U.ensure((size_t)NX * nky * sizeof(float2)) ||
+        Wk.part.ensure((size_t)L.total * 8) || Wk.stat.ensure((size_t)4 * n_elem * 8) ||
+        Wk.state.ensure(sizeof(DevState)) || Wk.cost.ensure((size_t)(R.maxiter > 0 ? R.maxiter : 1) * 8))
+      return MFISTA_E_HIP;
+    MF_CHECK(hipMemsetAsync(Wk.Hg.p, 0, (size_t)nky * g0.Lx * sizeof(float2), st));
+    MF_CHECK(hipMemcpyAsync(Wk.x.p, R.xinit, rb, hipMemcpyDeviceToDevice, st));
+    MF_CHECK(hipMemcpyAsync(Wk.z.p, R.xinit, rb, hipMemcpyDeviceToDevice, st));
+    DevState h;
+    memset(&h, 0, sizeof(h));
+    h.c = R.cinit;
+    h.lambda_l1 = R.lambda_l1;
+    h.lambda_tsv = R.lambda_tsv;
+    h.eps_abs = R.eps_abs;
+    h.rel_tol = R.rel_tol;
+    h.eta_first = R.eta_first;
+    h.maxiter = R.maxiter;
+    h.nonneg = R.nonneg;
+    h.lip_mode = 1;
+    MF_CHECK(hipMemcpyAsync(Wk.state.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    WProb& w = hw[b];
+    memset(&w, 0, sizeof(w));
+    w.g = N.geom;
+    w.pd = N.pd();
+    w.pf = N.pf();
+    w.chunks = (const int4*)N.chunks.p;
+    w.ent = (const unsigned*)N.ent.p;
+    w.nchunks = N.nchunks;
+    w.n_degrid = ndeg;
+    w.L = L;
+    w.x = (double*)Wk.x.p;
+    w.z = (double*)Wk.z.p;
+    w.xnew = (double*)Wk.xnew.p;
+    w.gr = (double*)Wk.gr.p;
+    w.d = (double*)Wk.d.p;
+    w.T64 = (double2*)Wk.T64.p;
+    w.Gf64 = (double2*)Wk.Gf64.p;
+    w.rz64 = (double2*)Wk.rz64.p;
+    w.rz32 = (float2*)Wk.rz32.p;
+    w.Hg = (float2*)Wk.Hg.p;
+    w.U = (float2*)Wk.U.p;
+    w.part = (double*)Wk.part.p;
+    w.stat = (double*)Wk.stat.p;
+    w.cost = (double*)Wk.cost.p;
+    w.S = (DevState*)Wk.state.p;
+    w.box = R.box_flag ? R.box : nullptr;
+    w.lambda_tsv = R.lambda_tsv;
+    max_deg = std::max(max_deg, ndeg);
+    max_chunks = std::max(max_chunks, N.nchunks);
+  }
+  if (C.wdesc.ensure((size_t)B * sizeof(WProb)) || C.wdone.ensure(sizeof(int))) return MFISTA_E_HIP;
+  MF_CHECK(hipMemcpyAsync(C.wdesc.p, hw.data(), (size_t)B * sizeof(WProb), hipMemcpyHostToDevice, st));
+  MF_CHECK(hipMemsetAsync(C.wdone.p, 0, sizeof(int), st));
+  sh.ndone = (int*)C.wdone.p;
+  const WProb* W = (const WProb*)C.wdesc.p;
+  WLaunch L;
+  L.rowf = dim3((NX + 2 * sh.nseq_r64 - 1) / (2 * sh.nseq_r64), B);
+  L.sh_rowf = 2 * (size_t)sh.nseq_r64 * g0.Ly * sizeof(double2);
+  L.colf = dim3((nky + sh.nseq_c64 - 1) / sh.nseq_c64, B);
+  L.sh_colf = 2 * (size_t)sh.nseq_c64 * g0.Lx * sizeof(double2);
+  L.deg = dim3(max_deg, B);
+  L.grid = dim3(max_chunks, B);
+  L.sh_grid = 4 * (size_t)g0.TX * g0.TY * sizeof(float2);
+  L.colinv = dim3((nky + sh.nseq_ci32 - 1) / sh.nseq_ci32, B);
+  L.sh_colinv = 2 * (size_t)sh.nseq_ci32 * g0.Lx * sizeof(float2);
+  L.c2r = dim3((NX + 2 * sh.nseq_c2r32 - 1) / (2 * sh.nseq_c2r32), B);
+  L.sh_c2r = 2 * (size_t)sh.nseq_c2r32 * g0.Ly * sizeof(float2);
+  L.elem = dim3(n_elem, B);
+  L.one = dim3(1, B);
+  // start: cost of xinit (chi^2, |x|_1, TSV) and the initial control state
+  launch_forward(L, W, sh, kWInit, MSP, st);
+  hipLaunchKernelGGL(kw_stats, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_init_control, L.one, dim3(MF_BLOCK), 0, st, W, n_elem);
+  MF_CHECK(hipGetLastError());
+  // one step captured as a graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  MF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  launch_step(L, W, sh, MSP, st);
+  if (hipStreamEndCapture(st, &graph) != hipSuccess) return MFISTA_E_HIP;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return MFISTA_E_HIP;
+  }
+  int maxit = 1;
+  for (const RunParams& R : P) maxit = std::max(maxit, R.maxiter);
+  const long cap = (long)maxit * 64 + 100000;
+  const int K = 16;  // steps between reads of the finished-problem counter
+  int* hflag = C.hflags;
+  int pending = -1, rc = 0;
+  long step = 0;
+  for (hipEvent_t& e : C.ev_t)
+    if (!e) MF_CHECK(hipEventCreate(&e));
+  MF_CHECK(hipEventRecord(C.ev_t[0], st));
+  for (;; step++) {
+    if (hipGraphLaunch(exec, st) != hipSuccess) {
+      rc = MFISTA_E_HIP;
+      break;
+    }
+    if ((step + 1) % K == 0) {
+      const int slot = (int)((step / K) & 1);
+      MF_CHECK(hipMemcpyAsync(&hflag[slot], C.wdone.p, sizeof(int), hipMemcpyDeviceToHost, st));
+      MF_CHECK(hipEventRecord(C.ev[slot], st));
+      if (pending >= 0) {
+        MF_CHECK(hipEventSynchronize(C.ev[pending]));
+        if (hflag[pending] >= B) break;
+      }
+      pending = slot;
+    }
+    if (step > cap) {
+      fprintf(stderr, "[mfista_amd] wide engine: solver did not terminate\n");
+      rc = MFISTA_E_ARG;
+      break;
+    }
+  }
+  MF_CHECK(hipEventRecord(C.ev_t[1], st));
+  MF_CHECK(hipStreamSynchronize(st));
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
+  float ms = 0;
+  MF_CHECK(hipEventElapsedTime(&ms, C.ev_t[0], C.ev_t[1]));
+  if (kernel_ms) *kernel_ms = ms;
+  out.assign(B, RunStats());
+  for (int b = 0; b < B; b++) {
+    DevState h;
+    MF_CHECK(hipMemcpy(&h, C.wwork[b].state.p, sizeof(h), hipMemcpyDeviceToHost));
+    RunStats& o = out[b];
+    o.ITER = h.ITER;
+    o.c = h.c;
+    o.max_trials = h.max_trials;
+    o.total_trials = h.total_trials;
+    o.status = h.status;
+    o.steps = step + 1;
+    o.rounds = h.total_trials;
+    o.kernel_ms = ms;
+    MF_CHECK(hipMemcpyAsync(P[b].xout, C.wwork[b].x.p, rb, hipMemcpyDeviceToDevice, st));
+  }
+  MF_CHECK(hipStreamSynchronize(st));
+  C.last_steps = step + 1;
+  return 0;
+}
+
+}  // namespace mf

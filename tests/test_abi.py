@@ -77,6 +77,39 @@ def test_batch_problem_and_opts_layout_match_ctypes(tmp_path):
                    ctypes.sizeof(sa.Opts)]
 
 
+def test_tune_struct_layout_matches_ctypes(tmp_path):
+    prog = tmp_path / "layout3.c"
+    prog.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "mfista.h"\n'
+        "int main(void){\n"
+        'printf("%zu %zu %zu %zu\\n", sizeof(mfista_tune), offsetof(mfista_tune, tile_x),'
+        " offsetof(mfista_tune, paired), offsetof(mfista_tune, us_per_iter)); return 0;}\n")
+    exe = tmp_path / "layout3"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    T = sa.Tune
+    assert out == [ctypes.sizeof(T), T.tile_x.offset, T.paired.offset, T.us_per_iter.offset]
+
+
+def test_tune_table_file_round_trip(tmp_path):
+    """The autotuner's table file (no GPU work): a saved entry loads back, and
+    lines with an unsupported tile or a bad field count are skipped."""
+    f = tmp_path / "tune.txt"
+    f.write_text("# header\n256 256 1 1 14 0 32 16 2 1 0 88.5\n256 256 1 1 14 3 24 16 2 1 0 1.0\n"
+                 "256 256 1 1 14 4 16 16 2 1 7 1.0\n1 2 3\n")
+    sa.tune_clear()
+    assert sa.tune_load(str(f)) == 1
+    t = sa.tune_table()
+    assert len(t) == 1 and t[0]["tile_x"] == 32 and t[0]["tile_y"] == 16 and t[0]["blocks_per_cu"] == 2
+    assert t[0]["m_log2"] == 14 and t[0]["engine"] == sa.ENGINE_PERSISTENT and abs(t[0]["us_per_iter"] - 88.5) < 1e-9
+    g = tmp_path / "tune2.txt"
+    sa.tune_save(str(g))
+    sa.tune_clear()
+    assert sa.tune_table() == []
+    assert sa.tune_load(str(g)) == 1 and sa.tune_table() == t
+    sa.tune_clear()
+
+
 def test_product_does_not_import_the_oracle():
     pkg = os.path.join(ROOT, "sparseimaging_amd")
     for dirpath, _, files in os.walk(pkg):

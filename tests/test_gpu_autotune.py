@@ -1,0 +1,146 @@
+"""The persistent solver's launch-configuration autotuner and the gridding
+tiles it chooses among (no reference counterpart; the reference's schedule is
+fixed, its nearest mechanism being configure_tile, c++/mfista_nufft_lib.cpp:
+643-678).
+
+* every tile kernel (16x16, 32x16, 32x32 cells; 1, 2 and 4 cells per thread),
+  with one and two workgroups per CU and with paired or single trial rounds,
+  solves NUFFT L1+TSV (both presets) and L1+TV (C-24) to the oracle's optimum:
+  image relative L2 <= 1e-3, objective <= 1e-6 -- the configuration is forced
+  through a loaded tune table, so the test also covers the table's lookup;
+* a batch launch with a forced 32x32 tile against the default single solves;
+* the tuner on the benchmark workload (256^2, M = 20000, 12 taps): 12
+  candidates timed twice each, the fastest kept; a solve with autotune=True
+  then times nothing new, and matches the oracle at its stopping count;
+* the tuner on the 6x6 lambda grid at 128^2 (M = 8000): the tuned batch
+  against the untuned batch at a fixed 300 iterations.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sparseimaging_amd as sa
+from sparseimaging_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(16, 16), (32, 16), (32, 32)]
+
+
+def _force(tmp_path, NX, NY, preset, reg, M, nprob, tile, bpc=2, paired=1, engine=0):
+    f = tmp_path / "tune.txt"
+    f.write_text(f"{NX} {NY} {preset} {reg} {int(np.log2(M))} {int(np.log2(nprob))} {tile[0]} {tile[1]} "
+                 f"{bpc} {paired} {engine} 0\n")
+    sa.tune_clear()
+    assert sa.tune_load(str(f)) == 1
+
+
+def _cmp(tag, x, r, xo, Fo, l2_bar=1e-3, f_bar=1e-6):
+    l2 = float(np.linalg.norm(x - xo) / np.linalg.norm(xo))
+    dF = abs(r["finalcost"] - Fo) / abs(Fo)
+    print(f"  {tag}: ITER {r['ITER']} image relL2 {l2:.2e} objective rel {dF:.2e}", flush=True)
+    assert l2 <= l2_bar
+    assert dF <= f_bar
+
+
+@pytest.mark.parametrize("tile,bpc,paired", [((16, 16), 2, 1), ((32, 16), 2, 1), ((32, 32), 2, 1),
+                                             ((32, 16), 1, 0), ((32, 32), 1, 1)])
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_tiles_l1_tsv_match_oracle(tmp_path, preset, tile, bpc, paired):
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    kw = dict(maxiter=20000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=10.0, cinit=1e4,
+              nonneg_flag=1, rel_tol=1e-9)
+    print(f"case tile {tile} bpc {bpc} paired {paired} preset {preset} N=64 M=2000 L1+TSV", flush=True)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"], nonneg=1,
+                                  preset=preset, rel_tol=kw["rel_tol"])
+    _force(tmp_path, 64, 64, preset, 1, 2000, 1, tile, bpc, paired)
+    runs = sa.tune_runs()
+    x, r, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                              preset=preset, autotune=True, **kw)
+    assert sa.tune_runs() == runs  # the loaded entry was used, nothing was timed
+    assert info["engine_used"] == sa.ENGINE_PERSISTENT
+    if paired == 0:
+        assert info["trial_rounds"] == info["total_trials"]
+    _cmp("tile", x, r, o["x"], o["result"]["finalcost"])
+    sa.tune_clear()
+
+
+@pytest.mark.parametrize("tile", [(32, 16), (32, 32)])
+def test_tiles_l1_tv_match_oracle(tmp_path, tile):
+    P = synth.make_nufft_problem(N=64, M=2500, seed=1003, amax=0.35, kind="tv", coverage="disc")
+    kw = dict(maxiter=3000, eps=0.0, lambda_l1=0.5, lambda_tv=5.0, lambda_tsv=0.0, cinit=1e4, rel_tol=1e-8)
+    print(f"case tile {tile} nufft L1+TV C-24 64x64 M=2500 fgp=50", flush=True)
+    o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
+                                  kw["eps"], kw["lambda_l1"], kw["lambda_tv"], 0.0, kw["cinit"], nonneg=1,
+                                  preset=oracle.PRESET_C24, fgp_iter=50, rel_tol=kw["rel_tol"])
+    _force(tmp_path, 64, 64, sa.PRESET_C24, 2, 2500, 1, tile, 2, 0)
+    x, r, _ = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                           nonneg_flag=1, preset=sa.PRESET_C24, fgp_iter=50, autotune=True, **kw)
+    _cmp("tv tile", x, r, o["x"], o["result"]["finalcost"])
+    sa.tune_clear()
+
+
+def test_batch_with_forced_tile_matches_single_solves(tmp_path):
+    P = synth.make_nufft_problem(N=64, M=1500, seed=1004, amax=0.35)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
+                maxiter=20000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
+    print("case batch of 3 (64^2, M=1500) with 32x32 tiles against single solves", flush=True)
+    singles = [sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
+                                            maxiter=20000, eps=0.0, lambda_l1=1.0, lambda_tsv=p["lambda_tsv"],
+                                            cinit=1e4, preset=sa.PRESET_CXX12, rel_tol=1e-9) for p in probs]
+    _force(tmp_path, 64, 64, sa.PRESET_CXX12, 1, 1500, 3, (32, 32))
+    xs, rs, _ = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-9, autotune=True)
+    for (xs1, rs1, _), xb, rb in zip(singles, xs, rs):
+        _cmp("batch tile 32x32", xb, rb, xs1, rs1["finalcost"])
+    sa.tune_clear()
+
+
+def test_autotune_benchmark_workload():
+    P = synth.make_nufft_problem(N=256, M=20000, seed=1002)
+    args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 256, 256)
+    sa.tune_clear()
+    runs = sa.tune_runs()
+    print("case autotune 256^2 M=20000 12 taps L1+TSV, 30 iterations per candidate", flush=True)
+    best, cand = sa.autotune_nufft(*args, lambda_l1=1.0, lambda_tsv=100.0, preset=sa.PRESET_CXX12, iters=30)
+    for c in cand:
+        print(f"  tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} paired {c['paired']}: "
+              f"{c['us_per_iter']:.1f} us/iteration", flush=True)
+    print(f"  chosen: {best}", flush=True)
+    assert len(cand) == 12 and sa.tune_runs() - runs == 24
+    assert best["us_per_iter"] == min(c["us_per_iter"] for c in cand)
+    assert (best["NX"], best["NY"], best["preset"], best["reg"], best["m_log2"]) == (256, 256, sa.PRESET_CXX12, 1, 14)
+    assert len(sa.tune_table()) == 1
+    runs = sa.tune_runs()
+    x, r, info = sa.mfista_imaging_core_nufft(*args, maxiter=50000, eps=0.0, lambda_l1=1.0, lambda_tsv=100.0,
+                                              cinit=1e4, preset=sa.PRESET_CXX12, rel_tol=1e-6, autotune=True)
+    assert sa.tune_runs() == runs
+    print(f"  tuned solve: ITER {r['ITER']} {info['kernel_ms']:.1f} ms", flush=True)
+    o = oracle.imaging_core_nufft(*args[:5], 256, 256, r["ITER"], 0.0, 1.0, 0.0, 100.0, 1e4, nonneg=1,
+                                  preset=oracle.PRESET_CXX12, rel_tol=0.0)
+    _cmp("tuned vs oracle at the same iteration count", x, r, o["x"], o["result"]["finalcost"])
+    sa.tune_clear()
+
+
+def test_autotune_lambda_grid_batch():
+    P = synth.make_nufft_problem(N=128, M=8000, seed=1004)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=128, NY=128,
+                maxiter=300, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0)
+    probs = [dict(base, lambda_l1=10.0 ** a, lambda_tsv=10.0 ** b)
+             for a in np.linspace(-2, 3, 6) for b in np.linspace(0, 5, 6)]
+    sa.tune_clear()
+    print("case autotune lambda grid 6x6 128^2 M=8000", flush=True)
+    best, cand = sa.autotune_nufft_batch(probs, preset=sa.PRESET_CXX12, iters=30)
+    for c in cand:
+        print(f"  tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} paired {c['paired']}: "
+              f"{c['us_per_iter']:.1f} us/iteration", flush=True)
+    print(f"  chosen: {best}", flush=True)
+    assert best["nprob_log2"] == 5 and len(cand) == 15
+    xa, ra, ia = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12)
+    xt, rt, it = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, autotune=True)
+    print(f"  300 iterations: default {ia['kernel_ms']:.1f} ms, tuned {it['kernel_ms']:.1f} ms", flush=True)
+    for a, b, c, d in zip(xa, ra, xt, rt):
+        assert b["ITER"] == d["ITER"] == 300
+        _cmp("tuned batch vs default batch", c, d, a, b["finalcost"])
+    sa.tune_clear()
