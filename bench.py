@@ -406,13 +406,40 @@ def workload_lambda128(sa, synth, args):
                 maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0)
     probs = [dict(base, lambda_l1=10.0 ** a, lambda_tsv=10.0 ** b)
              for a in np.linspace(-2, 3, 6) for b in np.linspace(0, 5, 6)]
+    eng = sa.ENGINE_STREAM if args.engine == "stream" else sa.ENGINE_PERSISTENT
     for _ in range(args.warmup):
-        sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+        sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6, engine=eng)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6)
+        xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6, engine=eng)
     dt = (time.perf_counter() - t0) / args.steps
-    return {"workload": "lambda_grid_6x6_128_m8000_12tap", "ms_per_batch": round(dt * 1e3, 2),
+    return {"workload": "lambda_grid_6x6_128_m8000_12tap", "engine": args.engine, "ms_per_batch": round(dt * 1e3, 2),
+            "solves_per_s": round(len(probs) / dt, 3), "kernel_ms": round(info["kernel_ms"], 3),
+            "iterations": [r["ITER"] for r in rs]}
+
+
+def workload_batch256(sa, synth, args):
+    """`args.nprob` held-out 256^2 problems (random ring + Gaussian skies,
+    M = 20000 arcs each, seeds from 9000; lambda_l1 = 1, lambda_tsv = 100,
+    12 taps, 1e-6 relative rule) as one batch on the engine `args.engine`."""
+    import numpy as np
+
+    from sparseimaging_amd import learn
+
+    rng = np.random.default_rng(9000)
+    probs = []
+    for _ in range(args.nprob):
+        P = learn.make_problem(256, 20000, rng)
+        probs.append(dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=256,
+                          NY=256, lambda_l1=1.0, lambda_tsv=100.0, maxiter=50000, eps=0.0, cinit=1e4, nonneg_flag=1))
+    eng = sa.ENGINE_STREAM if args.engine == "stream" else sa.ENGINE_PERSISTENT
+    for _ in range(args.warmup):
+        sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6, engine=eng)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, rel_tol=1e-6, engine=eng)
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"workload": f"batch_{args.nprob}x256_m20000_12tap", "engine": args.engine, "ms_per_batch": round(dt * 1e3, 2),
             "solves_per_s": round(len(probs) / dt, 3), "kernel_ms": round(info["kernel_ms"], 3),
             "iterations": [r["ITER"] for r in rs]}
 
@@ -431,9 +458,11 @@ def main():
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--warm", type=int, default=8,
                     help="held-out 256^2 problems for the learned warm start (0 = skip)")
-    ap.add_argument("--workload", choices=["headline", "tv512", "lambda128"], default="headline",
-                    help="headline: the bench line; tv512 / lambda128: time only the 512^2 L1+TV solve or the "
-                         "6x6 lambda-grid batch (profiling runs) and print their own JSON line")
+    ap.add_argument("--workload", choices=["headline", "tv512", "lambda128", "batch256"], default="headline",
+                    help="headline: the bench line; tv512 / lambda128 / batch256: time only the 512^2 L1+TV solve, "
+                         "the 6x6 lambda-grid batch or a batch of --nprob 256^2 problems (profiling runs; the "
+                         "batches on --engine) and print their own JSON line")
+    ap.add_argument("--nprob", type=int, default=16, help="problems of the batch256 workload")
     ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
                     help="also time the batched workloads (6x6 lambda grid at 128^2, 8 lambda_tsv values "
                          "at 256^2, 32-frame movie) against the same problems solved one by one")
@@ -451,7 +480,7 @@ def main():
     from sparseimaging_amd import synth
 
     if args.workload != "headline":
-        fn = workload_tv512 if args.workload == "tv512" else workload_lambda128
+        fn = {"tv512": workload_tv512, "lambda128": workload_lambda128, "batch256": workload_batch256}[args.workload]
         print(json.dumps(fn(sa, synth, args)), flush=True)
         return
 

@@ -191,6 +191,7 @@ int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out) {
   L.tsv_z = 3 * L.n_degrid;
   L.prox = L.tsv_z + L.n_elem;
   L.total = L.prox + kProxW * L.n_elem;
+  L.n_chiz = L.n_degrid;
   const bool delta = op.has_delta();
   const size_t rb = (size_t)n * sizeof(R);
   if (C.x.ensure(rb) || C.z.ensure(rb) || C.xnew.ensure(rb) || C.g.ensure(rb) || C.dstep.ensure(rb) ||

@@ -166,7 +166,7 @@ __device__ inline void dev_control(DevState* S, double* cost, const double* __re
   const bool prologue = (S->gate_prologue == 0);
   double chi_z = 0, tsv_z = 0;
   if (prologue) {
-    chi_z = block_reduce_range(part + L.chi_z, L.n_degrid, 1, red);
+    chi_z = block_reduce_range(part + L.chi_z, L.n_chiz, 1, red);
     tsv_z = block_reduce_range(part + L.tsv_z, L.n_elem, 1, red);
   }
   const double chi_x = block_reduce_range(part + L.chi_x, L.n_degrid, 1, red);

@@ -40,6 +40,7 @@ constexpr int kProxW = 6;  // per-block partials of the trial kernel
 struct PartialLayout {
   int chi_z, chi_x, lip, tsv_z, prox;  // prox: kProxW values per block
   int n_degrid, n_elem, total;
+  int n_chiz;  // partials of the chi^2 at z (n_degrid unless a separate kernel writes them)
 };
 
 template <typename R>

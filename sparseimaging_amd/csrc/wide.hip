@@ -2,10 +2,14 @@
 //
 // The loop is the stream engine's (driver.hip mfista_run, restating
 // c/mfista_nufft_lib.c:557-787): every step is one backtracking trial,
-// preceded by the iteration prologue (forward transform at z in FP64, FP32
-// adjoint of its residual, TSV gradient) when the previous trial was accepted,
-// and a one-workgroup FP64 control step per problem sets the gates of the next
-// step.  What changes is the launch shape: every kernel of a step covers all
+// preceded by the iteration prologue (FP32 adjoint of the residual at z, TSV
+// gradient) when the previous trial was accepted, and a one-workgroup FP64
+// control step per problem sets the gates of the next step.  As in the
+// persistent engine, the residual at z needs no forward transform: the
+// operator is linear, so with FP64 weighted models m_x = W A x and
+// m_xn = W A xnew, W A z = a m_x + b m_xn for z = a x + b xnew; each trial
+// computes W A d for its step d = xnew - z with an FP64 forward transform and
+// m_xn = m_z + W A d.  What changes is the launch shape: every kernel of a step covers all
 // problems of the batch at once (blockIdx.y = problem, blockIdx.x = the
 // stage's task within it), so a stage of 32 problems at 256^2 fills the GPU
 // with thousands of workgroups instead of a hundred, and the kernels keep
@@ -26,6 +30,7 @@
 
 #include "driver.h"
 #include "mfista_dev.h"
+#include "persistent_body.h"
 #include "solver.h"
 #include "stages.h"
 
@@ -38,12 +43,12 @@ struct WProb {
   VisParams<float> pf;
   const int4* chunks;  // gridding chunk list of the set
   const unsigned* ent;
-  int nchunks, n_degrid;
+  int nchunks, n_degrid, M;
   PartialLayout L;
   double *x, *z, *xnew, *gr, *d;  // FP64 iterates, gradient and step
-  double2 *T64, *Gf64;            // FP64 forward transform (of z, d or the start)
-  double2* rz64;                  // FP64 weighted residual at z
-  float2* rz32;                   // its FP32 copy (gridding input)
+  double2 *T64, *Gf64;            // FP64 forward transform (of the step or the start)
+  double2 *mx, *mxn, *mz;         // FP64 weighted models W A x, W A xnew, W A z (sorted order)
+  float2* rz32;                   // FP32 weighted residual W V - W A z (gridding input)
   float2 *Hg, *U;                 // FP32 adjoint grid (kept zero between adjoints) and column pass
   double *part, *stat, *cost;
   DevState* S;
@@ -59,26 +64,25 @@ struct WShared {
   const float2 *twx32, *twy32;
   const double *e4x64, *e4y64;
   const float *e4x32, *e4y32;
-  int NX, NY, Lx, Ly, MSP;
+  int NX, NY, Lx, Ly, MSP, TX, TY;
   int nseq_r64, nseq_c64, nseq_ci32, nseq_c2r32;
   float adj_scale;  // 1 / (8 NX NY)
   int* ndone;       // problems finished
 };
 
-// which transform a forward kernel computes
-enum WFwd { kWInit = 0, kWProlog = 1, kWTrial = 2 };
+// which transform a forward kernel computes: of the start (no gate) or of a
+// trial's step (gate_trial)
+enum WFwd { kWInit = 0, kWTrial = 2 };
 
 __device__ inline bool w_closed(const DevState* S, int which) {
-  if (which == kWProlog) return S->gate_prologue != 0;
-  if (which == kWTrial) return S->gate_trial != 0;
-  return false;
+  return which == kWTrial && S->gate_trial != 0;
 }
 
 __global__ void __launch_bounds__(MF_BLOCK) kw_rowfft64(const WProb* __restrict__ W, WShared sh, int which) {
   const WProb& p = W[blockIdx.y];
   if (w_closed(p.S, which)) return;
   extern __shared__ double4 smem_d4[];
-  const double* x = (which == kWInit) ? p.x : (which == kWProlog ? p.z : p.d);
+  const double* x = (which == kWInit) ? p.x : p.d;
   const int NY = sh.NY;
   auto load = [&](int i, int j) -> double { return x[(size_t)i * NY + j] * sh.e4x64[i] * sh.e4y64[j]; };
   st_rowfft_fwd<double>(blockIdx.x, sh.nseq_r64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64, p.T64,
@@ -93,42 +97,88 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_colfft64(const WProb* __restrict_
                         sh.Px, sh.twx64, reinterpret_cast<double2*>(smem_d4));
 }
 
-// init: chi^2 partials at the start (L.chi_x); prologue: residual at z (FP64
-// and FP32 copies) and chi^2 partials (L.chi_z); trial: step mode, chi^2 at
-// xnew (L.chi_x) and |W A d|^2 (L.lip)
+// FP64 degridding into the weighted models.
+//   init:  t = W A x0; m_x = t; chi^2 partials of the start (L.chi_x)
+//   trial: t = W A d; m_xn = m_z + t; partials of |W V - m_z - t|^2 (chi^2
+//          at xnew, L.chi_x) and of |t|^2 (L.lip)
 template <int MSPC>
 __global__ void __launch_bounds__(MF_BLOCK) kw_degrid64(const WProb* __restrict__ W, WShared sh, int which) {
   const WProb& p = W[blockIdx.y];
   if ((int)blockIdx.x >= p.n_degrid || w_closed(p.S, which)) return;
   __shared__ double red[32];
-  double v[2];
-  if (which == kWTrial)
-    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, nullptr, nullptr, p.rz64, 1, nullptr, nullptr,
-                            nullptr, v);
-  else if (which == kWProlog)
-    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, p.rz64, p.rz32, nullptr, 0, nullptr, nullptr,
-                            nullptr, v);
-  else
-    st_degrid<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
-                            nullptr, v);
+  double v[2] = {0.0, 0.0};
+  int s;
+  double2 F;
+  if (degrid_core<double, MSPC>(blockIdx.x, p.g, p.pd, p.Gf64, s, F)) {
+    const double w = p.pd.w[s];
+    const double2 V = p.pd.vis[s];
+    const double tx = F.x * w, ty = F.y * w;
+    if (which == kWTrial) {
+      const double2 mz = p.mz[s];
+      const double rx = V.x * w - mz.x - tx, ry = V.y * w - mz.y - ty;
+      v[0] = rx * rx + ry * ry;
+      v[1] = tx * tx + ty * ty;
+      p.mxn[s] = make_double2(mz.x + tx, mz.y + ty);
+    } else {
+      const double rx = V.x * w - tx, ry = V.y * w - ty;
+      v[0] = rx * rx + ry * ry;
+      p.mx[s] = make_double2(tx, ty);
+    }
+  }
   block_sum<2>(v, red);
   if (threadIdx.x == 0) {
-    if (which == kWTrial) {
-      p.part[p.L.chi_x + blockIdx.x] = v[0];
-      p.part[p.L.lip + blockIdx.x] = v[1];
-    } else {
-      p.part[(which == kWProlog ? p.L.chi_z : p.L.chi_x) + blockIdx.x] = v[0];
-    }
+    p.part[p.L.chi_x + blockIdx.x] = v[0];
+    if (which == kWTrial) p.part[p.L.lip + blockIdx.x] = v[1];
   }
 }
 
+// Momentum in the visibility domain, after the control step of an iteration
+// that ends (gate_momentum open), or at the start (init: z = x):
+// m_z = a m_x + b m_xn, m_x <- m_xn on acceptance; the FP32 residual
+// W V - m_z for the next adjoint and the partials of its chi^2 (L.chi_z).
+__global__ void __launch_bounds__(MF_BLOCK) kw_vis_momentum(const WProb* __restrict__ W, int init) {
+  const WProb& p = W[blockIdx.y];
+  const DevState* S = p.S;
+  const int nvb = (p.M + MF_BLOCK - 1) / MF_BLOCK;
+  if ((int)blockIdx.x >= nvb || (!init && S->gate_momentum)) return;
+  __shared__ double red[16];
+  const int s = blockIdx.x * MF_BLOCK + threadIdx.x;
+  double v[1] = {0.0};
+  if (s < p.M) {
+    double2 mz;
+    if (init) {
+      mz = p.mx[s];
+    } else {
+      const double a = S->mom_a, b = S->mom_b;
+      const double2 mx = p.mx[s], mn = p.mxn[s];
+      mz = make_double2(a * mx.x + b * mn.x, a * mx.y + b * mn.y);
+      if (S->accept_x) p.mx[s] = mn;
+    }
+    p.mz[s] = mz;
+    const double w = p.pd.w[s];
+    const double2 V = p.pd.vis[s];
+    const double rx = V.x * w - mz.x, ry = V.y * w - mz.y;
+    v[0] = rx * rx + ry * ry;
+    p.rz32[s] = make_float2((float)rx, (float)ry);
+  }
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) p.part[p.L.chi_z + blockIdx.x] = v[0];
+}
+
+// Gridding of the FP32 residual: workgroup x takes chunks [kGB x, kGB x + kGB)
+// of the problem's tile-ordered chunk list and gathers them with the
+// persistent engine's gatherer (grid_chunks: each thread owns CPT cells of
+// the tile and sums in registers; a tile whose chunks all fall in this
+// range is stored, others are flushed with atomics).
+template <int CPT>
 __global__ void __launch_bounds__(MF_BLOCK) kw_grid32(const WProb* __restrict__ W, WShared sh) {
   const WProb& p = W[blockIdx.y];
-  if ((int)blockIdx.x >= p.nchunks || p.S->gate_prologue) return;
+  const int cb = blockIdx.x * kGB;
+  if (cb >= p.nchunks || p.S->gate_prologue) return;
   extern __shared__ double4 smem_d4[];
   const float2* rz = p.rz32;
-  st_grid<float>(blockIdx.x, p.g, p.pf, [&](int s) { return rz[s]; }, p.chunks, p.ent, p.Hg,
-                 reinterpret_cast<float2*>(smem_d4));
+  grid_chunks<CPT>(p.g, p.pf, [&](int s) { return rz[s]; }, p.chunks, cb, min(cb + kGB, p.nchunks), p.ent,
+                   p.Hg, reinterpret_cast<char*>(smem_d4));
 }
 
 __global__ void __launch_bounds__(MF_BLOCK) kw_colinv32(const WProb* __restrict__ W, WShared sh) {
@@ -210,7 +260,7 @@ static int nseq_for(int L, size_t csize, int nseq_total) {
 }
 
 struct WLaunch {
-  dim3 rowf, colf, deg, grid, colinv, c2r, elem, one;
+  dim3 rowf, colf, deg, grid, colinv, c2r, elem, one, vis;
   size_t sh_rowf, sh_colf, sh_grid, sh_colinv, sh_c2r;
 };
 
@@ -224,10 +274,16 @@ static void launch_forward(const WLaunch& L, const WProb* W, const WShared& sh, 
     hipLaunchKernelGGL(kw_degrid64<12>, L.deg, dim3(MF_BLOCK), 0, st, W, sh, which);
 }
 
-// one trial step of every problem (the stream engine's step)
+// one trial step of every problem (the stream engine's step, with the
+// residual at z from the models)
 static void launch_step(const WLaunch& L, const WProb* W, const WShared& sh, int MSP, hipStream_t st) {
-  launch_forward(L, W, sh, kWProlog, MSP, st);
-  hipLaunchKernelGGL(kw_grid32, L.grid, dim3(MF_BLOCK), L.sh_grid, st, W, sh);
+  const int cpt = (sh.TX * sh.TY) / MF_BLOCK;
+  if (cpt == 1)
+    hipLaunchKernelGGL(kw_grid32<1>, L.grid, dim3(MF_BLOCK), L.sh_grid, st, W, sh);
+  else if (cpt == 2)
+    hipLaunchKernelGGL(kw_grid32<2>, L.grid, dim3(MF_BLOCK), L.sh_grid, st, W, sh);
+  else
+    hipLaunchKernelGGL(kw_grid32<4>, L.grid, dim3(MF_BLOCK), L.sh_grid, st, W, sh);
   hipLaunchKernelGGL(kw_colinv32, L.colinv, dim3(MF_BLOCK), L.sh_colinv, st, W, sh);
   hipLaunchKernelGGL(kw_c2r32, L.c2r, dim3(MF_BLOCK), L.sh_c2r, st, W, sh);
   hipLaunchKernelGGL(kw_tsv_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
@@ -235,6 +291,7 @@ static void launch_step(const WLaunch& L, const WProb* W, const WShared& sh, int
   launch_forward(L, W, sh, kWTrial, MSP, st);
   hipLaunchKernelGGL(kw_control, L.one, dim3(MF_BLOCK), 0, st, W, sh.ndone);
   hipLaunchKernelGGL(kw_momentum, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_vis_momentum, L.vis, dim3(MF_BLOCK), 0, st, W, 0);
 }
 
 int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
@@ -269,6 +326,8 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   sh.Lx = g0.Lx;
   sh.Ly = g0.Ly;
   sh.MSP = MSP;
+  sh.TX = g0.TX;
+  sh.TY = g0.TY;
   sh.nseq_r64 = nseq_for(g0.Ly, sizeof(double2), (NX + 1) / 2);
   sh.nseq_c64 = nseq_for(g0.Lx, sizeof(double2), nky);
   sh.nseq_ci32 = nseq_for(g0.Lx, sizeof(float2), nky);
@@ -279,7 +338,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   std::vector<WProb> hw(B);
   const int n_elem = elem_blocks(n);
   const size_t rb = (size_t)n * 8;
-  int max_deg = 1, max_chunks = 1;
+  int max_deg = 1, max_chunks = 1, max_vb = 1;
   for (int b = 0; b < B; b++) {
     NufftState& N = *sets[b];
     WideWork& Wk = C.wwork[b];
@@ -294,10 +353,15 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     L.tsv_z = 3 * ndeg;
     L.prox = L.tsv_z + n_elem;
     L.total = L.prox + kProxW * n_elem;
+    // chi^2 at z comes from the visibility-domain momentum, one partial per
+    // 256 visibilities, after the other regions
+    L.n_chiz = (N.M + MF_BLOCK - 1) / MF_BLOCK;
+    L.chi_z = L.total;
+    L.total += L.n_chiz;
     if (Wk.x.ensure(rb) || Wk.z.ensure(rb) || Wk.xnew.ensure(rb) || Wk.gr.ensure(rb) || Wk.d.ensure(rb) ||
         Wk.T64.ensure((size_t)nky * NX * sizeof(double2)) ||
         Wk.Gf64.ensure(((size_t)NY + MSP) * g0.Lx * sizeof(double2)) ||
-        Wk.rz64.ensure((size_t)N.M * sizeof(double2)) || Wk.rz32.ensure((size_t)N.M * sizeof(float2)) ||
+        Wk.mod.ensure((size_t)3 * N.M * sizeof(double2)) || Wk.rz32.ensure((size_t)N.M * sizeof(float2)) ||
         Wk.Hg.ensure((size_t)nky * g0.Lx * sizeof(float2)) || Wk.U.ensure((size_t)NX * nky * sizeof(float2)) ||
         Wk.part.ensure((size_t)L.total * 8) || Wk.stat.ensure((size_t)4 * n_elem * 8) ||
         Wk.state.ensure(sizeof(DevState)) || Wk.cost.ensure((size_t)(R.maxiter > 0 ? R.maxiter : 1) * 8))
@@ -334,7 +398,10 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     w.d = (double*)Wk.d.p;
     w.T64 = (double2*)Wk.T64.p;
     w.Gf64 = (double2*)Wk.Gf64.p;
-    w.rz64 = (double2*)Wk.rz64.p;
+    w.mx = (double2*)Wk.mod.p;
+    w.mxn = w.mx + N.M;
+    w.mz = w.mxn + N.M;
+    w.M = N.M;
     w.rz32 = (float2*)Wk.rz32.p;
     w.Hg = (float2*)Wk.Hg.p;
     w.U = (float2*)Wk.U.p;
@@ -346,6 +413,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     w.lambda_tsv = R.lambda_tsv;
     max_deg = std::max(max_deg, ndeg);
     max_chunks = std::max(max_chunks, N.nchunks);
+    max_vb = std::max(max_vb, L.n_chiz);
   }
   if (C.wdesc.ensure((size_t)B * sizeof(WProb)) || C.wdone.ensure(sizeof(int))) return MFISTA_E_HIP;
   MF_CHECK(hipMemcpyAsync(C.wdesc.p, hw.data(), (size_t)B * sizeof(WProb), hipMemcpyHostToDevice, st));
@@ -358,16 +426,19 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   L.colf = dim3((nky + sh.nseq_c64 - 1) / sh.nseq_c64, B);
   L.sh_colf = 2 * (size_t)sh.nseq_c64 * g0.Lx * sizeof(double2);
   L.deg = dim3(max_deg, B);
-  L.grid = dim3(max_chunks, B);
-  L.sh_grid = 4 * (size_t)g0.TX * g0.TY * sizeof(float2);
+  L.grid = dim3((max_chunks + kGB - 1) / kGB, B);
+  L.sh_grid = grid_lds_bytes(g0.TX, g0.TY);
   L.colinv = dim3((nky + sh.nseq_ci32 - 1) / sh.nseq_ci32, B);
   L.sh_colinv = 2 * (size_t)sh.nseq_ci32 * g0.Lx * sizeof(float2);
   L.c2r = dim3((NX + 2 * sh.nseq_c2r32 - 1) / (2 * sh.nseq_c2r32), B);
   L.sh_c2r = 2 * (size_t)sh.nseq_c2r32 * g0.Ly * sizeof(float2);
   L.elem = dim3(n_elem, B);
   L.one = dim3(1, B);
-  // start: cost of xinit (chi^2, |x|_1, TSV) and the initial control state
+  L.vis = dim3(max_vb, B);
+  // start: the model of xinit, its cost (chi^2, |x|_1, TSV), z = x in both
+  // domains, and the initial control state
   launch_forward(L, W, sh, kWInit, MSP, st);
+  hipLaunchKernelGGL(kw_vis_momentum, L.vis, dim3(MF_BLOCK), 0, st, W, 1);
   hipLaunchKernelGGL(kw_stats, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   hipLaunchKernelGGL(kw_init_control, L.one, dim3(MF_BLOCK), 0, st, W, n_elem);
   MF_CHECK(hipGetLastError());
