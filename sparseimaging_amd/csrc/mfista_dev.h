@@ -152,7 +152,7 @@ __device__ inline double block_reduce_range(const double* p, int cnt, int stride
 }
 
 // one workgroup: reduces a trial's partials, takes the backtracking and
-// stopping decisions and sets the gates of the next step
+// stopping decisions and sets the gates of the next step (red: >= 144 doubles)
 __device__ inline void dev_control(DevState* S, double* cost, const double* __restrict__ part, PartialLayout L,
                                    double* red) {
   if (S->done) {
@@ -164,19 +164,34 @@ __device__ inline void dev_control(DevState* S, double* cost, const double* __re
     return;
   }
   const bool prologue = (S->gate_prologue == 0);
-  double chi_z = 0, tsv_z = 0;
-  if (prologue) {
-    chi_z = block_reduce_range(part + L.chi_z, L.n_chiz, 1, red);
-    tsv_z = block_reduce_range(part + L.tsv_z, L.n_elem, 1, red);
+  // all nine sums in one pass: thread t adds entries t, t + 256, ... of each
+  // range in order, then one fixed-order block reduction -- the same sums,
+  // bit for bit, as nine separate block_reduce_range calls
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  {
+    const bool lip_on = S->lip_mode != 0;
+    const int nmax = max(max(L.n_chiz, L.n_degrid), L.n_elem);
+    for (int i = threadIdx.x; i < nmax; i += blockDim.x) {
+      if (prologue && i < L.n_chiz) v[0] += part[L.chi_z + i];
+      if (prologue && i < L.n_elem) v[1] += part[L.tsv_z + i];
+      if (i < L.n_degrid) {
+        v[2] += part[L.chi_x + i];
+        if (lip_on) v[3] += part[L.lip + i];
+      }
+      if (i < L.n_elem) {
+        const double* q = part + L.prox + (size_t)kProxW * i;
+        v[4] += q[0];
+        v[5] += q[1];
+        v[6] += q[2];
+        v[7] += q[3];
+        v[8] += q[4];
+      }
+    }
+    block_sum<9>(v, red);
   }
-  const double chi_x = block_reduce_range(part + L.chi_x, L.n_degrid, 1, red);
-  const double lip = S->lip_mode ? block_reduce_range(part + L.lip, L.n_degrid, 1, red) : 0.0;
-  const double s_dg = block_reduce_range(part + L.prox + 0, L.n_elem, kProxW, red);
-  const double s_d2 = block_reduce_range(part + L.prox + 1, L.n_elem, kProxW, red);
-  const double l1n = block_reduce_range(part + L.prox + 2, L.n_elem, kProxW, red);
-  const double tsvn = block_reduce_range(part + L.prox + 3, L.n_elem, kProxW, red);
-  const double tsvd = block_reduce_range(part + L.prox + 4, L.n_elem, kProxW, red);
   if (threadIdx.x != 0) return;
+  const double chi_z = v[0], tsv_z = v[1], chi_x = v[2], lip = v[3];
+  const double s_dg = v[4], s_d2 = v[5], l1n = v[6], tsvn = v[7], tsvd = v[8];
   const bool tsv = S->lambda_tsv > 0.0;
   const int it = S->iter;
   if (prologue) {

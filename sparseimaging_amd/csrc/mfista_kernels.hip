@@ -52,7 +52,7 @@ __global__ void k_cast(const Rs* __restrict__ a, Rd* __restrict__ b, int n, cons
 
 __global__ void __launch_bounds__(MF_BLOCK)
 k_control(DevState* S, double* cost, const double* __restrict__ part, PartialLayout L) {
-  __shared__ double red[16];
+  __shared__ double red[16 * 9];
   dev_control(S, cost, part, L, red);
 }
 

@@ -136,13 +136,12 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_degrid64(const WProb* __restrict_
 // that ends (gate_momentum open), or at the start (init: z = x):
 // m_z = a m_x + b m_xn, m_x <- m_xn on acceptance; the FP32 residual
 // W V - m_z for the next adjoint and the partials of its chi^2 (L.chi_z).
-__global__ void __launch_bounds__(MF_BLOCK) kw_vis_momentum(const WProb* __restrict__ W, int init) {
-  const WProb& p = W[blockIdx.y];
+__device__ inline void vis_momentum(const WProb& p, int blk, int init) {
   const DevState* S = p.S;
   const int nvb = (p.M + MF_BLOCK - 1) / MF_BLOCK;
-  if ((int)blockIdx.x >= nvb || (!init && S->gate_momentum)) return;
+  if (blk >= nvb || (!init && S->gate_momentum)) return;
   __shared__ double red[16];
-  const int s = blockIdx.x * MF_BLOCK + threadIdx.x;
+  const int s = blk * MF_BLOCK + threadIdx.x;
   double v[1] = {0.0};
   if (s < p.M) {
     double2 mz;
@@ -162,14 +161,149 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_vis_momentum(const WProb* __restr
     p.rz32[s] = make_float2((float)rx, (float)ry);
   }
   block_sum<1>(v, red);
-  if (threadIdx.x == 0) p.part[p.L.chi_z + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) p.part[p.L.chi_z + blk] = v[0];
+}
+
+__global__ void __launch_bounds__(MF_BLOCK) kw_vis_momentum(const WProb* __restrict__ W, int init) {
+  vis_momentum(W[blockIdx.y], blockIdx.x, init);
+}
+
+// Gridding with separable tap weights.  The Gaussian weight of tap (l, l') is
+// exp(l a_x - l^2 t3x) exp(l' a_y - l'^2 t3y), so for one entry and one tile
+// every column c has one x weight and every row r one y weight (zero outside
+// the 2 MSP footprint).  Per batch of kSepEB entries, the first half of the
+// workgroup tabulates the column weights and the deposit of entry q, the
+// second half its row weights (TX + TY exponentials per entry instead of
+// one per covered cell); each thread then adds wx[c] wy[r] dep to its cells.
+// Tile bookkeeping and flushes are grid_chunks'.
+constexpr int kSepChunks = 4;                         // chunks per batch
+constexpr int kSepEB = kSepChunks * kGridChunk;       // entries per batch (128)
+
+__host__ __device__ inline size_t grid_sep_lds_bytes(int TX, int TY) {
+  return (size_t)kSepEB * (TX + TY + 2) * sizeof(float) + kSepChunks * sizeof(int4);
+}
+
+template <int CPT, typename Resid>
+__device__ inline void grid_sep(const GridGeom& g, const VisParams<float>& vp, Resid resid,
+                                const int4* __restrict__ chunks, int cb, int ce, const unsigned* __restrict__ ent,
+                                float2* __restrict__ Hg, char* lds) {
+  const int TX = g.TX, TY = g.TY, SW = TX + TY + 2;
+  float* tab = reinterpret_cast<float*>(lds);
+  int4* cmeta = reinterpret_cast<int4*>(tab + (size_t)kSepEB * SW);
+  const int NX = g.NX, MSP = g.MSP;
+  const double t3x = g.t3x, t3y = g.t3y;
+  constexpr double kL2E = 1.4426950408889634074;  // log2(e)
+  int rr[CPT], cc[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) {
+    const int cell = threadIdx.x + c * MF_BLOCK;
+    rr[c] = cell / TX;
+    cc[c] = cell - rr[c] * TX;
+  }
+  float2 acc[CPT];
+#pragma unroll
+  for (int c = 0; c < CPT; c++) acc[c] = make_float2(0.f, 0.f);
+  int cur = -1, cur_first = 0, rb = 0, cbx = 0;
+  auto flush = [&](bool owned) {
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+      const int row = rb + rr[c], col = cbx + cc[c];
+      const float2 v = acc[c];
+      if (rr[c] < TY && row <= g.NY && col < g.Lx && (v.x != 0.f || v.y != 0.f)) {
+        float2* dst = Hg + (size_t)row * g.Lx + col;
+        if (owned) {
+          *dst = v;
+        } else {
+          unsafeAtomicAdd(&dst->x, v.x);
+          unsafeAtomicAdd(&dst->y, v.y);
+        }
+      }
+      acc[c] = make_float2(0.f, 0.f);
+    }
+  };
+  for (int c0 = cb; c0 < ce; c0 += kSepChunks) {
+    const int nc = min(kSepChunks, ce - c0);
+    __syncthreads();  // the previous batch's tables are consumed
+    {
+      const int q = threadIdx.x & (kSepEB - 1), half = threadIdx.x / kSepEB, j = q / kGridChunk;
+      if (j < nc) {
+        const int4 ch = chunks[c0 + j];
+        if (half == 0 && (q % kGridChunk) == 0) cmeta[j] = ch;
+        const int e = ch.y + (q % kGridChunk);
+        if (e < ch.z) {
+          const unsigned en = ent[e];
+          const int sv = (int)(en & 0x7fffffffu);
+          const bool mir = (en >> 31) != 0;
+          const int2 m = GP(vp.mxy)[sv];
+          const float2 a = GP(vp.a)[sv];
+          const int tr = ch.x / g.ntx, tc = ch.x - tr * g.ntx;
+          const int trb = tr * TY, tcb = tc * TX;
+          // tap offsets of cell (row, col): direct l' = row - my, l = col - mx - NX;
+          // mirror l' = -row - my, l = NX - mx - col
+          const int sg = mir ? -1 : 1;
+          float* rec = tab + (size_t)q * SW;
+          if (half == 0) {
+            const int ox = mir ? NX - m.x - tcb : tcb - m.x - NX;
+            const double ax = a.x;
+            for (int c = 0; c < TX; c++) {
+              const int l = ox + sg * c;
+              rec[c] = (l > -MSP && l <= MSP) ? __builtin_amdgcn_exp2f((float)(kL2E * (l * ax - (double)l * l * t3x)))
+                                              : 0.f;
+            }
+            const float ew = GP(vp.e1)[sv] * GP(vp.w)[sv];
+            const float2 r = resid(sv);
+            rec[TX + TY] = ew * r.x;
+            rec[TX + TY + 1] = mir ? ew * r.y : -ew * r.y;
+          } else {
+            const int oy = mir ? -(trb + m.y) : trb - m.y;
+            const double ay = a.y;
+            for (int r = 0; r < TY; r++) {
+              const int l = oy + sg * r;
+              rec[TX + r] = (l > -MSP && l <= MSP)
+                                ? __builtin_amdgcn_exp2f((float)(kL2E * (l * ay - (double)l * l * t3y)))
+                                : 0.f;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int j = 0; j < nc; j++) {
+      const int4 ch = cmeta[j];
+      if (ch.x != cur) {
+        if (cur >= 0) flush(false);
+        cur = ch.x;
+        cur_first = ch.w & 1;
+        const int tr = cur / g.ntx, tc = cur - tr * g.ntx;
+        rb = tr * TY;
+        cbx = tc * TX;
+      }
+      const int q0 = j * kGridChunk, q1 = q0 + min(kGridChunk, ch.z - ch.y);
+#pragma unroll 4
+      for (int q = q0; q < q1; q++) {
+        const float* rec = tab + (size_t)q * SW;
+        const float dx = rec[TX + TY], dy = rec[TX + TY + 1];
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+          const float w = rec[cc[c]] * rec[TX + rr[c]];
+          acc[c].x = fmaf(w, dx, acc[c].x);
+          acc[c].y = fmaf(w, dy, acc[c].y);
+        }
+      }
+      if (ch.w & 2) {  // last chunk of the tile
+        flush(cur_first != 0);
+        cur = -1;
+      }
+    }
+  }
+  if (cur >= 0) flush(false);
+  __syncthreads();
 }
 
 // Gridding of the FP32 residual: workgroup x takes chunks [kGB x, kGB x + kGB)
-// of the problem's tile-ordered chunk list and gathers them with the
-// persistent engine's gatherer (grid_chunks: each thread owns CPT cells of
-// the tile and sums in registers; a tile whose chunks all fall in this
-// range is stored, others are flushed with atomics).
+// of the problem's tile-ordered chunk list (each thread owns CPT cells of the
+// tile and sums in registers; a tile whose chunks all fall in this range is
+// stored, others are flushed with atomics).
 template <int CPT>
 __global__ void __launch_bounds__(MF_BLOCK) kw_grid32(const WProb* __restrict__ W, WShared sh) {
   const WProb& p = W[blockIdx.y];
@@ -177,8 +311,8 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_grid32(const WProb* __restrict__ 
   if (cb >= p.nchunks || p.S->gate_prologue) return;
   extern __shared__ double4 smem_d4[];
   const float2* rz = p.rz32;
-  grid_chunks<CPT>(p.g, p.pf, [&](int s) { return rz[s]; }, p.chunks, cb, min(cb + kGB, p.nchunks), p.ent,
-                   p.Hg, reinterpret_cast<char*>(smem_d4));
+  grid_sep<CPT>(p.g, p.pf, [&](int s) { return rz[s]; }, p.chunks, cb, min(cb + kGB, p.nchunks), p.ent, p.Hg,
+                reinterpret_cast<char*>(smem_d4));
 }
 
 __global__ void __launch_bounds__(MF_BLOCK) kw_colinv32(const WProb* __restrict__ W, WShared sh) {
@@ -223,15 +357,20 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_prox(const WProb* __restrict__ W,
 
 __global__ void __launch_bounds__(MF_BLOCK) kw_control(const WProb* __restrict__ W, int* ndone) {
   const WProb& p = W[blockIdx.y];
-  __shared__ double red[16];
+  __shared__ double red[16 * 9];
   const int was_done = p.S->done;
   dev_control(p.S, p.cost, p.part, p.L, red);
   if (threadIdx.x == 0 && !was_done && p.S->done) atomicAdd(ndone, 1);
 }
 
-__global__ void __launch_bounds__(MF_BLOCK) kw_momentum(const WProb* __restrict__ W, WShared sh) {
+// the momentum step in both domains: blocks [0, n_elem) the pixels
+// (z = a x + b xnew, x <- xnew on acceptance), the rest the models
+__global__ void __launch_bounds__(MF_BLOCK) kw_momentum(const WProb* __restrict__ W, WShared sh, int n_elem) {
   const WProb& p = W[blockIdx.y];
-  dev_momentum<double>(blockIdx.x, p.x, p.xnew, p.z, sh.NX * sh.NY, p.S);
+  if ((int)blockIdx.x < n_elem)
+    dev_momentum<double>(blockIdx.x, p.x, p.xnew, p.z, sh.NX * sh.NY, p.S);
+  else
+    vis_momentum(p, blockIdx.x - n_elem, 0);
 }
 
 __global__ void __launch_bounds__(MF_BLOCK) kw_stats(const WProb* __restrict__ W, WShared sh) {
@@ -290,8 +429,7 @@ static void launch_step(const WLaunch& L, const WProb* W, const WShared& sh, int
   hipLaunchKernelGGL(kw_prox, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   launch_forward(L, W, sh, kWTrial, MSP, st);
   hipLaunchKernelGGL(kw_control, L.one, dim3(MF_BLOCK), 0, st, W, sh.ndone);
-  hipLaunchKernelGGL(kw_momentum, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
-  hipLaunchKernelGGL(kw_vis_momentum, L.vis, dim3(MF_BLOCK), 0, st, W, 0);
+  hipLaunchKernelGGL(kw_momentum, dim3(L.elem.x + L.vis.x, L.elem.y), dim3(MF_BLOCK), 0, st, W, sh, (int)L.elem.x);
 }
 
 int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
@@ -338,7 +476,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   std::vector<WProb> hw(B);
   const int n_elem = elem_blocks(n);
   const size_t rb = (size_t)n * 8;
-  int max_deg = 1, max_chunks = 1, max_vb = 1;
+
   for (int b = 0; b < B; b++) {
     NufftState& N = *sets[b];
     WideWork& Wk = C.wwork[b];
@@ -411,30 +549,41 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     w.S = (DevState*)Wk.state.p;
     w.box = R.box_flag ? R.box : nullptr;
     w.lambda_tsv = R.lambda_tsv;
-    max_deg = std::max(max_deg, ndeg);
-    max_chunks = std::max(max_chunks, N.nchunks);
-    max_vb = std::max(max_vb, L.n_chiz);
   }
   if (C.wdesc.ensure((size_t)B * sizeof(WProb)) || C.wdone.ensure(sizeof(int))) return MFISTA_E_HIP;
   MF_CHECK(hipMemcpyAsync(C.wdesc.p, hw.data(), (size_t)B * sizeof(WProb), hipMemcpyHostToDevice, st));
   MF_CHECK(hipMemsetAsync(C.wdone.p, 0, sizeof(int), st));
   sh.ndone = (int*)C.wdone.p;
   const WProb* W = (const WProb*)C.wdesc.p;
-  WLaunch L;
-  L.rowf = dim3((NX + 2 * sh.nseq_r64 - 1) / (2 * sh.nseq_r64), B);
-  L.sh_rowf = 2 * (size_t)sh.nseq_r64 * g0.Ly * sizeof(double2);
-  L.colf = dim3((nky + sh.nseq_c64 - 1) / sh.nseq_c64, B);
-  L.sh_colf = 2 * (size_t)sh.nseq_c64 * g0.Lx * sizeof(double2);
-  L.deg = dim3(max_deg, B);
-  L.grid = dim3((max_chunks + kGB - 1) / kGB, B);
-  L.sh_grid = grid_lds_bytes(g0.TX, g0.TY);
-  L.colinv = dim3((nky + sh.nseq_ci32 - 1) / sh.nseq_ci32, B);
-  L.sh_colinv = 2 * (size_t)sh.nseq_ci32 * g0.Lx * sizeof(float2);
-  L.c2r = dim3((NX + 2 * sh.nseq_c2r32 - 1) / (2 * sh.nseq_c2r32), B);
-  L.sh_c2r = 2 * (size_t)sh.nseq_c2r32 * g0.Ly * sizeof(float2);
-  L.elem = dim3(n_elem, B);
-  L.one = dim3(1, B);
-  L.vis = dim3(max_vb, B);
+  // launch shapes for the problems hw[idx[0..n)] (descriptors in that order)
+  std::vector<int> act(B);
+  for (int b = 0; b < B; b++) act[b] = b;
+  auto shape = [&](const std::vector<int>& idx) {
+    const int n = (int)idx.size();
+    int mdeg = 1, mch = 1, mvb = 1;
+    for (int b : idx) {
+      mdeg = std::max(mdeg, hw[b].n_degrid);
+      mch = std::max(mch, hw[b].nchunks);
+      mvb = std::max(mvb, hw[b].L.n_chiz);
+    }
+    WLaunch L;
+    L.rowf = dim3((NX + 2 * sh.nseq_r64 - 1) / (2 * sh.nseq_r64), n);
+    L.sh_rowf = 2 * (size_t)sh.nseq_r64 * g0.Ly * sizeof(double2);
+    L.colf = dim3((nky + sh.nseq_c64 - 1) / sh.nseq_c64, n);
+    L.sh_colf = 2 * (size_t)sh.nseq_c64 * g0.Lx * sizeof(double2);
+    L.deg = dim3(mdeg, n);
+    L.grid = dim3((mch + kGB - 1) / kGB, n);
+    L.sh_grid = grid_sep_lds_bytes(g0.TX, g0.TY);
+    L.colinv = dim3((nky + sh.nseq_ci32 - 1) / sh.nseq_ci32, n);
+    L.sh_colinv = 2 * (size_t)sh.nseq_ci32 * g0.Lx * sizeof(float2);
+    L.c2r = dim3((NX + 2 * sh.nseq_c2r32 - 1) / (2 * sh.nseq_c2r32), n);
+    L.sh_c2r = 2 * (size_t)sh.nseq_c2r32 * g0.Ly * sizeof(float2);
+    L.elem = dim3(n_elem, n);
+    L.one = dim3(1, n);
+    L.vis = dim3(mvb, n);
+    return L;
+  };
+  WLaunch L = shape(act);
   // start: the model of xinit, its cost (chi^2, |x|_1, TSV), z = x in both
   // domains, and the initial control state
   launch_forward(L, W, sh, kWInit, MSP, st);
@@ -442,27 +591,35 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   hipLaunchKernelGGL(kw_stats, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   hipLaunchKernelGGL(kw_init_control, L.one, dim3(MF_BLOCK), 0, st, W, n_elem);
   MF_CHECK(hipGetLastError());
-  // one step captured as a graph
+  // one step captured as a graph; re-captured for the problems still running
+  // whenever they have become half as many (a batch waits for its slowest
+  // problem, and finished problems would otherwise keep their workgroups in
+  // every launch)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  MF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  launch_step(L, W, sh, MSP, st);
-  if (hipStreamEndCapture(st, &graph) != hipSuccess) return MFISTA_E_HIP;
-  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return MFISTA_E_HIP;
-  }
+  auto capture = [&]() -> int {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    MF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    launch_step(L, W, sh, MSP, st);
+    if (hipStreamEndCapture(st, &graph) != hipSuccess) return MFISTA_E_HIP;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) return MFISTA_E_HIP;
+    return 0;
+  };
+  int rc = capture();
   int maxit = 1;
   for (const RunParams& R : P) maxit = std::max(maxit, R.maxiter);
   const long cap = (long)maxit * 64 + 100000;
   const int K = 16;  // steps between reads of the finished-problem counter
   int* hflag = C.hflags;
-  int pending = -1, rc = 0;
+  int pending = -1;
   long step = 0;
   for (hipEvent_t& e : C.ev_t)
     if (!e) MF_CHECK(hipEventCreate(&e));
   MF_CHECK(hipEventRecord(C.ev_t[0], st));
-  for (;; step++) {
+  for (; rc == 0; step++) {
     if (hipGraphLaunch(exec, st) != hipSuccess) {
       rc = MFISTA_E_HIP;
       break;
@@ -473,7 +630,28 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
       MF_CHECK(hipEventRecord(C.ev[slot], st));
       if (pending >= 0) {
         MF_CHECK(hipEventSynchronize(C.ev[pending]));
-        if (hflag[pending] >= B) break;
+        const int ndone = hflag[pending];
+        if (ndone >= B) break;
+        if ((int)act.size() >= 4 && 2 * (B - ndone) <= (int)act.size()) {
+          MF_CHECK(hipStreamSynchronize(st));
+          std::vector<int> keep;
+          std::vector<WProb> sub;
+          for (int b : act) {
+            int done = 0;
+            MF_CHECK(hipMemcpy(&done, &((DevState*)C.wwork[b].state.p)->done, sizeof(int), hipMemcpyDeviceToHost));
+            if (!done) {
+              keep.push_back(b);
+              sub.push_back(hw[b]);
+            }
+          }
+          if (keep.empty()) break;
+          act = keep;
+          MF_CHECK(hipMemcpyAsync(C.wdesc.p, sub.data(), sub.size() * sizeof(WProb), hipMemcpyHostToDevice, st));
+          L = shape(act);
+          if ((rc = capture())) break;
+          pending = -1;
+          continue;
+        }
       }
       pending = slot;
     }
@@ -485,8 +663,8 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   }
   MF_CHECK(hipEventRecord(C.ev_t[1], st));
   MF_CHECK(hipStreamSynchronize(st));
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (rc) return rc;
   float ms = 0;
   MF_CHECK(hipEventElapsedTime(&ms, C.ev_t[0], C.ev_t[1]));

@@ -32,7 +32,7 @@ import time
 import numpy as np
 
 from . import lib as _lib_loader
-from . import MfistaError, PRESET_CXX12, mfista_imaging_core_nufft, mfista_imaging_core_nufft_batch
+from . import ENGINE_PERSISTENT, MfistaError, PRESET_CXX12, mfista_imaging_core_nufft, mfista_imaging_core_nufft_batch
 from . import nufft_adjoint, nufft_forward
 
 _vp = C.c_void_p
@@ -359,11 +359,11 @@ def make_problem(N, M, rng, preset=PRESET_CXX12, noise_frac=0.01):
 
 
 def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.0, 4.0), batch=32,
-                      rel_tol=1e-6, preset=PRESET_CXX12, log=None, channels=2):
+                      rel_tol=1e-6, preset=PRESET_CXX12, log=None, channels=2, engine=ENGINE_PERSISTENT):
     """n (input, target) pairs: random skies solved by the GPU MFISTA (cold
     start, `rel_tol` relative tolerance) with lambda_tsv log-uniform; `batch`
-    problems per launch (1: single solves, the fastest at 256^2, where a solve
-    fills the GPU)."""
+    problems per launch (1: single solves on the persistent engine; more: one
+    batch per launch on `engine`, ENGINE_STREAM being the wide engine)."""
     rng = np.random.default_rng(seed)
     X = np.empty((n, N, N, channels), dtype=np.float32)
     T = np.empty((n, N, N), dtype=np.float32)
@@ -385,7 +385,7 @@ def make_training_set(n, N=256, M=20000, seed=5000, lambda_l1=1.0, log10_tsv=(0.
             args = [q.pop(k) for k in ("u_dx", "v_dy", "vis_r", "vis_i", "vis_std", "NX", "NY")]
             xs = [mfista_imaging_core_nufft(*args, preset=preset, rel_tol=rel_tol, **q)[0]]
         else:
-            xs, _, _ = mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol)
+            xs, _, _ = mfista_imaging_core_nufft_batch(probs, preset=preset, rel_tol=rel_tol, engine=engine)
         for j, xk in enumerate(xs):
             T[k0 + j] = xk / ss[j]
         if log and ((k0 // batch) % max(1, 64 // batch) == 0 or k0 + batch >= n):

@@ -25,6 +25,8 @@ def main():
     ap.add_argument("--base", type=int, default=32)
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--solve-batch", type=int, default=1, help="problems per solver launch")
+    ap.add_argument("--solve-engine", choices=("persistent", "wide"), default="persistent",
+                    help="batch engine of the training solves (--solve-batch > 1)")
     ap.add_argument("--lr-final", type=float, default=None, help="final learning rate (default lr / 10)")
     ap.add_argument("--mse", action="store_true", help="plain MSE instead of the relative squared error")
     ap.add_argument("--in-ch", type=int, default=3, choices=(2, 3),
@@ -44,8 +46,11 @@ def main():
     else:
         net = learn.UNet(base=args.base, levels=args.levels, in_ch=args.in_ch, seed=args.seed)
     t0 = time.time()
+    from . import ENGINE_PERSISTENT, ENGINE_STREAM
+
     X, T, lam = learn.make_training_set(args.n, N=args.N, M=args.M, seed=args.seed, batch=args.solve_batch,
-                                        log=log, channels=args.in_ch)
+                                        log=log, channels=args.in_ch,
+                                        engine=ENGINE_STREAM if args.solve_engine == "wide" else ENGINE_PERSISTENT)
     t_data = time.time() - t0
     t1 = time.time()
     hist = learn.train(net, X, T, epochs=args.epochs, batch=args.batch, lr=args.lr,

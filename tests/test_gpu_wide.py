@@ -130,3 +130,25 @@ def test_wide_movie_32_frames_128():
         f = int(REF[f"movie{k}_frame"])
         assert _sha(frames[f]) == str(REF[f"movie{k}_sha256"])
         _check(f"frame {f} vs oracle", xs[f], rs[f], REF[f"movie{k}_x"], float(REF[f"movie{k}_F"]))
+
+
+def test_wide_compaction_uneven_iteration_counts():
+    """Problems that finish at different steps: the wide engine re-captures
+    its step for the problems still running each time they have halved, so
+    the late problems run on smaller launches (here 8, then 4, then 2).
+    Eight problems with maxiter 1000 .. 4000 (no stopping rule), every one
+    against the oracle run for its own count.  (Earlier stops are not used:
+    at 250 iterations the objective still falls fast enough that rounding in
+    the trajectory alone moves it by ~1e-6.)"""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
+                eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    its = [1000, 1000, 1500, 1500, 2000, 2000, 3000, 4000]
+    probs = [dict(base, maxiter=m, lambda_tsv=10.0 + k) for k, m in enumerate(its)]
+    print("case wide engine compaction, 8 problems, maxiter 1000..4000", flush=True)
+    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM)
+    assert info["engine_used"] == sa.ENGINE_STREAM
+    for p, x, r in zip(probs, xs, rs):
+        assert r["ITER"] == p["maxiter"]
+        xo, Fo = _oracle(p, sa.PRESET_CXX12)
+        _check(f"maxiter {p['maxiter']}", x, r, xo.reshape(64, 64), Fo)
