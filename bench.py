@@ -281,6 +281,24 @@ def bench_warm_start(sa, nprob):
         tkl, rkl = timed_batch(warm_probs + probs, engine)
         tkl += net_s
         kept = [min(w["finalcost"], c["finalcost"]) for w, c in zip(rkl[:nprob], rkl[nprob:])]
+        if name == "wide":
+            # at equal final objective: every problem of the wide batch stops at
+            # the first iteration whose objective is at or below the one its
+            # one-by-one cold solve reached (no other stopping rule), from the
+            # cold start and from the network's start (inference charged)
+            eq = {"engine": name, "rule": "target objective = the one-by-one cold solve's final objective"}
+            for start, base_ps in (("cold", probs), ("network", warm_probs)):
+                ps = [dict(p, target_objective=c["finalcost"]) for p, c in zip(base_ps, cold_results)]
+                sa.mfista_imaging_core_nufft_batch(ps, preset=sa.PRESET_CXX12, engine=engine)  # warm-up
+                t0 = time.perf_counter()
+                _, req, _ = sa.mfista_imaging_core_nufft_batch(ps, preset=sa.PRESET_CXX12, engine=engine)
+                teq = time.perf_counter() - t0 + (net_s if start == "network" else 0.0)
+                reached = [r["finalcost"] <= c["finalcost"] * (1 + 1e-9) for r, c in zip(req, cold_results)]
+                eq[start] = {"solves_per_s": round(nprob / teq, 3), "vs_single_cold": round(tc / teq, 3),
+                             "mean_iterations": float(np.mean([r["ITER"] for r in req])),
+                             "all_reached_target": bool(all(reached)),
+                             "target_3x_met": bool(all(reached) and tc / teq >= 3.0)}
+            out["equal_objective"] = eq
         key = "batched" if name == "persistent" else "batched_wide"
         out[key] = {
             "engine": name,
@@ -389,6 +407,15 @@ def workload_tv512(sa, synth, args):
     P = synth.make_nufft_problem(N=512, M=200000, seed=1003, kind="tv", coverage="disc", amax=0.48)
     kw = dict(maxiter=500, eps=0.0, lambda_l1=0.5, lambda_tv=5.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
               preset=sa.PRESET_C24, fgp_iter=50)
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        best, cand = sa.autotune_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512, lambda_l1=0.5,
+                                       lambda_tv=5.0, preset=sa.PRESET_C24, fgp_iter=50, iters=30)
+        tuned = {"seconds": round(time.perf_counter() - t0, 2), "choice": best,
+                 "candidates": [{k: c[k] for k in ("tile_x", "tile_y", "blocks_per_cu", "fgp_tw", "us_per_iter")}
+                                for c in cand]}
+        kw["autotune"] = True
     for _ in range(args.warmup):
         sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 512, 512, **kw)
     torch.cuda.synchronize()
@@ -399,7 +426,7 @@ def workload_tv512(sa, synth, args):
     return {"workload": "nufft_l1tv_512_m200000_24tap_fgp50", "ms_per_solve": round(dt * 1e3, 2),
             "kernel_ms": round(info["kernel_ms"], 3), "iterations": r["ITER"], "trials": info["total_trials"],
             "us_per_iteration": round(info["kernel_ms"] * 1e3 / r["ITER"], 2), "finalcost": r["finalcost"],
-            "stream_engine_kernels": stage_table(sa, 512, 200000, 12, args.profile_reps)}
+            "autotune": tuned, "stream_engine_kernels": stage_table(sa, 512, 200000, 12, args.profile_reps)}
 
 
 def workload_lambda128(sa, synth, args):
@@ -463,13 +490,15 @@ def main():
     ap.add_argument("--engine", choices=["persistent", "stream"], default="persistent")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--warm", type=int, default=16,
+    ap.add_argument("--warm", type=int, default=32,
                     help="held-out 256^2 problems for the learned warm start (0 = skip)")
     ap.add_argument("--workload", choices=["headline", "tv512", "lambda128", "batch256"], default="headline",
                     help="headline: the bench line; tv512 / lambda128 / batch256: time only the 512^2 L1+TV solve, "
                          "the 6x6 lambda-grid batch or a batch of --nprob 256^2 problems (profiling runs; the "
                          "batches on --engine) and print their own JSON line")
     ap.add_argument("--nprob", type=int, default=16, help="problems of the batch256 workload")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tv512: tune the launch configuration on the problem first, then time tuned solves")
     ap.add_argument("--batch", choices=["none", "lambda", "movie", "all"], default="all",
                     help="also time the batched workloads (6x6 lambda grid at 128^2, 8 lambda_tsv values "
                          "at 256^2, 32-frame movie) against the same problems solved one by one")

@@ -192,6 +192,9 @@ typedef struct {
   double *xout;
   int nonneg_flag, box_flag;
   const float *cl_box;
+  double target_objective; /* > 0: the solve also stops at the first iteration whose objective
+                              is at or below this value (timing at equal final objective);
+                              such batches run on the wide engine */
 } mfista_nufft_problem;
 
 /* Solves nprob independent NUFFT problems in one persistent launch (more
@@ -208,7 +211,9 @@ typedef struct {
  * MFISTA_ENGINE_STREAM runs the wide engine when every problem is L1 or
  * L1+TSV of one image size (otherwise the persistent launch): the stream
  * engine's loop with every kernel launched once for all problems, one step
- * captured as a hipGraph; opts->engine_used reports which ran. */
+ * captured as a hipGraph; opts->engine_used reports which ran.  A batch with
+ * a target_objective set runs on the wide engine (L1 / L1+TSV of one size,
+ * otherwise MFISTA_E_ARG). */
 int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem *probs,
                                     struct RESULT *results, mfista_opts *opts);
 
@@ -342,8 +347,8 @@ long mfista_device_allocations(void);
  * mechanism is configure_tile, c++/mfista_nufft_lib.cpp:643-678, which
  * balances gridding taps over OpenMP threads by rule).  The persistent
  * solver's gridding tile (16x16, 32x16 or 32x32 cells), workgroups per CU,
- * paired backtracking rounds and, for batches, the engine are timed on the
- * problem itself: every candidate
+ * paired backtracking rounds, the FGP block shape (L1+TV) and, for batches,
+ * the engine are timed on the problem itself: every candidate
  * solves it for `iters` iterations from its xinit (no stopping rule), twice,
  * and the fastest per iteration is kept in a process-wide table under the
  * shape key (NX, NY, preset, regulariser class, floor(log2 M), floor(log2
@@ -362,6 +367,8 @@ typedef struct mfista_tune {
   int engine;         /* batches: MFISTA_ENGINE_PERSISTENT (one cooperative launch) or
                          MFISTA_ENGINE_STREAM (the wide engine: every stage one launch for
                          all problems, a step replayed as a graph) */
+  int fgp_tw;         /* L1+TV: FGP block shape, 0 = pixels one per thread along the flat
+                         image, 64 / 32 / 16 = tiles of fgp_tw x (256 / fgp_tw) pixels */
   double us_per_iter; /* measured microseconds per iteration */
 } mfista_tune;
 

@@ -58,7 +58,7 @@ class NufftProblem(C.Structure):
         (n, C.c_int) for n in ("M", "NX", "NY", "maxiter")] + [
         (n, C.c_double) for n in ("eps", "lambda_l1", "lambda_tv", "lambda_tsv", "cinit")] + [
         ("xinit", C.POINTER(C.c_double)), ("xout", C.POINTER(C.c_double)), ("nonneg_flag", C.c_int),
-        ("box_flag", C.c_int), ("cl_box", C.POINTER(C.c_float))]
+        ("box_flag", C.c_int), ("cl_box", C.POINTER(C.c_float)), ("target_objective", C.c_double)]
 
 
 class RESULT(C.Structure):
@@ -91,7 +91,7 @@ class Tune(C.Structure):
     _fields_ = [
         ("NX", C.c_int), ("NY", C.c_int), ("preset", C.c_int), ("reg", C.c_int), ("m_log2", C.c_int),
         ("nprob_log2", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("blocks_per_cu", C.c_int),
-        ("paired", C.c_int), ("engine", C.c_int), ("us_per_iter", C.c_double),
+        ("paired", C.c_int), ("engine", C.c_int), ("fgp_tw", C.c_int), ("us_per_iter", C.c_double),
     ]
 
     def as_dict(self):
@@ -325,6 +325,7 @@ def _pack_problems(problems):
         bflag, b, pb = _box(p.get("cl_box"), NX * NY)
         keep.append(b)
         q.box_flag, q.cl_box = bflag, pb
+        q.target_objective = float(p.get("target_objective", 0.0))
     return arr, keep, outs
 
 
@@ -335,7 +336,7 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
     a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
     the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
     vis_std, NX, NY and optionally maxiter, eps, lambda_l1, lambda_tv, lambda_tsv,
-    cinit, xinit, nonneg_flag, cl_box).  Problems given the same numpy arrays for
+    cinit, xinit, nonneg_flag, cl_box, target_objective).  Problems given the same numpy arrays for
     u_dx, v_dy, vis_r, vis_i, vis_std share one NUFFT set-up.  autotune=True:
     the autotuner's launch configuration for the batch's shape.
     engine=ENGINE_STREAM: the wide engine (every stage one launch for all

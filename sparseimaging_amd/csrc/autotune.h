@@ -12,7 +12,9 @@
 //   * paired backtracking rounds (the trials c and c*eta evaluated together)
 //     or one trial per round;
 //   * for batches, the engine: one persistent launch for the whole batch, or
-//     the wide engine (every stage one launch for all problems, wide.hip).
+//     the wide engine (every stage one launch for all problems, wide.hip);
+//   * for L1+TV, the FGP block shape: pixels dealt one per thread along the
+//     flat image, or in tiles of 64x4, 32x8 or 16x16 pixels per workgroup.
 // Every candidate solves the caller's own problem for a fixed number of
 // iterations from its own start (no stopping rule); the fastest per
 // iteration is kept in a process-wide table under the problem's shape key,
@@ -36,6 +38,7 @@ struct TuneKey {
 struct TuneChoice {
   int tile_x = 16, tile_y = 16, blocks_per_cu = 2, paired = 1;
   int engine = MFISTA_ENGINE_PERSISTENT;  // batches: MFISTA_ENGINE_STREAM = the wide engine
+  int fgp_tw = 0;                         // L1+TV: FGP block shape (0 = along the flat image)
   double us_per_iter = 0;
 };
 

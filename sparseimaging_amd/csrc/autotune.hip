@@ -68,6 +68,7 @@ void tune_apply(Context& C, const TuneChoice& c) {
   C.persist_blocks_per_cu = c.blocks_per_cu;
   C.persist_pair = c.paired;
   C.tune_engine = c.engine;
+  C.fgp_tw = c.fgp_tw;
 }
 
 void tune_restore(Context& C) {
@@ -76,21 +77,25 @@ void tune_restore(Context& C) {
   C.persist_blocks_per_cu = C.def_blocks_per_cu;
   C.persist_pair = C.def_pair;
   C.tune_engine = -1;
+  C.fgp_tw = 0;
 }
 
 std::vector<TuneChoice> tune_candidates(int reg, bool batch) {
   static const int tiles[3][2] = {{16, 16}, {32, 16}, {32, 32}};
   std::vector<TuneChoice> v;
+  static const int fgp_shapes[4] = {0, 64, 32, 16};
   for (const auto& t : tiles)
     for (int bpc = 1; bpc <= 2; bpc++)
-      for (int pair = 1; pair >= (reg == 2 ? 1 : 0); pair--) {  // TV rounds are never paired
-        TuneChoice c;
-        c.tile_x = t[0];
-        c.tile_y = t[1];
-        c.blocks_per_cu = bpc;
-        c.paired = (reg == 2) ? 0 : pair;
-        v.push_back(c);
-      }
+      for (int pair = 1; pair >= (reg == 2 ? 1 : 0); pair--)  // TV rounds are never paired
+        for (int f = 0; f < (reg == 2 ? 4 : 1); f++) {      // FGP shapes for TV only
+          TuneChoice c;
+          c.tile_x = t[0];
+          c.tile_y = t[1];
+          c.blocks_per_cu = bpc;
+          c.paired = (reg == 2) ? 0 : pair;
+          c.fgp_tw = fgp_shapes[f];
+          v.push_back(c);
+        }
   if (batch && reg != 2)  // the wide engine: the gridding tile is its only knob
     for (const auto& t : tiles) {
       TuneChoice c;
@@ -148,21 +153,22 @@ void to_public(const TuneKey& k, const TuneChoice& c, mfista_tune* t) {
   t->blocks_per_cu = c.blocks_per_cu;
   t->paired = c.paired;
   t->engine = c.engine;
+  t->fgp_tw = c.fgp_tw;
   t->us_per_iter = c.us_per_iter;
 }
 
 // text file: a header line, then one entry per line,
-// "NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired engine us_per_iter"
+// "NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired engine fgp_tw us_per_iter"
 static int save_table(const char* path) {
   FILE* f = fopen(path, "w");
   if (!f) return MFISTA_E_ARG;
   fprintf(f, "# mfista_amd tune table: NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired "
-             "engine us_per_iter\n");
+             "engine fgp_tw us_per_iter\n");
   for (const auto& e : g_table) {
     const TuneKey& k = e.first;
     const TuneChoice& c = e.second;
-    fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %.3f\n", k.NX, k.NY, k.preset, k.reg, k.m_log2, k.nprob_log2,
-            c.tile_x, c.tile_y, c.blocks_per_cu, c.paired, c.engine, c.us_per_iter);
+    fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %.3f\n", k.NX, k.NY, k.preset, k.reg, k.m_log2,
+            k.nprob_log2, c.tile_x, c.tile_y, c.blocks_per_cu, c.paired, c.engine, c.fgp_tw, c.us_per_iter);
   }
   return fclose(f) == 0 ? 0 : MFISTA_E_ARG;
 }
@@ -176,12 +182,13 @@ static int load_table(const char* path) {
     if (line[0] == '#') continue;
     TuneKey k;
     TuneChoice c;
-    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %lf", &k.NX, &k.NY, &k.preset, &k.reg, &k.m_log2,
-               &k.nprob_log2, &c.tile_x, &c.tile_y, &c.blocks_per_cu, &c.paired, &c.engine,
-               &c.us_per_iter) != 12)
+    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf", &k.NX, &k.NY, &k.preset, &k.reg, &k.m_log2,
+               &k.nprob_log2, &c.tile_x, &c.tile_y, &c.blocks_per_cu, &c.paired, &c.engine, &c.fgp_tw,
+               &c.us_per_iter) != 13)
       continue;
     if (!tile_supported(c.tile_x, c.tile_y) || c.blocks_per_cu < 1 || c.blocks_per_cu > 4 ||
-        (c.engine != MFISTA_ENGINE_PERSISTENT && c.engine != MFISTA_ENGINE_STREAM))
+        (c.engine != MFISTA_ENGINE_PERSISTENT && c.engine != MFISTA_ENGINE_STREAM) ||
+        (c.fgp_tw != 0 && c.fgp_tw != 16 && c.fgp_tw != 32 && c.fgp_tw != 64))
       continue;
     g_table[k] = c;
     n++;

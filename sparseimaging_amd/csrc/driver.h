@@ -27,6 +27,9 @@ struct RunParams {
   double eta_first = 1.1;
   int single_trials = 0;          // persistent engine: one backtracking trial per round
   int deterministic = 0;          // persistent engine: gridding without atomics (tile-exact ranges)
+  double target_F = 0;            // stream and wide engines: > 0 also stops at this objective
+  int fgp_tw = 0;                 // persistent engine: FGP block shape (0 = along the flat image,
+                                  // else tiles of fgp_tw x 256 / fgp_tw pixels)
   const double* xinit = nullptr;  // device, FP64
   double* xout = nullptr;         // device, FP64
   const float* box = nullptr;     // device

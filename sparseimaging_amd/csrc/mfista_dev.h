@@ -262,6 +262,9 @@ __device__ inline void dev_control(DevState* S, double* cost, const double* __re
     const double d = cost[it - kTD] - cost[it];
     stop = (S->rel_tol > 0) ? (d <= S->rel_tol * cost[it]) : (d < S->eps_abs);
   }
+  // a target objective (timing at equal final objective): the iterate after
+  // this iteration is xnew if accepted, else x
+  if (!stop && S->target_F > 0 && (F < cost[it] ? F : cost[it]) <= S->target_F) stop = true;
   S->mu = munew;
   S->trial = 0;
   S->gate_momentum = 0;

@@ -33,6 +33,7 @@ struct DevState {
   int total_trials, lip_mode;
   int gate_prologue, gate_trial, gate_momentum;  // nonzero: the kernel returns at once
   int rounds;                                    // persistent engine: trial rounds evaluated
+  double target_F;  // > 0: also stop once the objective is at or below it (stream and wide engines)
 };
 
 // offsets (in doubles) of the partial-sum regions written by the kernels

@@ -229,6 +229,8 @@ static int persistent_prepare(NufftState& N, PersistWork& W, const RunParams& P,
   A.S = (DevState*)W.state.p;
   A.bar = (unsigned long long*)W.bar.p;
   A.hier_barrier = hier;
+  // FGP block shape: tiles of fgp_tw x (256 / fgp_tw) pixels, 0 = along the flat image
+  A.fgp_tw = (P.fgp_tw == 16 || P.fgp_tw == 32 || P.fgp_tw == 64) ? P.fgp_tw : 0;
   return 0;
 }
 

@@ -71,6 +71,8 @@ struct PArgs {
   double* cost;
   DevState* S;
   unsigned long long* bar;  // barrier words, see BarCtx
+  int fgp_tw;               // FGP block shape: 0 = one pixel per thread along the flat image,
+                            // else tiles of fgp_tw x (256 / fgp_tw) pixels per workgroup
   int hier_barrier;         // XCD-hierarchical grid barrier
   int pair;                 // L1+TSV: evaluate two backtracking trials per round
   // Coupled movie (frames of one solve, matlab/MFISTA_movie_eht.m): every
@@ -799,8 +801,9 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
       for (int k = 0; k < A.fgp_iter; k++) {
         const float c1 = (float)GP(A.fgp_c)[2 * k], c2 = (float)GP(A.fgp_c)[2 * k + 1];
         const int first = (k == 0);
-        for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-          const int i = f % W, j = f / W;
+        // one FGP update of pixel (i, j): x at (i, j), (i + 1, j), (i, j + 1)
+        // from b and the previous duals, then the new duals
+        auto fgp_pixel = [&](const int i, const int j) {
           auto xat = [&](int ii, int jj) -> float {
             const int ff = jj * W + ii;
             const float bb = bshift + cinv * GP(A.gr)[ff] + (float)GP(A.z)[ff];
@@ -838,6 +841,21 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
             const float qold = first ? 0.f : GP(A.qq)[kk];
             so[kk] = nq_ * c1 + c2 * qold;
             GP(A.qq)[kk] = nq_;
+          }
+        };
+        // pixel -> thread mapping (the FGP block shape, PArgs::fgp_tw): rows
+        // of the flat image one pixel per thread (0), or tiles of fgp_tw x
+        // (256 / fgp_tw) pixels dealt to the workgroups; every pixel's update
+        // is the same either way
+        if (A.fgp_tw == 0) {
+          for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) fgp_pixel(f % W, f / W);
+        } else {
+          const int tw = A.fgp_tw, th = MF_BLOCK / tw;
+          const int ntx = (W + tw - 1) / tw, nty = (H + th - 1) / th;
+          const int ti0 = threadIdx.x % tw, tj0 = threadIdx.x / tw;
+          for (int t = lb; t < ntx * nty; t += nb) {
+            const int i = (t % ntx) * tw + ti0, j = (t / ntx) * th + tj0;
+            if (i < W && j < H) fgp_pixel(i, j);
           }
         }
         if (!grid_sync(B, epoch, 6)) return;

@@ -123,6 +123,7 @@ struct Context {
   int grid_tile_x = 16, grid_tile_y = 16;  // gridding tile of the next NUFFT set-up (autotuner)
   int def_blocks_per_cu = 2, def_pair = 1;  // the knobs' defaults (environment overrides included)
   int tune_engine = -1;  // batches: the autotuner's engine (-1: the caller's opts->engine)
+  int fgp_tw = 0;        // FGP block shape of the persistent engine (autotuner; 0 = along the flat image)
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

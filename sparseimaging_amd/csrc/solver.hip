@@ -579,6 +579,7 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     P.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
     P.single_trials = (opts && opts->single_trials) ? 1 : 0;
     P.deterministic = (opts && opts->deterministic) ? 1 : 0;
+    P.fgp_tw = C.fgp_tw;
     P.xinit = A.xinit;
     P.xout = A.xout;
     P.box = A.box;
@@ -848,6 +849,8 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
     R.eta_first = (preset == MFISTA_PRESET_CXX12) ? 10.0 : kEta;
     R.single_trials = (opts && opts->single_trials) ? 1 : 0;
     R.deterministic = (opts && opts->deterministic) ? 1 : 0;
+    R.fgp_tw = C->fgp_tw;
+    R.target_F = iters > 0 ? 0.0 : q.target_objective;
     R.xinit = (const double*)W.x0.p;
     R.xout = (double*)W.xo.p;
     R.box = q.box_flag ? (const float*)W.box.p : nullptr;
@@ -859,11 +862,19 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   float kms = 0;
   // the wide engine (every kernel launched once for all problems) takes
   // batches of L1 / L1+TSV problems of one image size; others, and the
-  // default, run in one persistent launch
+  // default, run in one persistent launch.  Target objectives are a control
+  // rule of the wide engine only.
   const int engine = C->tune_engine >= 0 ? C->tune_engine : (opts ? opts->engine : MFISTA_ENGINE_PERSISTENT);
-  bool wide = engine == MFISTA_ENGINE_STREAM && !P.empty();
-  for (size_t j = 0; wide && j < P.size(); j++)
-    if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide = false;
+  bool targets = false, wide_ok = !P.empty();
+  for (size_t j = 0; j < P.size(); j++) {
+    if (P[j].target_F > 0) targets = true;
+    if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide_ok = false;
+  }
+  if (targets && !wide_ok) {
+    fprintf(stderr, "[mfista_amd] target objectives need L1 or L1+TSV problems of one image size\n");
+    return MFISTA_E_ARG;
+  }
+  const bool wide = wide_ok && (targets || engine == MFISTA_ENGINE_STREAM);
   if (P.empty())
     st.clear();
   else if (wide)

@@ -518,6 +518,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     h.maxiter = R.maxiter;
     h.nonneg = R.nonneg;
     h.lip_mode = 1;
+    h.target_F = R.target_F;
     MF_CHECK(hipMemcpyAsync(Wk.state.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
     WProb& w = hw[b];
     memset(&w, 0, sizeof(w));

@@ -95,8 +95,8 @@ def test_tune_table_file_round_trip(tmp_path):
     """The autotuner's table file (no GPU work): a saved entry loads back, and
     lines with an unsupported tile or a bad field count are skipped."""
     f = tmp_path / "tune.txt"
-    f.write_text("# header\n256 256 1 1 14 0 32 16 2 1 0 88.5\n256 256 1 1 14 3 24 16 2 1 0 1.0\n"
-                 "256 256 1 1 14 4 16 16 2 1 7 1.0\n1 2 3\n")
+    f.write_text("# header\n256 256 1 1 14 0 32 16 2 1 0 0 88.5\n256 256 1 1 14 3 24 16 2 1 0 0 1.0\n"
+                 "256 256 1 1 14 4 16 16 2 1 7 0 1.0\n256 256 0 2 14 0 16 16 2 0 0 12 1.0\n1 2 3\n")
     sa.tune_clear()
     assert sa.tune_load(str(f)) == 1
     t = sa.tune_table()
