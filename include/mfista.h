@@ -115,6 +115,16 @@ void y2x_nufft(double *u_dx, double *v_dy, int M, int Nx, int Ny, double *yin_r,
 #define MFISTA_ENGINE_PERSISTENT 0
 #define MFISTA_ENGINE_STREAM 1
 
+/* The solve loop of a single NUFFT solve on the persistent engine (opts->loop).
+ * GRIDDING: every iteration grids the residual and every trial degrids its
+ * step, the reference's structure.  TOEPLITZ: the data term's normal operator
+ * A^H W^2 A applied as a convolution with FP64 FFTs on the padded grid, its
+ * kernel and A^H W^2 V computed once per solve with a 24-tap FP64 NUFFT
+ * adjoint; no gridding inside the loop.  No counterpart in the reference. */
+#define MFISTA_LOOP_AUTO 0
+#define MFISTA_LOOP_GRIDDING 1
+#define MFISTA_LOOP_TOEPLITZ 2
+
 #define MFISTA_OK 0
 #define MFISTA_E_BOTH_REG -1   /* lambda_tv and lambda_tsv both positive */
 #define MFISTA_E_TV_PRESET -2  /* TV with the C++ preset (not in that library) */
@@ -153,6 +163,13 @@ typedef struct mfista_opts {
                         launch configuration the autotuner chose for their shape, tuning
                         first (on the problem itself) when the table has none */
   int tune_iters;    /* in: iterations per autotuner candidate (default 30) */
+  int loop;          /* in: single NUFFT solves on the persistent engine: MFISTA_LOOP_AUTO
+                        (default) runs the Toeplitz loop for L1 and L1+TSV square images
+                        up to 512 x 512 (not with deterministic, single_trials or the
+                        allocation-free entry point), the gridding loop otherwise;
+                        MFISTA_LOOP_GRIDDING always runs the gridding loop */
+  int loop_used;     /* out: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ (persistent
+                        engine), 0 when the stream engine ran */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);

@@ -27,6 +27,9 @@ PRESET_C24 = 0
 PRESET_CXX12 = 1
 ENGINE_PERSISTENT = 0  # one cooperative launch per NUFFT solve
 ENGINE_STREAM = 1      # one launch per kernel, device-side gating
+LOOP_AUTO = 0          # single NUFFT solves: the Toeplitz loop where it applies, else gridding
+LOOP_GRIDDING = 1      # grid the residual every iteration, degrid every trial (the reference's structure)
+LOOP_TOEPLITZ = 2      # (loop_used) the normal operator as an FFT convolution, no gridding in the loop
 MAXITER, MINITER, FGPITER, TD, ETA, EPS = 50000, 100, 100, 50, 1.1, 1.0e-5
 
 STATUS = {
@@ -82,7 +85,7 @@ class Opts(C.Structure):
         ("max_trials", C.c_int), ("solve_ms", C.c_double), ("total_trials", C.c_int),
         ("engine", C.c_int), ("kernel_ms", C.c_double), ("trial_rounds", C.c_int),
         ("single_trials", C.c_int), ("engine_used", C.c_int), ("deterministic", C.c_int),
-        ("autotune", C.c_int), ("tune_iters", C.c_int),
+        ("autotune", C.c_int), ("tune_iters", C.c_int), ("loop", C.c_int), ("loop_used", C.c_int),
     ]
 
 
@@ -233,7 +236,7 @@ def _box(cl_box, n):
 
 
 def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engine=ENGINE_PERSISTENT,
-              single_trials=False, deterministic=False, autotune=False, tune_iters=30):
+              single_trials=False, deterministic=False, autotune=False, tune_iters=30, loop=LOOP_AUTO):
     o = Opts()
     lib().mfista_default_opts(C.byref(o))
     o.preset, o.fgp_iter, o.rel_tol, o.verbose, o.engine = preset, fgp_iter, rel_tol, verbose, engine
@@ -241,12 +244,14 @@ def make_opts(preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, verbose=0, engin
     o.deterministic = 1 if deterministic else 0
     o.autotune = 1 if autotune else 0
     o.tune_iters = int(tune_iters)
+    o.loop = int(loop)
     return o
 
 
 def _info(o):
     return {"max_trials": o.max_trials, "solve_ms": o.solve_ms, "total_trials": o.total_trials,
-            "kernel_ms": o.kernel_ms, "trial_rounds": o.trial_rounds, "engine_used": o.engine_used}
+            "kernel_ms": o.kernel_ms, "trial_rounds": o.trial_rounds, "engine_used": o.engine_used,
+            "loop_used": o.loop_used}
 
 
 # ---------------------------------------------------------------- solvers
@@ -254,14 +259,15 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
                               lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, xinit=None,
                               nonneg_flag=1, cl_box=None, preset=PRESET_C24, fgp_iter=FGPITER,
                               rel_tol=0.0, engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False,
-                              autotune=False, tune_iters=30):
+                              autotune=False, tune_iters=30, loop=LOOP_AUTO):
     """NUFFT-mode MFISTA on the GPU (c/mfista_nufft_lib.c:909).  Returns (x[NX,NY], RESULT dict, opts).
     single_trials=True evaluates one backtracking trial per round in the persistent
     engine (default: L1+TSV rounds evaluate the trials c and c*ETA together);
     deterministic=True grids without floating-point atomics (bit-identical
     repeated solves, persistent engine); autotune=True uses the autotuner's
     launch configuration for this shape (tuned first, with tune_iters
-    iterations per candidate, when the table has none)."""
+    iterations per candidate, when the table has none); loop=LOOP_GRIDDING
+    keeps the gridding loop where the Toeplitz loop would run (LOOP_AUTO)."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
     vr, pvr = _d(vis_r)
@@ -274,7 +280,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     bflag, b, pb = _box(cl_box, NX * NY)
     res = RESULT()
     o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
-                  deterministic=deterministic, autotune=autotune, tune_iters=tune_iters)
+                  deterministic=deterministic, autotune=autotune, tune_iters=tune_iters, loop=loop)
     rc = lib().mfista_imaging_core_nufft_ex(pu, pv, pvr, pvi, pvs, M, NX, NY, maxiter, eps, lambda_l1,
                                             lambda_tv, lambda_tsv, cinit, px0,
                                             xout.ctypes.data_as(_dp), nonneg_flag, bflag, pb,
@@ -357,7 +363,7 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
 # ---------------------------------------------------------------- autotuner
 def autotune_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0,
                    cinit=1e4, xinit=None, nonneg_flag=1, preset=PRESET_C24, fgp_iter=FGPITER, iters=30,
-                   deterministic=False):
+                   deterministic=False, loop=LOOP_AUTO):
     """Times the persistent solver's launch configurations on this problem
     (`iters` iterations each) and stores the fastest for its shape.  Returns
     (choice dict, list of every candidate's dict with its us_per_iter)."""
@@ -368,7 +374,7 @@ def autotune_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, lambda_l1=1.0, lam
     vs, pvs = _d(vis_std)
     x0 = np.zeros(NX * NY) if xinit is None else np.asarray(xinit, dtype=np.float64).ravel()
     x0, px0 = _d(x0)
-    o = make_opts(preset, fgp_iter, deterministic=deterministic)
+    o = make_opts(preset, fgp_iter, deterministic=deterministic, loop=loop)
     best, allc, n = Tune(), (Tune * 32)(), C.c_int(0)
     rc = lib().mfista_autotune_nufft(pu, pv, pvr, pvi, pvs, u.size, NX, NY, lambda_l1, lambda_tv, lambda_tsv,
                                      cinit, px0, nonneg_flag, C.byref(o), iters, C.byref(best), allc, 32,
@@ -593,7 +599,8 @@ def compute_looe(A, x, yAx, NX=1, NY=1, lambda_tsv=0.0):
 def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxiter=MAXITER, eps=EPS,
                        lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=0.0, cinit=1e4, nonneg_flag=1,
                        t_box=None, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0, stream=None,
-                       engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False, autotune=False):
+                       engine=ENGINE_PERSISTENT, single_trials=False, deterministic=False, autotune=False,
+                       loop=LOOP_AUTO):
     """HBM-resident solve: every array argument is a device pointer (int) or a
     torch CUDA tensor (float64; cl_box float32).  The solve is queued on
     `stream` (a hipStream_t as int); by default on torch's current stream, so
@@ -614,7 +621,7 @@ def solve_nufft_device(t_u, t_v, t_vr, t_vi, t_sd, NX, NY, t_xinit, t_xout, maxi
         stream = torch.cuda.current_stream().cuda_stream
     res = RESULT()
     o = make_opts(preset, fgp_iter, rel_tol, engine=engine, single_trials=single_trials,
-                  deterministic=deterministic, autotune=autotune)
+                  deterministic=deterministic, autotune=autotune, loop=loop)
     rc = lib().mfista_nufft_solve_device(ptr(t_u), ptr(t_v), ptr(t_vr), ptr(t_vi), ptr(t_sd), M, NX, NY,
                                          maxiter, eps, lambda_l1, lambda_tv, lambda_tsv, cinit,
                                          ptr(t_xinit), ptr(t_xout), nonneg_flag,

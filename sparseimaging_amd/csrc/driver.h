@@ -50,6 +50,12 @@ int mfista_run(Context& C, Operator<R>& op, const RunParams& P, RunStats* out);
 // kCoopRefused, with nothing run, if the runtime refuses the cooperative launch
 constexpr int kCoopRefused = MFISTA_E_COOP;
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
+// the Toeplitz loop (toeplitz_body.h): its set-up for the problem of C.nu
+// (toeplitz.hip; MFISTA_E_SIZE or MFISTA_E_FOV when it does not apply) and
+// the solve in one cooperative launch (persistent.hip)
+int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const double* d_vr, const double* d_vi,
+                   const double* d_sd, int M, int NX, int NY, int preset, double wv2);
+int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out);
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms);
 // independent solves of one image size and preset with every kernel launched

@@ -1,7 +1,8 @@
 // persistent.hip -- host side of the persistent NUFFT MFISTA solver: launch
 // configuration, workspace set-up, cooperative launches of single solves,
-// batches and coupled movies.  The device code is in persistent_body.h.
-#include "persistent_body.h"
+// batches and coupled movies.  The device code is in persistent_body.h (the
+// gridding loop) and toeplitz_body.h (the Toeplitz loop of single solves).
+#include "toeplitz_body.h"
 
 namespace mf {
 
@@ -298,9 +299,13 @@ static int timed_launch(const PLaunch& L, void** args, int nblk, hipStream_t st,
 // workgroup's time from leaving the previous barrier to arriving at this one,
 // and the barrier's tail (last arrival to last release), appended to the file.
 static void write_trace(const char* path, const std::vector<unsigned long long>& tr, const std::vector<int>& tag,
-                        int nbar, int nrow, double clock_khz, const RunStats& rs, const RunParams& P, int M) {
-  static const char* names[] = {"?",        "init-row", "init-col", "init-degrid", "P1-grid", "P2-colinv",
-                                "FGP-iter", "TV-c2r",   "T1-rows",  "T2-cols",     "T3-degrid"};
+                        int nbar, int nrow, double clock_khz, const RunStats& rs, const RunParams& P, int M,
+                        bool toep = false) {
+  static const char* grid_names[] = {"?",        "init-row", "init-col", "init-degrid", "P1-grid", "P2-colinv",
+                                     "FGP-iter", "TV-c2r",   "T1-rows",  "T2-cols",     "T3-degrid"};
+  static const char* toep_names[] = {"?", "init-row", "init-conv", "init-c2r", "momentum", "?",
+                                     "?", "?",        "T1-rows",   "T2-conv",  "T3-c2r"};
+  const char* const* names = toep ? toep_names : grid_names;
   double st[11] = {0}, tl[11] = {0};
   int cnt[11] = {0};
   const double us = 1e3 / clock_khz;
@@ -321,8 +326,8 @@ static void write_trace(const char* path, const std::vector<unsigned long long>&
   }
   FILE* f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "persistent solve %dx%d M=%d: ITER %d trials %d rounds %d kernel %.3f ms, %d barriers traced\n", P.NX,
-          P.NY, M, rs.ITER, rs.total_trials, rs.rounds, rs.kernel_ms, nbar);
+  fprintf(f, "persistent solve (%s loop) %dx%d M=%d: ITER %d trials %d rounds %d kernel %.3f ms, %d barriers traced\n",
+          toep ? "Toeplitz" : "gridding", P.NX, P.NY, M, rs.ITER, rs.total_trials, rs.rounds, rs.kernel_ms, nbar);
   fprintf(f, "%-12s %7s %14s %13s %10s\n", "stage", "count", "mean stage us", "mean tail us", "total ms");
   double tot = 0;
   for (int k = 1; k <= 10; k++) {
@@ -333,6 +338,23 @@ static void write_trace(const char* path, const std::vector<unsigned long long>&
   }
   fprintf(f, "sum over traced barriers %.3f ms\n\n", tot * 1e-3);
   fclose(f);
+}
+
+// copies the barrier trace of the last single-solve launch back and appends its summary to `path`
+static int read_trace(Context& C, const char* path, int tcap, int nblk, const RunStats& rs, const RunParams& P,
+                      bool toep) {
+  const long nbar = tcap;
+  std::vector<unsigned long long> tr((size_t)nbar * nblk * 2);
+  std::vector<int> tag(nbar);
+  MF_CHECK(hipMemcpy(tr.data(), C.trace.p, tr.size() * 8, hipMemcpyDeviceToHost));
+  MF_CHECK(hipMemcpy(tag.data(), C.trace_tag.p, tag.size() * 4, hipMemcpyDeviceToHost));
+  int dev = 0, khz = 0;
+  MF_CHECK(hipGetDevice(&dev));
+  MF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  int used = 0;  // barriers actually passed: their release clocks are set
+  while (used < nbar && tr[((size_t)used * nblk) * 2 + 1] != 0) used++;
+  write_trace(path, tr, tag, used, nblk, khz > 0 ? khz : 100000, rs, P, C.nu.M, toep);
+  return 0;
 }
 
 int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
@@ -363,19 +385,110 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   C.persist_blocks = L.nblk;
   if ((rc = persistent_collect(C.pw, P, st, out))) return rc;
   MF_CHECK(hipStreamSynchronize(st));
-  if (A.trace) {
-    const long nbar = tcap;
-    std::vector<unsigned long long> tr((size_t)nbar * L.nblk * 2);
-    std::vector<int> tag(nbar);
-    MF_CHECK(hipMemcpy(tr.data(), C.trace.p, tr.size() * 8, hipMemcpyDeviceToHost));
-    MF_CHECK(hipMemcpy(tag.data(), C.trace_tag.p, tag.size() * 4, hipMemcpyDeviceToHost));
-    int dev = 0, khz = 0;
-    MF_CHECK(hipGetDevice(&dev));
-    MF_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    int used = 0;  // barriers actually passed: their release clocks are set
-    while (used < nbar && tr[((size_t)used * L.nblk) * 2 + 1] != 0) used++;
-    write_trace(trace_path, tr, tag, used, L.nblk, khz > 0 ? khz : 100000, *out, P, C.nu.M);
+  if (A.trace) return read_trace(C, trace_path, tcap, L.nblk, *out, P, false);
+  return 0;
+}
+
+// Toeplitz loop of a single solve (toeplitz_body.h); toeplitz_setup has run
+// for the problem of C.nu.  Paired trial rounds always.
+int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
+  NufftState& N = C.nu;
+  ToepWork& T = C.tw;
+  PersistWork& W = C.pw;
+  hipStream_t st = C.stream;
+  const GridGeom& g = N.geom;
+  const int NX = P.NX, NY = P.NY, n = NX * NY, nky = NY + 1;
+  if (P.tv_mode || NX != g.NX || NY != g.NY) return MFISTA_E_ARG;
+  // launch configuration: [FP64 twiddle tables][stage scratch: two FP64
+  // sequences of the longer axis, double-buffered]
+  int dev = 0, ncu = 0, lds_max = 0;
+  MF_CHECK(hipGetDevice(&dev));
+  MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  MF_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + sizeof(BarCtx) + 64;
+  const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
+  PLaunch L;
+  L.sh = (size_t)4 * Lm * sizeof(double2);
+  const size_t tables = (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255;
+  L.lds_tables = (tables + L.sh + static_lds <= (size_t)lds_max) ? 1 : 0;
+  if (L.lds_tables) L.sh += tables;
+  if (L.sh + static_lds > (size_t)lds_max) return MFISTA_E_SIZE;
+  L.fn = L.lds_tables ? (const void*)k_mfista_toeplitz<true> : (const void*)k_mfista_toeplitz<false>;
+  int per_cu = 0;
+  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L.fn, MF_BLOCK, L.sh));
+  if (per_cu < 1) return MFISTA_E_HIP;
+  const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 2;
+  L.nblk = ncu * (per_cu < want ? per_cu : want);
+  L.pair = 1;
+  // workspace
+  const size_t t64 = (size_t)nky * NX * sizeof(double2);
+  if (W.px.ensure((size_t)n * 8) || W.pz.ensure((size_t)n * 8) || W.pxn.ensure((size_t)2 * n * 8) ||
+      W.T64.ensure(2 * t64) || W.part.ensure((size_t)L.nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
+      W.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || W.bar.ensure(kBarBytes) ||
+      T.tvec.ensure((size_t)4 * n * 8) || T.u64.ensure(2 * t64))
+    return MFISTA_E_HIP;
+  if (P.xinit != W.px.p)
+    MF_CHECK(hipMemcpyAsync(W.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+  MF_CHECK(hipMemsetAsync(W.bar.p, 0, kBarBytes, st));
+  PArgs A;
+  memset(&A, 0, sizeof(A));
+  A.g = g;
+  A.Px = N.Px;
+  A.Py = N.Py;
+  A.twxd = (const double2*)N.twxd.p;
+  A.twyd = (const double2*)N.twyd.p;
+  A.T64 = (double2*)W.T64.p;
+  A.T64b = (double2*)((char*)W.T64.p + t64);
+  A.pair = 1;
+  A.x = (double*)W.px.p;
+  A.z = (double*)W.pz.p;
+  A.xn[0] = (double*)W.pxn.p;
+  A.xn[1] = (double*)W.pxn.p + n;
+  A.box = P.box_flag ? P.box : nullptr;
+  A.maxiter = P.maxiter;
+  A.nonneg = P.nonneg;
+  A.lambda_l1 = P.lambda_l1;
+  A.lambda_tsv = P.lambda_tsv;
+  A.eps_abs = P.eps_abs;
+  A.rel_tol = P.rel_tol;
+  A.eta_first = P.eta_first;
+  A.cinit = P.cinit;
+  A.part = (double*)W.part.p;
+  A.n_row = (NX + 1) / 2;
+  A.n_ky = nky;
+  A.lds_tables = L.lds_tables;
+  A.nblk = L.nblk;
+  A.cost = (double*)W.cost.p;
+  A.S = (DevState*)W.state.p;
+  A.bar = (unsigned long long*)W.bar.p;
+  A.hier_barrier = C.persist_hier_barrier;
+  A.kh = (const double*)T.kh.p;
+  A.bimg = (const double*)T.bimg.p;
+  for (int q = 0; q < 4; q++) A.tv[q] = (double*)T.tvec.p + (size_t)q * n;
+  A.u64a = (double2*)T.u64.p;
+  A.u64b = (double2*)((char*)T.u64.p + t64);
+  A.wv2 = T.wv2;
+  const char* trace_path = getenv("MFISTA_PERSISTENT_TRACE");
+  const int tcap = 16384;
+  if (trace_path && *trace_path) {
+    if (C.trace.ensure((size_t)tcap * L.nblk * 16) || C.trace_tag.ensure((size_t)tcap * 4)) return MFISTA_E_HIP;
+    MF_CHECK(hipMemsetAsync(C.trace.p, 0, (size_t)tcap * L.nblk * 16, st));
+    A.trace = (unsigned long long*)C.trace.p;
+    A.ttag = (int*)C.trace_tag.p;
+    A.tcap = tcap;
   }
+  if (W.pargs.ensure(sizeof(PArgs))) return MFISTA_E_HIP;
+  MF_CHECK(hipMemcpyAsync(W.pargs.p, &A, sizeof(PArgs), hipMemcpyHostToDevice, st));
+  const PArgs* dA = (const PArgs*)W.pargs.p;
+  void* args[] = {&dA};
+  float kms = 0;
+  int rc;
+  if ((rc = timed_launch(L, args, L.nblk, st, C.no_alloc ? nullptr : &kms))) return rc;
+  out->kernel_ms = kms;
+  C.persist_blocks = L.nblk;
+  if ((rc = persistent_collect(W, P, st, out))) return rc;
+  MF_CHECK(hipStreamSynchronize(st));
+  if (A.trace) return read_trace(C, trace_path, tcap, L.nblk, *out, P, true);
   return 0;
 }
 

@@ -88,6 +88,15 @@ struct PArgs {
   const double *z_prev, *z_next;  // the neighbouring frames' z (nullptr at the ends)
   const double* x_next;           // the next frame's x (initial cost)
   const double* xn_next[2];       // the next frame's trial images
+  // Toeplitz loop (toeplitz_body.h): Kh[ky][k] the real spectrum of the
+  // data term's normal operator, b = Re A^H W^2 V, the operator applied to
+  // x, xnew (two trials) and z (roles as vm[]), the column-pass outputs of
+  // the two trials, |W V|^2
+  const double* kh;
+  const double* bimg;
+  double* tv[4];
+  double2 *u64a, *u64b;
+  double wv2;
 };
 
 // Grid-wide barriers for a cooperative launch.  bar[] layout (64-bit words):
@@ -515,6 +524,104 @@ __device__ inline void grid_chunks(const GridGeom& g, const VisParams<float>& vp
   }
   if (cur >= 0) flush(false);
   __syncthreads();
+}
+
+// The control step's decision for one trial round, taken by thread 0 of every
+// workgroup with the same arithmetic everywhere.  sum: the round's reduced
+// partials, trial k at sum[k kPTrial + slot] (slots kPChi, kPLip, ...  in the
+// order chi^2(xnew), |W A d|^2, |d|^2, |xnew|_1, reg(xnew), TSV(d), d_X(xnew),
+// d_X(d)); chi_add is added to every trial's chi^2 (the constant |W V|^2 of
+// the Toeplitz loop, 0 otherwise).  The trials of a round are taken in order,
+// exactly as the reference's backtracking loop (c/mfista_nufft_lib.c:692-723)
+// takes them: the first that passes is accepted; one that fails multiplies c
+// by eta.
+__device__ __forceinline__ void round_decide(const PArgs& A, BlockState& bs, double* ring, const double* sum, bool pr,
+                                    bool tv, bool tsv, bool movie, int lb, int prologue, double chi_add) {
+  const int it = bs.iter;
+  if (prologue) {
+    ring[it % (kTD + 1)] = bs.costtmp;
+    if (lb == 0) GP(A.cost)[it] = bs.costtmp;
+    bs.prologue = 0;
+  }
+  bs.rounds += 1;
+  const double etaf = (it == 0) ? A.eta_first : kEta;
+  const long cap = (long)A.maxiter * 64 + 100000;
+  const int K = pr ? 2 : 1;
+  for (int k = 0; k < K; k++) {
+    const double* sk = sum + k * kPTrial;
+    const double chi_x = sk[0] + chi_add, lip = sk[1], s_d2 = sk[2], l1n = sk[3], regn = sk[4], tsvd = sk[5];
+    double F = chi_x / 2 + ((!tv && tsv) ? A.lambda_tsv * regn : 0.0);
+    double lhs = lip / 2 + ((!tv && tsv) ? A.lambda_tsv * tsvd : 0.0);
+    if (movie) {  // + lambda_t d_X(xnew); d_X is quadratic, so its step term is d_X(d)
+      F += A.lambda_t * sk[6];
+      lhs += A.lambda_t * sk[7];
+    }
+    const double rhs = bs.c * s_d2 / 2;
+    bs.trial += 1;
+    bs.total_trials += 1;
+    if (bs.total_trials > cap) {  // cannot happen for a convergent run; bounds the launch
+      bs.status = 2;
+      bs.stop = 1;
+      bs.ITER = it + 1;
+      bs.mom_pending = 0;
+      break;
+    }
+    if (!isfinite(F) || !isfinite(lhs)) {
+      bs.status = 1;
+      bs.stop = 1;
+      bs.ITER = it + 1;
+      bs.mom_pending = 0;
+      break;
+    }
+    const bool ok = lhs <= rhs;
+    if (!ok && bs.trial < A.maxiter) {
+      bs.c *= etaf;
+      continue;  // the next trial of this round, or a new round
+    }
+    if (!ok) bs.c *= etaf;
+    if (bs.trial > bs.max_trials) bs.max_trials = bs.trial;
+    bs.kacc = k;
+    bs.c /= kEta;
+    const double mu = bs.mu;
+    const double munew = (1 + sqrt(1 + 4 * mu * mu)) / 2;
+    F += A.lambda_l1 * l1n;
+    if (tv) F += A.lambda_tv * regn;
+    bool stop = false;
+    const double cit = ring[it % (kTD + 1)];
+    if (F < cit) {
+      bs.costtmp = F;
+      bs.mom_a = (1 - mu) / munew;
+      bs.mom_b = 1 + ((mu - 1) / munew);
+      bs.accept = 1;
+      bs.l1_x = l1n;
+    } else {
+      bs.mom_a = 1 - (mu / munew);
+      bs.mom_b = mu / munew;
+      bs.accept = 0;
+      // c/mfista_nufft_lib.c:737 (the movie solver has no such rule)
+      if (!movie && it > 1 && bs.l1_x == 0) stop = true;
+    }
+    if (!stop && it >= kMinIter) {
+      const double d = ring[(it - kTD) % (kTD + 1)] - cit;
+      stop = (A.rel_tol > 0) ? (d <= A.rel_tol * cit) : (d < A.eps_abs);
+    }
+    bs.mu = munew;
+    bs.trial = 0;
+    bs.mom_pending = 1;
+    if (stop) {
+      bs.stop = 1;
+      bs.ITER = it + 1;
+    } else {
+      bs.iter = it + 1;
+      if (bs.iter >= A.maxiter) {
+        bs.stop = 1;
+        bs.ITER = A.maxiter;
+      } else {
+        bs.prologue = 1;
+      }
+    }
+    break;
+  }
 }
 
 // LDS bytes of the batched gridder's entry records
@@ -1096,10 +1203,8 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
     }
     if (!grid_sync(B, epoch, 10)) return;
 
-    // ---------------- control (every workgroup, identical arithmetic)
-    // The trials of a round are taken in order, exactly as the reference's
-    // backtracking loop (c/mfista_nufft_lib.c:692-723) takes them: the first
-    // that passes is accepted; one that fails multiplies c by eta.
+    // ---------------- control (every workgroup, identical arithmetic): the
+    // round's partials reduced in a fixed order, then round_decide
     double sum[2 * kPTrial];
     if (pr && (movie || MSPC > 8)) {
       const int sl[2 * kPTrial] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd, kPTx, kPTd,
@@ -1127,93 +1232,7 @@ __device__ __forceinline__ void mfista_body(const PArgs& A, const int lb, const 
 #pragma unroll
       for (int q = 0; q < kPTrial; q++) sum[q] = s1[q];
     }
-    if (threadIdx.x == 0) {
-      const int it = bs.iter;
-      if (prologue) {
-        ring[it % (kTD + 1)] = bs.costtmp;
-        if (lb == 0) GP(A.cost)[it] = bs.costtmp;
-        bs.prologue = 0;
-      }
-      bs.rounds += 1;
-      const double etaf = (it == 0) ? A.eta_first : kEta;
-      const long cap = (long)A.maxiter * 64 + 100000;
-      const int K = pr ? 2 : 1;
-      for (int k = 0; k < K; k++) {
-        const double* sk = sum + k * kPTrial;
-        const double chi_x = sk[0], lip = sk[1], s_d2 = sk[2], l1n = sk[3], regn = sk[4], tsvd = sk[5];
-        double F = chi_x / 2 + ((!tv && tsv) ? A.lambda_tsv * regn : 0.0);
-        double lhs = lip / 2 + ((!tv && tsv) ? A.lambda_tsv * tsvd : 0.0);
-        if (movie) {  // + lambda_t d_X(xnew); d_X is quadratic, so its step term is d_X(d)
-          F += A.lambda_t * sk[6];
-          lhs += A.lambda_t * sk[7];
-        }
-        const double rhs = bs.c * s_d2 / 2;
-        bs.trial += 1;
-        bs.total_trials += 1;
-        if (bs.total_trials > cap) {  // cannot happen for a convergent run; bounds the launch
-          bs.status = 2;
-          bs.stop = 1;
-          bs.ITER = it + 1;
-          bs.mom_pending = 0;
-          break;
-        }
-        if (!isfinite(F) || !isfinite(lhs)) {
-          bs.status = 1;
-          bs.stop = 1;
-          bs.ITER = it + 1;
-          bs.mom_pending = 0;
-          break;
-        }
-        const bool ok = lhs <= rhs;
-        if (!ok && bs.trial < A.maxiter) {
-          bs.c *= etaf;
-          continue;  // the next trial of this round, or a new round
-        }
-        if (!ok) bs.c *= etaf;
-        if (bs.trial > bs.max_trials) bs.max_trials = bs.trial;
-        bs.kacc = k;
-        bs.c /= kEta;
-        const double mu = bs.mu;
-        const double munew = (1 + sqrt(1 + 4 * mu * mu)) / 2;
-        F += A.lambda_l1 * l1n;
-        if (tv) F += A.lambda_tv * regn;
-        bool stop = false;
-        const double cit = ring[it % (kTD + 1)];
-        if (F < cit) {
-          bs.costtmp = F;
-          bs.mom_a = (1 - mu) / munew;
-          bs.mom_b = 1 + ((mu - 1) / munew);
-          bs.accept = 1;
-          bs.l1_x = l1n;
-        } else {
-          bs.mom_a = 1 - (mu / munew);
-          bs.mom_b = mu / munew;
-          bs.accept = 0;
-          // c/mfista_nufft_lib.c:737 (the movie solver has no such rule)
-          if (!movie && it > 1 && bs.l1_x == 0) stop = true;
-        }
-        if (!stop && it >= kMinIter) {
-          const double d = ring[(it - kTD) % (kTD + 1)] - cit;
-          stop = (A.rel_tol > 0) ? (d <= A.rel_tol * cit) : (d < A.eps_abs);
-        }
-        bs.mu = munew;
-        bs.trial = 0;
-        bs.mom_pending = 1;
-        if (stop) {
-          bs.stop = 1;
-          bs.ITER = it + 1;
-        } else {
-          bs.iter = it + 1;
-          if (bs.iter >= A.maxiter) {
-            bs.stop = 1;
-            bs.ITER = A.maxiter;
-          } else {
-            bs.prologue = 1;
-          }
-        }
-        break;
-      }
-    }
+    if (threadIdx.x == 0) round_decide(A, bs, ring, sum, pr, tv, tsv, movie, (int)lb, prologue, 0.0);
     __syncthreads();
     if (bs.stop) break;
   }

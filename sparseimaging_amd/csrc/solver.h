@@ -70,6 +70,14 @@ struct PersistWork {
   DevMem pargs;            // the solver kernel's arguments (device copy)
 };
 
+// The Toeplitz loop of a single solve (toeplitz.hip, toeplitz_body.h).
+struct ToepWork {
+  NufftState nu;                                // the set-up's 2L x 2L, 24-tap FP64 transform
+  DevMem rb, rp, imgp, imgb, circ, tbuf, fbuf;  // set-up scratch
+  DevMem kh, bimg, tvec, u64;                   // Kh, b, T applied to x / xnew / z, column-pass outputs
+  double wv2 = 0;                               // |W V|^2
+};
+
 // One visibility set of a batch, with its NUFFT set-up.
 struct BatchSet {
   DevMem u, v, vr, vi, sd;
@@ -103,6 +111,7 @@ struct Context {
   DevMem fgp;   // TV prox duals
   DevMem fgpc;  // FGP momentum coefficients
   PersistWork pw;  // the persistent solver's workspace (single solves)
+  ToepWork tw;     // the Toeplitz loop's set-up and workspace (single solves)
   // batched solves: visibility sets (each with its NUFFT set-up) and one
   // workspace per problem
   std::vector<BatchSet> bset;

@@ -539,17 +539,21 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     if (rc == MFISTA_E_FOV) printf("Error: the field of view is too small for the (u,v) coverage.\n");
     return rc;
   }
-  // eps scaled by the mean |V|^2 (c/mfista_nufft_lib.c:923-926)
+  // eps scaled by the mean |V|^2 (c/mfista_nufft_lib.c:923-926); |W V|^2 for
+  // the Toeplitz loop
   SolveArgs B = A;
+  double wv2 = 0;
   {
     double s = 0;
-    std::vector<double> hr(A.M), hi(A.M);
+    std::vector<double> hr(A.M), hi(A.M), hs(A.M);
     MF_CHECK(hipMemcpyAsync(hr.data(), A.vr, A.M * 8, hipMemcpyDeviceToHost, C.stream));
     MF_CHECK(hipMemcpyAsync(hi.data(), A.vi, A.M * 8, hipMemcpyDeviceToHost, C.stream));
+    MF_CHECK(hipMemcpyAsync(hs.data(), A.sd, A.M * 8, hipMemcpyDeviceToHost, C.stream));
     MF_CHECK(hipStreamSynchronize(C.stream));
     for (int i = 0; i < A.M; i++) s += hr[i] * hr[i];
     for (int i = 0; i < A.M; i++) s += hi[i] * hi[i];
     B.eps_abs = A.eps * s / (double)A.M;
+    for (int i = 0; i < A.M; i++) wv2 += (hr[i] * hr[i] + hi[i] * hi[i]) / (hs[i] * hs[i]);
   }
   RunStats rs;
   rs.c = A.cinit;
@@ -588,16 +592,41 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev);
     const int engine = opts ? opts->engine : MFISTA_ENGINE_PERSISTENT;
     rc = kCoopRefused;
-    int used = MFISTA_ENGINE_PERSISTENT;
-    if (engine == MFISTA_ENGINE_PERSISTENT && coop) rc = solve_nufft_persistent(C, P, &rs);
+    int used = MFISTA_ENGINE_PERSISTENT, loop_used = 0;
+    if (engine == MFISTA_ENGINE_PERSISTENT && coop) {
+      // the Toeplitz loop where it applies: L1 and L1+TSV, paired rounds,
+      // floating-point atomics allowed in its set-up, square images up to
+      // 512^2 (the presets size both axes' kernels from NX, so for NX != NY
+      // the reference's transform is not the exact one the loop applies)
+      bool toep = !tv_mode && !C.no_alloc && !P.single_trials && !P.deterministic &&
+                  (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512;
+      if (toep) {
+        const int trc = toeplitz_setup(C, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY, preset, wv2);
+        if (trc == MFISTA_E_SIZE || trc == MFISTA_E_FOV)
+          toep = false;  // the set-up transform does not apply: the gridding loop
+        else if (trc)
+          return trc;
+      }
+      if (toep) {
+        rc = solve_nufft_toeplitz(C, P, &rs);
+        loop_used = MFISTA_LOOP_TOEPLITZ;
+      } else {
+        rc = solve_nufft_persistent(C, P, &rs);
+        loop_used = MFISTA_LOOP_GRIDDING;
+      }
+    }
     if (rc == kCoopRefused && !C.no_alloc) {  // the stream engine: the same loop, one launch per stage
       if (engine == MFISTA_ENGINE_PERSISTENT && coop)
         fprintf(stderr, "[mfista_amd] solving with the stream engine instead\n");
       rs = RunStats();
       used = MFISTA_ENGINE_STREAM;
+      loop_used = 0;
       rc = mfista_run<double>(C, op, P, &rs);
     }
-    if (opts) opts->engine_used = used;
+    if (opts) {
+      opts->engine_used = used;
+      opts->loop_used = loop_used;
+    }
     C.last_steps = rs.steps;
   }
   res->ITER = rs.ITER;

@@ -1,0 +1,378 @@
+// toeplitz_body.h -- device code of the persistent Toeplitz-loop MFISTA solver.
+//
+// For a real image x the data term's normal operator T x = Re A^H W^2 A x is
+// a convolution: (T x)(a) = sum_b x(b) p(a - b) with the point-spread function
+// p(D) = sum_m w_m^2 cos(u_m . D), D in (-N, N)^2.  Embedded in the circulant
+// of the 2NX x 2NY grid it is diagonal in Fourier space, with a real spectrum
+// Kh (p is real and even).  So the gradient at z, b - T z with b = Re A^H W^2 V,
+// and the objective of a trial,
+//   chi^2(x) = |W V|^2 - 2 <x, b> + <x, T x>,
+// need no gridding or degridding inside the loop: every trial applies T to its
+// step d = xnew - z with an FP64 real-to-complex FFT on the padded grid, the
+// product with Kh and the inverse FFT (T xnew = T z + T d, and the momentum
+// carries T x and T xnew as it carries x and xnew).  b, p and so Kh come from
+// a 24-tap FP64 NUFFT adjoint at set-up (toeplitz.hip); to that transform's
+// accuracy the loop is the reference's loop (c/mfista_nufft_lib.c:557-787)
+// with its exact operator.
+//
+// One cooperative launch per solve, L1 and L1+TSV, paired trial rounds.  Per
+// iteration: [momentum] | prox + FP64 row FFTs of the two steps | column FFT,
+// product with Kh, inverse column FFT | c2r rows, T xnew and the trial partials
+// -> control: 1 + 3 r grid barriers for r rounds (the gridding loop of
+// persistent_body.h: 5 + 3 (r - 1)).
+#pragma once
+#include "persistent_body.h"
+
+namespace mf {
+
+// Column pass of the Toeplitz product for ky row `ky` of nseq (1 or 2) row
+// transforms T0, T1 (layout T[ky][i], i < NX, as st_rowfft_fwd writes it):
+// the rows placed on the Lx-point grid as st_colfft_fwd places them, the
+// forward FFT along x, the product with the real kernel row Kh[ky][k], the
+// backward FFT and the NX image rows stored transposed, U[i][ky] (as
+// st_colfft_inv stores them, for st_rowfft_c2r).
+template <typename R>
+__device__ inline void st_colconv(int ky, int nseq, const cplx<R>* __restrict__ T0, const cplx<R>* __restrict__ T1,
+                                  const R* __restrict__ Kh, cplx<R>* __restrict__ U0, cplx<R>* __restrict__ U1,
+                                  int NX, int nky, int Lx, int shift, const FFTPlan& P,
+                                  const cplx<R>* __restrict__ tw, cplx<R>* sm) {
+  using C = cplx<R>;
+  C* b0 = sm;
+  C* b1 = sm + nseq * Lx;
+  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
+    const int q = t >= Lx, a = t - q * Lx;
+    int i = a + shift;
+    if (i >= Lx) i -= Lx;
+    b0[t] = (i < NX) ? (q ? T1 : T0)[(size_t)ky * NX + i] : mk<R>(0, 0);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
+  C* oth = (res == b0) ? b1 : b0;
+  const R* kr = Kh + (size_t)ky * Lx;
+  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
+    const int q = t >= Lx, k = t - q * Lx;
+    const R s = kr[k];
+    const C v = res[t];
+    res[t] = mk<R>(v.x * s, v.y * s);
+  }
+  __syncthreads();
+  res = lds_fft<R>(res, oth, nseq, P, tw, +1);
+  for (int t = threadIdx.x; t < nseq * NX; t += blockDim.x) {
+    const int q = t >= NX, i = t - q * NX;
+    int a = i - shift;
+    if (a < 0) a += Lx;
+    (q ? U1 : U0)[(size_t)i * nky + ky] = res[q * Lx + a];
+  }
+  __syncthreads();
+}
+
+// LDS bytes of the FP64 twiddle tables of both axes
+__host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) { return (size_t)(Lx + Ly) * 16; }
+
+// The solve by workgroups lb = 0 .. nb-1 of a cooperative launch.  TAB: the
+// twiddle tables are staged in LDS.
+template <bool TAB>
+__device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, const unsigned nb) {
+  extern __shared__ double4 smem_d4[];
+  __shared__ double red[16 * kPW];
+  __shared__ BlockState bs;
+  __shared__ double ring[kTD + 1];
+  __shared__ BarCtx B;
+  const GridGeom& g = A.g;
+  const int NX = g.NX, NY = g.NY, n = NX * NY, W = NX, H = NY;
+  const int nky = NY + 1;
+  const bool tsv = A.lambda_tsv > 0.0;
+  unsigned long long epoch = 0;
+  double* prow = GP(A.part) + (int)lb;
+  const size_t pstr = nb;
+
+  if (!bar_setup(B, GP(A.bar), nb, A.hier_barrier)) return;
+  if (threadIdx.x == 0) {
+    B.trace = A.trace;
+    B.ttag = A.ttag;
+    B.tcap = A.tcap;
+    B.trow = (int)lb;
+    B.tnrow = (int)nb;
+  }
+  __syncthreads();
+  char* lds = reinterpret_cast<char*>(smem_d4);
+  const double2 *twxd = GP(A.twxd), *twyd = GP(A.twyd);
+  if constexpr (TAB) {
+    double2* t64x = reinterpret_cast<double2*>(lds);
+    double2* t64y = t64x + g.Lx;
+    for (int i = threadIdx.x; i < g.Lx; i += blockDim.x) t64x[i] = GP(A.twxd)[i];
+    for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) t64y[i] = GP(A.twyd)[i];
+    twxd = t64x;
+    twyd = t64y;
+    lds += (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255;
+    __syncthreads();
+  }
+  double2* sm64 = reinterpret_cast<double2*>(lds);
+  const double* bv = GP(A.bimg);
+  const double* kh = GP(A.kh);
+  const float* boxp = GP(A.box);
+  double* xs = GP(A.x);
+  double* zs = GP(A.z);
+
+  // ---------------- initial cost: z = x0, T x0 -> tv[0] (x) and tv[2] (z)
+  {
+    double st[2] = {0, 0};
+    for (int t = lb; t < A.n_row; t += nb) {
+      auto load = [&](int i, int j) -> double {
+        const int f = i * NY + j;
+        const double xc = xs[f];
+        zs[f] = xc;
+        st[0] += fabs(xc);
+        const int c = f % W, r = f / W;
+        double dh = 0, dv = 0;
+        if (c < W - 1) dh = xc - xs[f + 1];
+        if (r < H - 1) dv = xc - xs[f + W];
+        st[1] += dh * dh + dv * dv;
+        return xc;
+      };
+      st_rowfft_fwd<double>(t, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), sm64, load);
+    }
+    block_sum<2>(st, red);
+    if (threadIdx.x == 0) {
+      prow[(kPIl1) * pstr] = st[0];
+      prow[(kPItsv) * pstr] = st[1];
+    }
+  }
+  if (!grid_sync(B, epoch, 1)) return;
+  for (int t = lb; t < nky; t += nb)
+    st_colconv<double>(t, 1, GP(A.T64), GP(A.T64), kh, GP(A.u64a), GP(A.u64a), NX, nky, g.Lx, NX / 2, A.Px, twxd,
+                       sm64);
+  if (!grid_sync(B, epoch, 2)) return;
+  {
+    double v[1] = {0};
+    double* tx = GP(A.tv[0]);
+    double* tz = GP(A.tv[2]);
+    for (int t = lb; t < A.n_row; t += nb) {
+      auto store = [&](int i, int j, double val) {
+        const int f = i * NY + j;
+        tx[f] = val;
+        tz[f] = val;
+        v[0] += xs[f] * (val - 2.0 * bv[f]);
+      };
+      st_rowfft_c2r<double>(t, 1, GP(A.u64a), NX, NY, g.Ly, NY / 2, A.Py, twyd, sm64, store);
+    }
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) prow[(kPIchi) * pstr] = v[0];
+  }
+  if (!grid_sync(B, epoch, 3)) return;
+  {
+    const int slot[3] = {kPIchi, kPIl1, kPItsv};
+    double sum[3];
+    part_reduce<3>(GP(A.part), nb, slot, sum, red);
+    if (threadIdx.x == 0) {
+      double c0 = (A.wv2 + sum[0]) / 2 + A.lambda_l1 * sum[1];
+      if (tsv) c0 += A.lambda_tsv * sum[2];
+      bs.c = A.cinit;
+      bs.mu = 1.0;
+      bs.costtmp = c0;
+      bs.l1_x = sum[1];
+      bs.iter = 0;
+      bs.trial = 0;
+      bs.accept = 0;
+      bs.mom_pending = 0;
+      bs.stop = 0;
+      bs.ITER = 0;
+      bs.max_trials = 0;
+      bs.total_trials = 0;
+      bs.status = 0;
+      bs.prologue = 1;
+      bs.ix = 0;
+      bs.in = 1;
+      bs.iz = 2;
+      bs.in1 = 3;
+      bs.kacc = 0;
+      bs.rounds = 0;
+    }
+    __syncthreads();
+  }
+
+  // ---------------- the MFISTA loop
+  while (true) {
+    const int prologue = bs.prologue;
+    if (prologue && bs.mom_pending) {
+      // momentum: z = a x + b xnew, T z = a T x + b T xnew; x <- xnew if accepted
+      const double ma = bs.mom_a, mb = bs.mom_b;
+      const bool acc = bs.accept != 0;
+      const int ka = bs.kacc;
+      const double* xnw = GP(A.xn[ka]);
+      const double* tx = GP(A.tv[bs.ix]);
+      const double* tn = GP(A.tv[ka ? bs.in1 : bs.in]);
+      double* tz = GP(A.tv[bs.iz]);
+      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
+        const double xf = xs[f], xn = xnw[f];
+        zs[f] = ma * xf + mb * xn;
+        tz[f] = ma * tx[f] + mb * tn[f];
+        if (acc) xs[f] = xn;
+      }
+      if (!grid_sync(B, epoch, 4)) return;
+      if (threadIdx.x == 0 && acc) {  // T x <- T xnew: swap roles
+        int& rn = ka ? bs.in1 : bs.in;
+        const int tmp = bs.ix;
+        bs.ix = rn;
+        rn = tmp;
+      }
+      __syncthreads();
+    }
+    // T1: gradient at z, the two trials' prox images and steps, FP64 row FFTs
+    // of the steps (c and c * eta side by side)
+    const double c = bs.c;
+    const double etaf = (bs.iter == 0) ? A.eta_first : kEta;
+    const double c1 = c * etaf;  // the value the control step gives c after a failed trial
+    const double ci0 = 1.0 / c, e0 = A.lambda_l1 / c, ci1 = 1.0 / c1, e1 = A.lambda_l1 / c1;
+    const double lt = A.lambda_tsv;
+    const double* tz = GP(A.tv[bs.iz]);
+    {
+      double v[4] = {0, 0, 0, 0};
+      double* xa_out = GP(A.xn[0]);
+      double* xb_out = GP(A.xn[1]);
+      for (int t = lb; t < A.n_row; t += nb) {
+        auto load2 = [&](int i, int j, double& o0, double& o1) {
+          const int f = i * NY + j;
+          const double zf = zs[f];
+          double gf = bv[f] - tz[f];
+          if (tsv) {  // - lambda_tsv grad TSV(z)
+            const int cc = f % W, r = f / W;
+            double d = 0.0;
+            if (cc < W - 1) d += zf - zs[f + 1];
+            if (cc > 0) d += zf - zs[f - 1];
+            if (r < H - 1) d += zf - zs[f + W];
+            if (r > 0) d += zf - zs[f - W];
+            gf -= lt * (2.0 * d);
+          }
+          const bool boxed = boxp && boxp[f] == 0.0f;
+          const double xa = prox_p<double>(gf * ci0 + zf, e0, A.nonneg, boxed);
+          const double xb = prox_p<double>(gf * ci1 + zf, e1, A.nonneg, boxed);
+          xa_out[f] = xa;
+          xb_out[f] = xb;
+          const double da = xa - zf, db = xb - zf;
+          v[0] += da * da;
+          v[1] += fabs(xa);
+          v[2] += db * db;
+          v[3] += fabs(xb);
+          o0 = da;
+          o1 = db;
+        };
+        st_rowfft_fwd_pair<double>(t, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), GP(A.T64b), sm64, load2);
+      }
+      block_sum<4>(v, red);
+      if (threadIdx.x == 0) {
+        prow[(kPD2) * pstr] = v[0];
+        prow[(kPL1) * pstr] = v[1];
+        prow[(kPTrial + kPD2) * pstr] = v[2];
+        prow[(kPTrial + kPL1) * pstr] = v[3];
+      }
+    }
+    if (!grid_sync(B, epoch, 8)) return;
+    // T2: column FFT, product with Kh, inverse column FFT (both trials), and
+    // the regulariser partials of xnew and d; when at least half as many
+    // workgroups as pixels / 256 have no column task, those take the partials
+    {
+      const int nreg = (int)nb - nky;
+      const bool split = nreg > 0 && 2 * (long)nreg * blockDim.x >= n;
+      const bool do_reg = !split || (int)lb >= nky;
+      const int rg0 = split ? (int)lb - nky : (int)lb;
+      const int rgn = split ? nreg : (int)nb;
+      for (int t = lb; t < nky; t += nb)
+        st_colconv<double>(t, 2, GP(A.T64), GP(A.T64b), kh, GP(A.u64a), GP(A.u64b), NX, nky, g.Lx, NX / 2, A.Px,
+                           twxd, sm64);
+      double r[4] = {0, 0, 0, 0};
+      for (int f = rg0 * blockDim.x + threadIdx.x; do_reg && f < n; f += rgn * blockDim.x) {
+        reg_terms(GP(A.xn[0]), zs, f, W, H, false, r);
+        reg_terms(GP(A.xn[1]), zs, f, W, H, false, r + 2);
+      }
+      block_sum<4>(r, red);
+      if (threadIdx.x == 0) {
+        prow[(kPReg) * pstr] = r[0];
+        prow[(kPTsvd) * pstr] = r[1];
+        prow[(kPTrial + kPReg) * pstr] = r[2];
+        prow[(kPTrial + kPTsvd) * pstr] = r[3];
+      }
+    }
+    if (!grid_sync(B, epoch, 9)) return;
+    // T3: c2r rows of both trials: T d, T xnew = T z + T d, chi^2(xnew) - |W V|^2
+    // and |W A d|^2 = <d, T d>; task 2 q + k is row task q of trial k
+    {
+      double v[4] = {0, 0, 0, 0};
+      for (int t = lb; t < 2 * A.n_row; t += nb) {
+        const int k = t & 1;
+        const double* xk = GP(A.xn[k]);
+        double* tk = GP(A.tv[k ? bs.in1 : bs.in]);
+        double chi = 0.0, lip = 0.0;
+        auto store = [&](int i, int j, double td) {
+          const int f = i * NY + j;
+          const double xv = xk[f];
+          const double tn = tz[f] + td;
+          tk[f] = tn;
+          lip += (xv - zs[f]) * td;
+          chi += xv * (tn - 2.0 * bv[f]);
+        };
+        st_rowfft_c2r<double>(t >> 1, 1, k ? GP(A.u64b) : GP(A.u64a), NX, NY, g.Ly, NY / 2, A.Py, twyd, sm64, store);
+        if (k) {  // (no runtime index into v: it stays in registers)
+          v[2] += chi;
+          v[3] += lip;
+        } else {
+          v[0] += chi;
+          v[1] += lip;
+        }
+      }
+      block_sum<4>(v, red);
+      if (threadIdx.x == 0) {
+        prow[(kPChi) * pstr] = v[0];
+        prow[(kPLip) * pstr] = v[1];
+        prow[(kPTrial + kPChi) * pstr] = v[2];
+        prow[(kPTrial + kPLip) * pstr] = v[3];
+      }
+    }
+    if (!grid_sync(B, epoch, 10)) return;
+    // ---------------- control (every workgroup, identical arithmetic)
+    double sum[2 * kPTrial];
+    {
+      const int sl[12] = {kPChi, kPLip, kPD2, kPL1, kPReg, kPTsvd,
+                          kPTrial + kPChi, kPTrial + kPLip, kPTrial + kPD2, kPTrial + kPL1,
+                          kPTrial + kPReg, kPTrial + kPTsvd};
+      double s1[12];
+      part_reduce<12>(GP(A.part), nb, sl, s1, red);
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) sum[k * kPTrial + q] = s1[k * 6 + q];
+        sum[k * kPTrial + kPTx] = 0.0;
+        sum[k * kPTrial + kPTd] = 0.0;
+      }
+    }
+    if (threadIdx.x == 0) round_decide(A, bs, ring, sum, true, false, tsv, false, (int)lb, prologue, A.wv2);
+    __syncthreads();
+    if (bs.stop) break;
+  }
+
+  // the accepted image of the last iteration (the reference's x = xnew)
+  if (bs.mom_pending && bs.accept) {
+    const double* xnw = GP(A.xn[bs.kacc]);
+    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) xs[f] = xnw[f];
+  }
+  if (lb == 0 && threadIdx.x == 0) {
+    DevState* S = A.S;
+    S->c = bs.c;
+    S->mu = bs.mu;
+    S->costtmp = bs.costtmp;
+    S->ITER = bs.ITER;
+    S->iter = bs.iter;
+    S->max_trials = bs.max_trials;
+    S->total_trials = bs.total_trials;
+    S->status = bs.status;
+    S->rounds = bs.rounds;
+    S->done = 1;
+  }
+}
+
+template <bool TAB>
+__global__ void __launch_bounds__(MF_BLOCK) k_mfista_toeplitz(const PArgs* __restrict__ A) {
+  toeplitz_body<TAB>(*A, blockIdx.x, gridDim.x);
+}
+
+}  // namespace mf

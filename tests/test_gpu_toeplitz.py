@@ -1,0 +1,122 @@
+"""The Toeplitz loop of single NUFFT solves against the FP64 oracle and the
+gridding loop.
+
+The loop applies the data term's normal operator A^H W^2 A as a convolution
+(FP64 FFTs on the padded grid, kernel and A^H W^2 V from a 24-tap FP64 NUFFT
+adjoint at set-up) instead of gridding and degridding every iteration; the
+reference (c/mfista_nufft_lib.c:557-787) evaluates the same operator with its
+preset's NUFFT.  Parity bar as for every NUFFT solve: image relative L2
+<= 1e-3 and final objective (the FP64 RESULT pass, with the preset's own
+NUFFT) within 1e-6 of the oracle's.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import sparseimaging_amd as sa
+from sparseimaging_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(P, NX, NY, preset, maxiter, rel_tol, lambda_l1, lambda_tsv, nonneg=1, box=None):
+    return oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], NX, NY, maxiter, 0.0,
+                                     lambda_l1, 0.0, lambda_tsv, 1e4, nonneg=nonneg, cl_box=box, preset=preset,
+                                     rel_tol=rel_tol)
+
+
+def _gpu(P, NX, NY, preset, maxiter, rel_tol, lambda_l1, lambda_tsv, nonneg=1, box=None, loop=sa.LOOP_AUTO):
+    return sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], NX, NY, maxiter=maxiter,
+                                        eps=0.0, lambda_l1=lambda_l1, lambda_tsv=lambda_tsv, cinit=1e4,
+                                        nonneg_flag=nonneg, cl_box=box, preset=preset, rel_tol=rel_tol, loop=loop)
+
+
+def _compare(xg, rg, o):
+    xo, ro = o["x"], o["result"]
+    l2 = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    dc = abs(rg["finalcost"] - ro["finalcost"]) / abs(ro["finalcost"])
+    return l2, dc
+
+
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_toeplitz_converged_matches_oracle(preset):
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    print(f"case toeplitz converged preset={preset} N=64 M=2000 rel_tol=1e-9", flush=True)
+    o = _oracle(P, 64, 64, preset, 20000, 1e-9, 1.0, 10.0)
+    xg, rg, info = _gpu(P, 64, 64, preset, 20000, 1e-9, 1.0, 10.0)
+    l2, dc = _compare(xg, rg, o)
+    print(f"  loop {info['loop_used']} ITER gpu={rg['ITER']} oracle={o['result']['ITER']} relL2={l2:.2e} "
+          f"cost rel={dc:.2e} kernel {info['kernel_ms']:.2f} ms")
+    assert info["loop_used"] == sa.LOOP_TOEPLITZ
+    assert l2 <= 1e-3 and dc <= 1e-6
+
+
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_toeplitz_and_gridding_loops_at_fixed_iterations(preset):
+    """Both loops and the oracle for the same 4000 iterations (past
+    convergence) of a 96^2 problem with lambda_tsv = 100."""
+    P = synth.make_nufft_problem(N=96, M=3000, seed=555, amax=0.35)
+    print(f"case toeplitz vs gridding preset={preset} N=96 M=3000 4000 iterations", flush=True)
+    o = _oracle(P, 96, 96, preset, 4000, 0.0, 1.0, 100.0)
+    xt, rt, it = _gpu(P, 96, 96, preset, 4000, 0.0, 1.0, 100.0)
+    xg, rg, ig = _gpu(P, 96, 96, preset, 4000, 0.0, 1.0, 100.0, loop=sa.LOOP_GRIDDING)
+    lt, dt = _compare(xt, rt, o)
+    lg, dg = _compare(xg, rg, o)
+    print(f"  toeplitz: relL2={lt:.2e} cost rel={dt:.2e} ({it['kernel_ms']:.1f} ms); gridding: relL2={lg:.2e} "
+          f"cost rel={dg:.2e} ({ig['kernel_ms']:.1f} ms)")
+    assert it["loop_used"] == sa.LOOP_TOEPLITZ and ig["loop_used"] == sa.LOOP_GRIDDING
+    assert rt["ITER"] == 4000 and rg["ITER"] == 4000
+    assert lt <= 1e-3 and dt <= 1e-6
+    assert lg <= 1e-3 and dg <= 1e-6
+
+
+@pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
+def test_toeplitz_signed_with_box(preset):
+    """nonneg_flag = 0 (the signed soft threshold, c/mfista_tools.c:81-112)
+    with a cl_box window, lambda_tsv = 30, 5000 iterations."""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=77, amax=0.35)
+    box = np.zeros((64, 64), dtype=np.float32)
+    box[8:56, 10:54] = 1.0
+    print(f"case toeplitz signed+box preset={preset} N=64 M=2000 5000 iterations", flush=True)
+    o = _oracle(P, 64, 64, preset, 5000, 0.0, 1.0, 30.0, nonneg=0, box=box)
+    xg, rg, info = _gpu(P, 64, 64, preset, 5000, 0.0, 1.0, 30.0, nonneg=0, box=box)
+    l2, dc = _compare(xg, rg, o)
+    print(f"  loop {info['loop_used']} relL2={l2:.2e} cost rel={dc:.2e} min x {xg.min():.3e}")
+    assert info["loop_used"] == sa.LOOP_TOEPLITZ
+    assert np.all(xg[box == 0] == 0.0)
+    assert xg.min() < 0.0  # the signed threshold does produce negative pixels here
+    assert l2 <= 1e-3 and dc <= 1e-6
+
+
+def test_toeplitz_l1_only():
+    """lambda_tsv = 0: the L1 solve (the image is the unique minimiser only up
+    to the data term's null space, so the objective is the bar)."""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=99, amax=0.35)
+    print("case toeplitz L1 only N=64 M=2000 3000 iterations", flush=True)
+    o = _oracle(P, 64, 64, sa.PRESET_CXX12, 3000, 0.0, 5.0, 0.0)
+    xg, rg, info = _gpu(P, 64, 64, sa.PRESET_CXX12, 3000, 0.0, 5.0, 0.0)
+    l2, dc = _compare(xg, rg, o)
+    print(f"  loop {info['loop_used']} relL2={l2:.2e} cost rel={dc:.2e}")
+    assert info["loop_used"] == sa.LOOP_TOEPLITZ
+    assert dc <= 1e-6
+
+
+def test_loop_selection():
+    """TV solves, deterministic mode, single-trial rounds and rectangular
+    images keep the gridding loop (the presets size both axes' kernels from
+    NX, so for NX != NY the reference's transform is not the exact one the
+    Toeplitz loop applies); LOOP_GRIDDING forces it."""
+    P = synth.make_nufft_problem(N=32, M=600, seed=5, amax=0.3)
+    base = dict(maxiter=50, eps=0.0, lambda_l1=1.0, cinit=1e4, nonneg_flag=1)
+    args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 32, 32)
+    print("case loop selection N=32", flush=True)
+    _, _, i0 = sa.mfista_imaging_core_nufft(*args, lambda_tsv=1.0, preset=sa.PRESET_CXX12, **base)
+    _, _, i1 = sa.mfista_imaging_core_nufft(*args, lambda_tv=1.0, preset=sa.PRESET_C24, **base)
+    _, _, i2 = sa.mfista_imaging_core_nufft(*args, lambda_tsv=1.0, preset=sa.PRESET_CXX12, deterministic=True, **base)
+    _, _, i3 = sa.mfista_imaging_core_nufft(*args, lambda_tsv=1.0, preset=sa.PRESET_CXX12, single_trials=True, **base)
+    _, _, i4 = sa.mfista_imaging_core_nufft(*args, lambda_tsv=1.0, preset=sa.PRESET_CXX12, loop=sa.LOOP_GRIDDING,
+                                            **base)
+    _, _, i5 = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 32, 48,
+                                            lambda_tsv=1.0, preset=sa.PRESET_CXX12, **base)
+    assert i0["loop_used"] == sa.LOOP_TOEPLITZ
+    assert [i["loop_used"] for i in (i1, i2, i3, i4, i5)] == [sa.LOOP_GRIDDING] * 5
