@@ -422,10 +422,9 @@ int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
   L.pair = 1;
   // workspace
   const size_t t64 = (size_t)nky * NX * sizeof(double2);
-  if (W.px.ensure((size_t)n * 8) || W.pz.ensure((size_t)n * 8) || W.pxn.ensure((size_t)2 * n * 8) ||
-      W.T64.ensure(2 * t64) || W.part.ensure((size_t)L.nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
+  if (W.px.ensure((size_t)n * 8) || W.pz.ensure((size_t)n * 8) || W.T64.ensure(2 * t64) || W.part.ensure((size_t)L.nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
       W.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || W.bar.ensure(kBarBytes) ||
-      T.tvec.ensure((size_t)4 * n * 8) || T.u64.ensure(2 * t64))
+      T.tvec.ensure((size_t)4 * n * 8) || T.u64.ensure(2 * t64) || T.xrot.ensure((size_t)4 * n * 8))
     return MFISTA_E_HIP;
   if (P.xinit != W.px.p)
     MF_CHECK(hipMemcpyAsync(W.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
@@ -442,8 +441,6 @@ int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
   A.pair = 1;
   A.x = (double*)W.px.p;
   A.z = (double*)W.pz.p;
-  A.xn[0] = (double*)W.pxn.p;
-  A.xn[1] = (double*)W.pxn.p + n;
   A.box = P.box_flag ? P.box : nullptr;
   A.maxiter = P.maxiter;
   A.nonneg = P.nonneg;
@@ -468,6 +465,7 @@ int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
   A.u64a = (double2*)T.u64.p;
   A.u64b = (double2*)((char*)T.u64.p + t64);
   A.wv2 = T.wv2;
+  for (int q = 0; q < 4; q++) A.xb[q] = (double*)T.xrot.p + (size_t)q * n;
   const char* trace_path = getenv("MFISTA_PERSISTENT_TRACE");
   const int tcap = 16384;
   if (trace_path && *trace_path) {

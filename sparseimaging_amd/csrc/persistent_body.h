@@ -97,6 +97,7 @@ struct PArgs {
   double* tv[4];
   double2 *u64a, *u64b;
   double wv2;
+  double* xb[4];  // image buffers whose roles rotate (BlockState::xr)
 };
 
 // Grid-wide barriers for a cooperative launch.  bar[] layout (64-bit words):
@@ -297,6 +298,7 @@ struct BlockState {
   int in1;         // W A xnew of the second trial of a paired round
   int kacc;        // trial of the last round that was accepted (0 or 1): xn[kacc]
   int rounds;      // trial rounds evaluated
+  int xr[4];       // Toeplitz loop: image buffers of x, the two trials' xnew, a spare
 };
 
 // regulariser sums at pixel f over its right and lower neighbours:

@@ -75,6 +75,7 @@ struct ToepWork {
   NufftState nu;                                // the set-up's 2L x 2L, 24-tap FP64 transform
   DevMem rb, rp, imgp, imgb, circ, tbuf, fbuf;  // set-up scratch
   DevMem kh, bimg, tvec, u64;                   // Kh, b, T applied to x / xnew / z, column-pass outputs
+  DevMem xrot;                                  // the four image buffers of x and the trials' xnew
   double wv2 = 0;                               // |W V|^2
 };
 

@@ -16,10 +16,14 @@
 // with its exact operator.
 //
 // One cooperative launch per solve, L1 and L1+TSV, paired trial rounds.  Per
-// iteration: [momentum] | prox + FP64 row FFTs of the two steps | column FFT,
-// product with Kh, inverse column FFT | c2r rows, T xnew and the trial partials
-// -> control: 1 + 3 r grid barriers for r rounds (the gridding loop of
-// persistent_body.h: 5 + 3 (r - 1)).
+// round: momentum (first round of an iteration) + gradient + prox + FP64 row
+// FFTs of the two steps | column FFT, product with Kh, inverse column FFT |
+// c2r rows, T xnew and the trial partials -> control: 3 r grid barriers for
+// r rounds (the gridding loop of persistent_body.h: 5 + 3 (r - 1)).  The
+// momentum needs no stage of its own: the images live in four buffers whose
+// roles rotate (x, the two trials' xnew, a spare), so the first round reads
+// x and the accepted xnew, which no workgroup writes in that stage, and forms
+// z (also at the neighbours its TSV gradient needs) as it goes.
 #pragma once
 #include "persistent_body.h"
 
@@ -122,6 +126,7 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
         const int f = i * NY + j;
         const double xc = xs[f];
         zs[f] = xc;
+        GP(A.xb[0])[f] = xc;
         st[0] += fabs(xc);
         const int c = f % W, r = f / W;
         double dh = 0, dv = 0;
@@ -187,6 +192,10 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
       bs.in1 = 3;
       bs.kacc = 0;
       bs.rounds = 0;
+      bs.xr[0] = 0;
+      bs.xr[1] = 1;
+      bs.xr[2] = 2;
+      bs.xr[3] = 3;
     }
     __syncthreads();
   }
@@ -194,30 +203,34 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
   // ---------------- the MFISTA loop
   while (true) {
     const int prologue = bs.prologue;
-    if (prologue && bs.mom_pending) {
-      // momentum: z = a x + b xnew, T z = a T x + b T xnew; x <- xnew if accepted
-      const double ma = bs.mom_a, mb = bs.mom_b;
-      const bool acc = bs.accept != 0;
-      const int ka = bs.kacc;
-      const double* xnw = GP(A.xn[ka]);
-      const double* tx = GP(A.tv[bs.ix]);
-      const double* tn = GP(A.tv[ka ? bs.in1 : bs.in]);
-      double* tz = GP(A.tv[bs.iz]);
-      for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) {
-        const double xf = xs[f], xn = xnw[f];
-        zs[f] = ma * xf + mb * xn;
-        tz[f] = ma * tx[f] + mb * tn[f];
-        if (acc) xs[f] = xn;
-      }
-      if (!grid_sync(B, epoch, 4)) return;
-      if (threadIdx.x == 0 && acc) {  // T x <- T xnew: swap roles
-        int& rn = ka ? bs.in1 : bs.in;
+    // momentum of the first round of an iteration: z = a x + b xnew and
+    // T z = a T x + b T xnew, with xnew the last accepted-by-the-test trial;
+    // x <- xnew when its objective decreased.  The roles read here are the
+    // ones before the rotation below.
+    const bool mom = prologue && bs.mom_pending;
+    const double ma = bs.mom_a, mb = bs.mom_b;
+    const int ka = bs.kacc;
+    const double* xo = GP(A.xb[bs.xr[0]]);            // x
+    const double* xq = GP(A.xb[bs.xr[ka ? 2 : 1]]);   // xnew
+    const double* tx = GP(A.tv[bs.ix]);
+    const double* tq = GP(A.tv[ka ? bs.in1 : bs.in]);
+    __syncthreads();  // every thread has read the roles
+    if (mom && threadIdx.x == 0) {
+      const int rx = bs.xr[0], rq = bs.xr[ka ? 2 : 1], ro = bs.xr[ka ? 1 : 2], rs = bs.xr[3];
+      if (bs.accept) {  // x <- xnew (buffer rq); the old x is not written before the next momentum
+        bs.xr[0] = rq;
+        bs.xr[3] = rx;
+        int& rn = ka ? bs.in1 : bs.in;  // T x <- T xnew: swap roles
         const int tmp = bs.ix;
         bs.ix = rn;
         rn = tmp;
+      } else {
+        bs.xr[3] = rq;
       }
-      __syncthreads();
+      bs.xr[1] = ro;  // the trials write the two buffers nobody reads in this stage
+      bs.xr[2] = rs;
     }
+    __syncthreads();
     // T1: gradient at z, the two trials' prox images and steps, FP64 row FFTs
     // of the steps (c and c * eta side by side)
     const double c = bs.c;
@@ -225,23 +238,35 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
     const double c1 = c * etaf;  // the value the control step gives c after a failed trial
     const double ci0 = 1.0 / c, e0 = A.lambda_l1 / c, ci1 = 1.0 / c1, e1 = A.lambda_l1 / c1;
     const double lt = A.lambda_tsv;
-    const double* tz = GP(A.tv[bs.iz]);
+    double* tz = GP(A.tv[bs.iz]);
+    double* xa_out = GP(A.xb[bs.xr[1]]);
+    double* xb_out = GP(A.xb[bs.xr[2]]);
     {
       double v[4] = {0, 0, 0, 0};
-      double* xa_out = GP(A.xn[0]);
-      double* xb_out = GP(A.xn[1]);
       for (int t = lb; t < A.n_row; t += nb) {
         auto load2 = [&](int i, int j, double& o0, double& o1) {
           const int f = i * NY + j;
-          const double zf = zs[f];
-          double gf = bv[f] - tz[f];
+          // z and T z at f; z at a neighbour (formed from x and xnew in the
+          // momentum round, read back otherwise)
+          double zf, tzf;
+          if (mom) {
+            zf = ma * xo[f] + mb * xq[f];
+            tzf = ma * tx[f] + mb * tq[f];
+            zs[f] = zf;
+            tz[f] = tzf;
+          } else {
+            zf = zs[f];
+            tzf = tz[f];
+          }
+          auto zat = [&](int q) -> double { return mom ? ma * xo[q] + mb * xq[q] : zs[q]; };
+          double gf = bv[f] - tzf;
           if (tsv) {  // - lambda_tsv grad TSV(z)
             const int cc = f % W, r = f / W;
             double d = 0.0;
-            if (cc < W - 1) d += zf - zs[f + 1];
-            if (cc > 0) d += zf - zs[f - 1];
-            if (r < H - 1) d += zf - zs[f + W];
-            if (r > 0) d += zf - zs[f - W];
+            if (cc < W - 1) d += zf - zat(f + 1);
+            if (cc > 0) d += zf - zat(f - 1);
+            if (r < H - 1) d += zf - zat(f + W);
+            if (r > 0) d += zf - zat(f - W);
             gf -= lt * (2.0 * d);
           }
           const bool boxed = boxp && boxp[f] == 0.0f;
@@ -282,8 +307,8 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
                            twxd, sm64);
       double r[4] = {0, 0, 0, 0};
       for (int f = rg0 * blockDim.x + threadIdx.x; do_reg && f < n; f += rgn * blockDim.x) {
-        reg_terms(GP(A.xn[0]), zs, f, W, H, false, r);
-        reg_terms(GP(A.xn[1]), zs, f, W, H, false, r + 2);
+        reg_terms(xa_out, zs, f, W, H, false, r);
+        reg_terms(xb_out, zs, f, W, H, false, r + 2);
       }
       block_sum<4>(r, red);
       if (threadIdx.x == 0) {
@@ -300,7 +325,7 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
       double v[4] = {0, 0, 0, 0};
       for (int t = lb; t < 2 * A.n_row; t += nb) {
         const int k = t & 1;
-        const double* xk = GP(A.xn[k]);
+        const double* xk = k ? xb_out : xa_out;
         double* tk = GP(A.tv[k ? bs.in1 : bs.in]);
         double chi = 0.0, lip = 0.0;
         auto store = [&](int i, int j, double td) {
@@ -350,10 +375,12 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
     if (bs.stop) break;
   }
 
-  // the accepted image of the last iteration (the reference's x = xnew)
-  if (bs.mom_pending && bs.accept) {
-    const double* xnw = GP(A.xn[bs.kacc]);
-    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) xs[f] = xnw[f];
+  // the final image (x, or the accepted image of the last iteration: the
+  // reference's x = xnew) into the output buffer
+  {
+    const int rf = (bs.mom_pending && bs.accept) ? bs.xr[bs.kacc ? 2 : 1] : bs.xr[0];
+    const double* xf = GP(A.xb[rf]);
+    for (int f = lb * blockDim.x + threadIdx.x; f < n; f += nb * blockDim.x) xs[f] = xf[f];
   }
   if (lb == 0 && threadIdx.x == 0) {
     DevState* S = A.S;
