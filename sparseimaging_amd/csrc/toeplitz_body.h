@@ -41,8 +41,18 @@ __device__ inline void st_colconv(int ky, int nseq, const cplx<R>* __restrict__ 
                                   int NX, int nky, int Lx, int shift, const FFTPlan& P,
                                   const cplx<R>* __restrict__ tw, cplx<R>* sm) {
   using C = cplx<R>;
+  constexpr int kPre = 8;  // kernel values a thread prefetches (Lx <= kPre * blockDim)
   C* b0 = sm;
   C* b1 = sm + nseq * Lx;
+  // the kernel row is read before the forward transform, so that its load
+  // latency overlaps the transform
+  const R* kr = Kh + (size_t)ky * Lx;
+  R kv[kPre];
+#pragma unroll
+  for (int m = 0; m < kPre; m++) {
+    const int k = threadIdx.x + m * blockDim.x;
+    kv[m] = (k < Lx) ? kr[k] : (R)0;
+  }
   for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
     const int q = t >= Lx, a = t - q * Lx;
     int i = a + shift;
@@ -52,12 +62,15 @@ __device__ inline void st_colconv(int ky, int nseq, const cplx<R>* __restrict__ 
   __syncthreads();
   C* res = lds_fft<R>(b0, b1, nseq, P, tw, -1);
   C* oth = (res == b0) ? b1 : b0;
-  const R* kr = Kh + (size_t)ky * Lx;
-  for (int t = threadIdx.x; t < nseq * Lx; t += blockDim.x) {
-    const int q = t >= Lx, k = t - q * Lx;
-    const R s = kr[k];
-    const C v = res[t];
-    res[t] = mk<R>(v.x * s, v.y * s);
+#pragma unroll
+  for (int m = 0; m < kPre; m++) {
+    const int k = threadIdx.x + m * blockDim.x;
+    if (k < Lx) {
+      for (int q = 0; q < nseq; q++) {
+        const C v = res[q * Lx + k];
+        res[q * Lx + k] = mk<R>(v.x * kv[m], v.y * kv[m]);
+      }
+    }
   }
   __syncthreads();
   res = lds_fft<R>(res, oth, nseq, P, tw, +1);
@@ -66,6 +79,83 @@ __device__ inline void st_colconv(int ky, int nseq, const cplx<R>* __restrict__ 
     int a = i - shift;
     if (a < 0) a += Lx;
     (q ? U1 : U0)[(size_t)i * nky + ky] = res[q * Lx + a];
+  }
+  __syncthreads();
+}
+
+// st_colconv for one transform, with the two real ky rows packed: task ky = 0
+// carries rows 0 and NY (= nky - 1) together, as w = T[0] + i T[NY] (both
+// rows of a real image's half spectrum are real).  After the forward FFT the
+// two spectra are W(k) = F0(k) + i FN(k) with F0, FN Hermitian, so
+//   K0 F0 + i KN FN = W (K0 + KN) / 2 + conj(W(-k)) (K0 - KN) / 2,
+// and the backward FFT returns the two rows' products as its real and
+// imaginary parts (stored with zero imaginary parts, which the c2r pass
+// ignores at ky = 0 and NY).  Tasks ky = 1 .. NY-1 are ordinary columns, so a
+// transform takes NY tasks instead of NY + 1.
+template <typename R>
+__device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const R* __restrict__ Kh,
+                                   cplx<R>* __restrict__ U, int NX, int nky, int Lx, int shift, const FFTPlan& P,
+                                   const cplx<R>* __restrict__ tw, cplx<R>* sm) {
+  using C = cplx<R>;
+  constexpr int kPre = 4;  // kernel values a thread prefetches (Lx <= kPre * blockDim)
+  const int kyN = nky - 1;
+  const bool packed = (ky == 0);
+  C* b0 = sm;
+  C* b1 = sm + Lx;
+  const R* kr = Kh + (size_t)ky * Lx;
+  const R* kn = Kh + (size_t)kyN * Lx;
+  R kv[kPre], kw[kPre];
+#pragma unroll
+  for (int m = 0; m < kPre; m++) {
+    const int k = threadIdx.x + m * blockDim.x;
+    kv[m] = (k < Lx) ? kr[k] : (R)0;
+    kw[m] = (packed && k < Lx) ? kn[k] : (R)0;
+  }
+  for (int a = threadIdx.x; a < Lx; a += blockDim.x) {
+    int i = a + shift;
+    if (i >= Lx) i -= Lx;
+    C v = mk<R>(0, 0);
+    if (i < NX) v = packed ? mk<R>(T[i].x, T[(size_t)kyN * NX + i].x) : T[(size_t)ky * NX + i];
+    b0[a] = v;
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 1, P, tw, -1);
+  C* oth = (res == b0) ? b1 : b0;
+  if (!packed) {
+#pragma unroll
+    for (int m = 0; m < kPre; m++) {
+      const int k = threadIdx.x + m * blockDim.x;
+      if (k < Lx) {
+        const C v = res[k];
+        res[k] = mk<R>(v.x * kv[m], v.y * kv[m]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < kPre; m++) {
+      const int k = threadIdx.x + m * blockDim.x;
+      if (k < Lx) {
+        const C w = res[k], wm = res[k == 0 ? 0 : Lx - k];  // W(k), W(-k)
+        const R sp = (R)0.5 * (kv[m] + kw[m]), sn = (R)0.5 * (kv[m] - kw[m]);
+        oth[k] = mk<R>(w.x * sp + wm.x * sn, w.y * sp - wm.y * sn);
+      }
+    }
+    C* t = res;  // the product is in oth: transform it, res is the scratch
+    res = oth;
+    oth = t;
+  }
+  __syncthreads();
+  res = lds_fft<R>(res, oth, 1, P, tw, +1);
+  for (int i = threadIdx.x; i < NX; i += blockDim.x) {
+    int a = i - shift;
+    if (a < 0) a += Lx;
+    const C v = res[a];
+    if (!packed) {
+      U[(size_t)i * nky + ky] = v;
+    } else {
+      U[(size_t)i * nky] = mk<R>(v.x, 0);
+      U[(size_t)i * nky + kyN] = mk<R>(v.y, 0);
+    }
   }
   __syncthreads();
 }
@@ -242,47 +332,64 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
     double* xa_out = GP(A.xb[bs.xr[1]]);
     double* xb_out = GP(A.xb[bs.xr[2]]);
     {
+      // task 2 q + k: row pair q (rows 2q, 2q+1), trial k -- two workgroups
+      // per row pair, each forming the gradient and taking one trial's prox
+      // and FFT.  z of rows 2q-1 .. 2q+2 (the pair and its TSV neighbours),
+      // formed from x and xnew in the momentum round, is staged in the upper
+      // half of the row pass's LDS scratch (unused until its first FFT stage:
+      // 4 NY doubles = Ly complex doubles)
       double v[4] = {0, 0, 0, 0};
-      for (int t = lb; t < A.n_row; t += nb) {
-        auto load2 = [&](int i, int j, double& o0, double& o1) {
+      double* zl = reinterpret_cast<double*>(sm64 + g.Ly);
+      for (int t = lb; t < 2 * A.n_row; t += nb) {
+        const int k = t & 1, rt = t >> 1;
+        const int r0 = 2 * rt - 1;
+        const double cik = k ? ci1 : ci0, ek = k ? e1 : e0;
+        double* xk_out = k ? xb_out : xa_out;
+        for (int q = threadIdx.x; q < 4 * NY; q += blockDim.x) {
+          const int r = r0 + q / NY, j = q - (q / NY) * NY;
+          double zv = 0.0;
+          if (r >= 0 && r < NX) {
+            const int f = r * NY + j;
+            zv = mom ? ma * xo[f] + mb * xq[f] : zs[f];
+          }
+          zl[q] = zv;
+        }
+        __syncthreads();
+        double d2 = 0.0, l1 = 0.0;
+        auto load = [&](int i, int j) -> double {
           const int f = i * NY + j;
-          // z and T z at f; z at a neighbour (formed from x and xnew in the
-          // momentum round, read back otherwise)
-          double zf, tzf;
-          if (mom) {
-            zf = ma * xo[f] + mb * xq[f];
-            tzf = ma * tx[f] + mb * tq[f];
+          const int l = (i - r0) * NY + j;  // (i, j) in the staged rows
+          const double zf = zl[l];
+          const double tzf = mom ? ma * tx[f] + mb * tq[f] : tz[f];
+          if (mom && !k) {  // the trial-0 workgroup of the pair stores z and T z
             zs[f] = zf;
             tz[f] = tzf;
-          } else {
-            zf = zs[f];
-            tzf = tz[f];
           }
-          auto zat = [&](int q) -> double { return mom ? ma * xo[q] + mb * xq[q] : zs[q]; };
           double gf = bv[f] - tzf;
-          if (tsv) {  // - lambda_tsv grad TSV(z)
-            const int cc = f % W, r = f / W;
+          if (tsv) {  // - lambda_tsv grad TSV(z); square images: column j, row i
             double d = 0.0;
-            if (cc < W - 1) d += zf - zat(f + 1);
-            if (cc > 0) d += zf - zat(f - 1);
-            if (r < H - 1) d += zf - zat(f + W);
-            if (r > 0) d += zf - zat(f - W);
+            if (j < W - 1) d += zf - zl[l + 1];
+            if (j > 0) d += zf - zl[l - 1];
+            if (i < H - 1) d += zf - zl[l + NY];
+            if (i > 0) d += zf - zl[l - NY];
             gf -= lt * (2.0 * d);
           }
           const bool boxed = boxp && boxp[f] == 0.0f;
-          const double xa = prox_p<double>(gf * ci0 + zf, e0, A.nonneg, boxed);
-          const double xb = prox_p<double>(gf * ci1 + zf, e1, A.nonneg, boxed);
-          xa_out[f] = xa;
-          xb_out[f] = xb;
-          const double da = xa - zf, db = xb - zf;
-          v[0] += da * da;
-          v[1] += fabs(xa);
-          v[2] += db * db;
-          v[3] += fabs(xb);
-          o0 = da;
-          o1 = db;
+          const double xv = prox_p<double>(gf * cik + zf, ek, A.nonneg, boxed);
+          xk_out[f] = xv;
+          const double dv = xv - zf;
+          d2 += dv * dv;
+          l1 += fabs(xv);
+          return dv;
         };
-        st_rowfft_fwd_pair<double>(t, NX, NY, g.Ly, NY / 2, A.Py, twyd, GP(A.T64), GP(A.T64b), sm64, load2);
+        st_rowfft_fwd<double>(rt, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, k ? GP(A.T64b) : GP(A.T64), sm64, load);
+        if (k) {  // (no runtime index into v: it stays in registers)
+          v[2] += d2;
+          v[3] += l1;
+        } else {
+          v[0] += d2;
+          v[1] += l1;
+        }
       }
       block_sum<4>(v, red);
       if (threadIdx.x == 0) {
@@ -293,38 +400,43 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
       }
     }
     if (!grid_sync(B, epoch, 8)) return;
-    // T2: column FFT, product with Kh, inverse column FFT (both trials), and
-    // the regulariser partials of xnew and d; when at least half as many
-    // workgroups as pixels / 256 have no column task, those take the partials
+    // T2: column FFT, product with Kh, inverse column FFT; task 2 q + k is
+    // ky row q of trial k (row 0 carrying the Nyquist row too)
+    for (int t = lb; t < 2 * NY; t += nb) {
+      const int k = t & 1;
+      st_colconv1<double>(t >> 1, k ? GP(A.T64b) : GP(A.T64), kh, k ? GP(A.u64b) : GP(A.u64a), NX, nky, g.Lx,
+                          NX / 2, A.Px, twxd, sm64);
+    }
+    if (!grid_sync(B, epoch, 9)) return;
+    // T3: c2r rows of both trials: T d, T xnew = T z + T d, chi^2(xnew) - |W V|^2
+    // and |W A d|^2 = <d, T d>; task 2 q + k is row task q of trial k.  The
+    // regulariser partials of xnew and d: on the workgroups without a row
+    // task when at least half as many of them as pixels / 256 exist, else on
+    // every workgroup after its row tasks
     {
-      const int nreg = (int)nb - nky;
+      const int ntask = 2 * A.n_row;
+      const int nreg = (int)nb - ntask;
       const bool split = nreg > 0 && 2 * (long)nreg * blockDim.x >= n;
-      const bool do_reg = !split || (int)lb >= nky;
-      const int rg0 = split ? (int)lb - nky : (int)lb;
+      const bool do_reg = !split || (int)lb >= ntask;
+      const int rg0 = split ? (int)lb - ntask : (int)lb;
       const int rgn = split ? nreg : (int)nb;
-      for (int t = lb; t < nky; t += nb)
-        st_colconv<double>(t, 2, GP(A.T64), GP(A.T64b), kh, GP(A.u64a), GP(A.u64b), NX, nky, g.Lx, NX / 2, A.Px,
-                           twxd, sm64);
       double r[4] = {0, 0, 0, 0};
-      for (int f = rg0 * blockDim.x + threadIdx.x; do_reg && f < n; f += rgn * blockDim.x) {
-        reg_terms(xa_out, zs, f, W, H, false, r);
-        reg_terms(xb_out, zs, f, W, H, false, r + 2);
+      if (do_reg) {  // (uniform per workgroup: the reduction's barriers are safe)
+        for (int f = rg0 * blockDim.x + threadIdx.x; f < n; f += rgn * blockDim.x) {
+          reg_terms(xa_out, zs, f, W, H, false, r);
+          reg_terms(xb_out, zs, f, W, H, false, r + 2);
+        }
+        block_sum<4>(r, red);
       }
-      block_sum<4>(r, red);
       if (threadIdx.x == 0) {
         prow[(kPReg) * pstr] = r[0];
         prow[(kPTsvd) * pstr] = r[1];
         prow[(kPTrial + kPReg) * pstr] = r[2];
         prow[(kPTrial + kPTsvd) * pstr] = r[3];
       }
-    }
-    if (!grid_sync(B, epoch, 9)) return;
-    // T3: c2r rows of both trials: T d, T xnew = T z + T d, chi^2(xnew) - |W V|^2
-    // and |W A d|^2 = <d, T d>; task 2 q + k is row task q of trial k
-    {
       double v[4] = {0, 0, 0, 0};
       for (int t = lb; t < 2 * A.n_row; t += nb) {
-        const int k = t & 1;
+        const int k = t & 1, rt = t >> 1;
         const double* xk = k ? xb_out : xa_out;
         double* tk = GP(A.tv[k ? bs.in1 : bs.in]);
         double chi = 0.0, lip = 0.0;
@@ -336,7 +448,7 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
           lip += (xv - zs[f]) * td;
           chi += xv * (tn - 2.0 * bv[f]);
         };
-        st_rowfft_c2r<double>(t >> 1, 1, k ? GP(A.u64b) : GP(A.u64a), NX, NY, g.Ly, NY / 2, A.Py, twyd, sm64, store);
+        st_rowfft_c2r<double>(rt, 1, k ? GP(A.u64b) : GP(A.u64a), NX, NY, g.Ly, NY / 2, A.Py, twyd, sm64, store);
         if (k) {  // (no runtime index into v: it stays in registers)
           v[2] += chi;
           v[3] += lip;

@@ -90,11 +90,14 @@ def test_toeplitz_signed_with_box(preset):
 
 def test_toeplitz_l1_only():
     """lambda_tsv = 0: the L1 solve (the image is the unique minimiser only up
-    to the data term's null space, so the objective is the bar)."""
+    to the data term's null space, so the objective is the bar).  It converges
+    slowly: at 3000 iterations the objective is still ~1e-6 from the optimum
+    and two trajectories that differ in the last bits (the set-up adjoint's
+    FP64 atomics) end up to 1.1e-6 apart, so both run 12000 iterations."""
     P = synth.make_nufft_problem(N=64, M=2000, seed=99, amax=0.35)
-    print("case toeplitz L1 only N=64 M=2000 3000 iterations", flush=True)
-    o = _oracle(P, 64, 64, sa.PRESET_CXX12, 3000, 0.0, 5.0, 0.0)
-    xg, rg, info = _gpu(P, 64, 64, sa.PRESET_CXX12, 3000, 0.0, 5.0, 0.0)
+    print("case toeplitz L1 only N=64 M=2000 12000 iterations", flush=True)
+    o = _oracle(P, 64, 64, sa.PRESET_CXX12, 12000, 0.0, 5.0, 0.0)
+    xg, rg, info = _gpu(P, 64, 64, sa.PRESET_CXX12, 12000, 0.0, 5.0, 0.0)
     l2, dc = _compare(xg, rg, o)
     print(f"  loop {info['loop_used']} relL2={l2:.2e} cost rel={dc:.2e}")
     assert info["loop_used"] == sa.LOOP_TOEPLITZ
