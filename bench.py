@@ -40,8 +40,12 @@ HBM_PEAK_GBS = 8000.0
 # size).  The counters cannot be read from inside the process, so the line
 # reports the figures of that named run, per iteration and per trial round,
 # scaled to this run's counts.
-TRAFFIC_PROFILE = "profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_nufft256_m20000.json")
+TRAFFIC = {  # solve loop -> (kernel, profile summary, figures per iteration and per trial round)
+    "gridding": ("k_mfista_persistent", "profiles/k_mfista_persistent_nufft256_m20000_hbm_traffic.md",
+                 os.path.join(ROOT, "profiles", "hbm_traffic_nufft256_m20000.json")),
+    "toeplitz": ("k_mfista_toeplitz", "profiles/k_mfista_toeplitz_nufft256_m20000_hbm_traffic.md",
+                 os.path.join(ROOT, "profiles", "hbm_traffic_toeplitz256_m20000.json")),
+}
 
 
 def log(*a):
@@ -91,6 +95,28 @@ def algorithmic_bytes(N, M, iters, trials):
     per_iter = 28 * P + 12 * G + 32 * H + 44 * M
     per_trial = 20 * P + 8 * G + 16 * H + 28 * M
     return per_iter * iters + per_trial * trials, per_iter, per_trial
+
+
+def toeplitz_bytes(N, iters, rounds):
+    """Algorithmic bytes of one Toeplitz-loop solve (one launch), FP64 data.
+
+    P = N^2 pixels, T = N(N+1) cells of a step's half spectrum (the row pass
+    output, the column pass output, and the kernel Kh of 2 T doubles).
+    Per paired trial round:
+      T1 gradient + prox + rows  read z, T z, b (24 B/px), write two xnew (16 B/px)
+                                 and two row transforms (32 B/cell)
+      T2 columns + kernel        read two row transforms (32 B/cell) and Kh
+                                 (16 B/cell), write two column outputs (32 B/cell)
+      T3 c2r + partials          read two column outputs (32 B/cell), two xnew,
+                                 z, T z, b (40 B/px), write two T xnew (16 B/px)
+    Per iteration (the momentum, folded into the first round): read x, xnew,
+    T x, T xnew instead of z and T z (+16 B/px), write z and T z (16 B/px).
+    Returns (total, per iteration, per round).
+    """
+    P, T = N * N, N * (N + 1)
+    per_round = (40 * P + 32 * T) + (80 * T) + (56 * P + 32 * T)
+    per_iter = 32 * P
+    return per_iter * iters + per_round * rounds, per_iter, per_round
 
 
 def persistent_bytes(N, M, MSP, n_entries, iters, trials, rounds=None):
@@ -488,6 +514,9 @@ def main():
                          "single-threaded variant runs one (0 = skip both)")
     ap.add_argument("--profile-reps", type=int, default=50)
     ap.add_argument("--engine", choices=["persistent", "stream"], default="persistent")
+    ap.add_argument("--loop", choices=["auto", "gridding"], default="auto",
+                    help="single solves on the persistent engine: the Toeplitz loop where it applies (auto) or "
+                         "the gridding loop")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--warm", type=int, default=32,
@@ -523,7 +552,8 @@ def main():
     N, M, seed = 256, 20000, 1002
     cfg = dict(maxiter=50000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=100.0, cinit=1e4,
                nonneg_flag=1, preset=sa.PRESET_CXX12, rel_tol=1e-6,
-               engine=sa.ENGINE_PERSISTENT if args.engine == "persistent" else sa.ENGINE_STREAM)
+               engine=sa.ENGINE_PERSISTENT if args.engine == "persistent" else sa.ENGINE_STREAM,
+               loop=sa.LOOP_GRIDDING if args.loop == "gridding" else sa.LOOP_AUTO)
     P = synth.make_nufft_problem(N=N, M=M, seed=seed)
     dev = torch.device("cuda:0")
     t = {k: torch.tensor(P[k], dtype=torch.float64, device=dev) for k in ("u", "v", "vis_r", "vis_i", "std")}
@@ -573,24 +603,30 @@ def main():
         kms = info["kernel_ms"]
         tot, per_it, per_tr = algorithmic_bytes(N, M, ITER, trials)
         ach = tot / (kms * 1e-3) / 1e9
-        # the byte model of what this kernel actually moves (FP64 iterates,
-        # models and step transforms; paired trial rounds), for comparison
-        tot64, per_it64, per_pr64 = persistent_bytes(N, M, MSP, n_entries, ITER, trials, rounds)
-        roofline = {"bound": "hbm", "kernel": "k_mfista_persistent", "achieved": round(ach, 1),
+        # the byte model of what this kernel actually moves (FP64 data, paired
+        # trial rounds), for comparison: the Toeplitz loop (single solves'
+        # default) or the gridding loop
+        loop = "toeplitz" if info.get("loop_used") == sa.LOOP_TOEPLITZ else "gridding"
+        kname, traffic_profile, traffic_file = TRAFFIC[loop]
+        if loop == "toeplitz":
+            tot64, per_it64, per_pr64 = toeplitz_bytes(N, ITER, rounds)
+        else:
+            tot64, per_it64, per_pr64 = persistent_bytes(N, M, MSP, n_entries, ITER, trials, rounds)
+        roofline = {"bound": "hbm", "kernel": kname, "loop": loop, "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                     "traffic": None, "bytes_per_launch": tot, "bytes_per_iteration": per_it,
                     "bytes_per_trial": per_tr, "iterations": ITER, "trials": trials, "trial_rounds": rounds,
                     "launch_ms": round(kms, 3), "us_per_iteration": round(kms * 1e3 / ITER, 2),
                     "kernel_byte_model": {"bytes_per_launch": tot64, "achieved": round(tot64 / (kms * 1e-3) / 1e9, 1),
                                           "bytes_per_iteration": per_it64, "bytes_per_trial_round": per_pr64}}
-        if os.path.exists(TRAFFIC_FILE):
+        if os.path.exists(traffic_file):
             # measured HBM bytes of the named profiling run, scaled to this
             # run's iteration and trial-round counts
-            with open(TRAFFIC_FILE) as f:
+            with open(traffic_file) as f:
                 tr = json.load(f)
             scale = (tr["bytes_per_iteration"] * ITER + tr["bytes_per_round"] * rounds)
             roofline["traffic"] = int(scale)
-            roofline["traffic_source"] = {"profile": TRAFFIC_PROFILE, **tr}
+            roofline["traffic_source"] = {"profile": traffic_profile, **tr}
     else:
         share = {k: prof[k] * launches[k] for k in prof}
         dom = max(share, key=share.get)
@@ -653,7 +689,9 @@ def main():
         "data": "synthetic (seeded ring + 3 Gaussians, 12 elliptical uv arcs)",
         "config": {"workload": "nufft_l1tsv_256_m20000_12tap", "N": N, "M": M, "taps": 12,
                    "lambda_l1": 1.0, "lambda_tsv": 100.0, "rel_tol": 1e-6, "start": "cold",
-                   "iterations": ITER, "trials": trials, "engine": args.engine},
+                   "iterations": ITER, "trials": trials, "engine": args.engine,
+                   "loop": {0: "stream", sa.LOOP_GRIDDING: "gridding", sa.LOOP_TOEPLITZ: "toeplitz"}.get(
+                       info.get("loop_used"), "?")},
         "iterations_per_s": round(ITER / (ms_per_step / 1e3), 2),
         "roofline": roofline,
         "cpu_baseline": cpu,
