@@ -387,6 +387,8 @@ typedef struct mfista_tune {
   int fgp_tw;         /* L1+TV: FGP block shape, 0 = pixels one per thread along the flat
                          image, 64 / 32 / 16 = tiles of fgp_tw x (256 / fgp_tw) pixels */
   double us_per_iter; /* measured microseconds per iteration */
+  int loop;           /* single solves: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ (the loop
+                         the entry was timed on and that tuned solves run), 0 = by rule */
 } mfista_tune;
 
 /* Tunes the shape of this problem (arguments as mfista_imaging_core_nufft_ex,

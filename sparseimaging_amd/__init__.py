@@ -95,6 +95,7 @@ class Tune(C.Structure):
         ("NX", C.c_int), ("NY", C.c_int), ("preset", C.c_int), ("reg", C.c_int), ("m_log2", C.c_int),
         ("nprob_log2", C.c_int), ("tile_x", C.c_int), ("tile_y", C.c_int), ("blocks_per_cu", C.c_int),
         ("paired", C.c_int), ("engine", C.c_int), ("fgp_tw", C.c_int), ("us_per_iter", C.c_double),
+        ("loop", C.c_int),
     ]
 
     def as_dict(self):
@@ -337,7 +338,7 @@ def _pack_problems(problems):
 
 def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITER, rel_tol=0.0,
                                     single_trials=False, deterministic=False, autotune=False, tune_iters=30,
-                                    engine=ENGINE_PERSISTENT):
+                                    engine=ENGINE_PERSISTENT, loop=LOOP_AUTO):
     """Independent NUFFT solves in one persistent launch (no reference counterpart:
     a loop of c/mfista_nufft_lib.c:909 calls).  `problems` is a list of dicts with
     the keyword arguments of mfista_imaging_core_nufft (u_dx, v_dy, vis_r, vis_i,
@@ -346,13 +347,15 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
     u_dx, v_dy, vis_r, vis_i, vis_std share one NUFFT set-up.  autotune=True:
     the autotuner's launch configuration for the batch's shape.
     engine=ENGINE_STREAM: the wide engine (every stage one launch for all
-    problems; L1 and L1+TSV of one image size).  Returns
+    problems; L1 and L1+TSV of one image size).  The persistent launch runs
+    the Toeplitz loop when every problem is L1 / L1+TSV on a square image up
+    to 512^2 (loop=LOOP_GRIDDING: the gridding loop).  Returns
     (list of x[NX,NY], list of RESULT dicts, opts dict for the whole batch)."""
     n = len(problems)
     arr, keep, outs = _pack_problems(problems)
     res = (RESULT * n)()
     o = make_opts(preset, fgp_iter, rel_tol, single_trials=single_trials, deterministic=deterministic,
-                  autotune=autotune, tune_iters=tune_iters, engine=engine)
+                  autotune=autotune, tune_iters=tune_iters, engine=engine, loop=loop)
     rc = lib().mfista_imaging_core_nufft_batch(n, arr, res, C.byref(o))
     if rc != 0:
         raise MfistaError(rc)

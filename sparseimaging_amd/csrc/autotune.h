@@ -14,7 +14,10 @@
 //   * for batches, the engine: one persistent launch for the whole batch, or
 //     the wide engine (every stage one launch for all problems, wide.hip);
 //   * for L1+TV, the FGP block shape: pixels dealt one per thread along the
-//     flat image, or in tiles of 64x4, 32x8 or 16x16 pixels per workgroup.
+//     flat image, or in tiles of 64x4, 32x8 or 16x16 pixels per workgroup;
+//   * for single solves where the Toeplitz loop applies, the loop: the
+//     gridding loop's candidates above, or the Toeplitz loop at 1 or 2
+//     workgroups per CU (its only knob).
 // Every candidate solves the caller's own problem for a fixed number of
 // iterations from its own start (no stopping rule); the fastest per
 // iteration is kept in a process-wide table under the problem's shape key,
@@ -39,6 +42,7 @@ struct TuneChoice {
   int tile_x = 16, tile_y = 16, blocks_per_cu = 2, paired = 1;
   int engine = MFISTA_ENGINE_PERSISTENT;  // batches: MFISTA_ENGINE_STREAM = the wide engine
   int fgp_tw = 0;                         // L1+TV: FGP block shape (0 = along the flat image)
+  int loop = 0;  // single solves: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ (0: the loop by rule)
   double us_per_iter = 0;
 };
 
@@ -52,14 +56,16 @@ void tune_store(const TuneKey& k, const TuneChoice& c);
 // back the defaults (environment overrides included)
 void tune_apply(Context& C, const TuneChoice& c);
 void tune_restore(Context& C);
-// the candidates for a solve (batch = false) or a batch of this regulariser class
-std::vector<TuneChoice> tune_candidates(int reg, bool batch);
+// the candidates for a solve (batch = false) or a batch of this regulariser
+// class; toep: a single solve the Toeplitz loop applies to (its candidates
+// are added and the gridding loop's are marked as such)
+std::vector<TuneChoice> tune_candidates(int reg, bool batch, bool toep = false);
 // Times `run` (which solves with the context's current knobs and returns 0
 // and the time per iteration of the launch, in microseconds) for every
 // candidate, twice each; the fastest in *best, every candidate's time in
 // *all (optional).  The context's knobs are restored afterwards.
 int tune_search(Context& C, int reg, bool batch, const std::function<int(double*)>& run, TuneChoice* best,
-                std::vector<TuneChoice>* all);
+                std::vector<TuneChoice>* all, bool toep = false);
 long tune_runs();  // candidate solves timed so far in this process
 void tune_load_env();
 

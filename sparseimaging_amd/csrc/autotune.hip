@@ -69,6 +69,7 @@ void tune_apply(Context& C, const TuneChoice& c) {
   C.persist_pair = c.paired;
   C.tune_engine = c.engine;
   C.fgp_tw = c.fgp_tw;
+  C.tune_loop = c.loop;
 }
 
 void tune_restore(Context& C) {
@@ -78,9 +79,10 @@ void tune_restore(Context& C) {
   C.persist_pair = C.def_pair;
   C.tune_engine = -1;
   C.fgp_tw = 0;
+  C.tune_loop = 0;
 }
 
-std::vector<TuneChoice> tune_candidates(int reg, bool batch) {
+std::vector<TuneChoice> tune_candidates(int reg, bool batch, bool toep) {
   static const int tiles[3][2] = {{16, 16}, {32, 16}, {32, 32}};
   std::vector<TuneChoice> v;
   static const int fgp_shapes[4] = {0, 64, 32, 16};
@@ -94,8 +96,16 @@ std::vector<TuneChoice> tune_candidates(int reg, bool batch) {
           c.blocks_per_cu = bpc;
           c.paired = (reg == 2) ? 0 : pair;
           c.fgp_tw = fgp_shapes[f];
+          if (toep) c.loop = MFISTA_LOOP_GRIDDING;
           v.push_back(c);
         }
+  if (toep)  // the Toeplitz loop: workgroups per CU are its only knob (persistent launches)
+    for (int bpc = 1; bpc <= 2; bpc++) {
+      TuneChoice c;
+      c.blocks_per_cu = bpc;
+      c.loop = MFISTA_LOOP_TOEPLITZ;
+      v.push_back(c);
+    }
   if (batch && reg != 2)  // the wide engine: the gridding tile is its only knob
     for (const auto& t : tiles) {
       TuneChoice c;
@@ -108,8 +118,8 @@ std::vector<TuneChoice> tune_candidates(int reg, bool batch) {
 }
 
 int tune_search(Context& C, int reg, bool batch, const std::function<int(double*)>& run, TuneChoice* best,
-                std::vector<TuneChoice>* all) {
-  std::vector<TuneChoice> cand = tune_candidates(reg, batch);
+                std::vector<TuneChoice>* all, bool toep) {
+  std::vector<TuneChoice> cand = tune_candidates(reg, batch, toep);
   int ibest = -1;
   for (size_t i = 0; i < cand.size(); i++) {
     TuneChoice& c = cand[i];
@@ -155,20 +165,23 @@ void to_public(const TuneKey& k, const TuneChoice& c, mfista_tune* t) {
   t->engine = c.engine;
   t->fgp_tw = c.fgp_tw;
   t->us_per_iter = c.us_per_iter;
+  t->loop = c.loop;
 }
 
 // text file: a header line, then one entry per line,
-// "NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired engine fgp_tw us_per_iter"
+// "NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired engine fgp_tw loop us_per_iter"
+// (lines without the loop field, 13 fields, load with loop 0)
 static int save_table(const char* path) {
   FILE* f = fopen(path, "w");
   if (!f) return MFISTA_E_ARG;
   fprintf(f, "# mfista_amd tune table: NX NY preset reg m_log2 nprob_log2 tile_x tile_y blocks_per_cu paired "
-             "engine fgp_tw us_per_iter\n");
+             "engine fgp_tw loop us_per_iter\n");
   for (const auto& e : g_table) {
     const TuneKey& k = e.first;
     const TuneChoice& c = e.second;
-    fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %.3f\n", k.NX, k.NY, k.preset, k.reg, k.m_log2,
-            k.nprob_log2, c.tile_x, c.tile_y, c.blocks_per_cu, c.paired, c.engine, c.fgp_tw, c.us_per_iter);
+    fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %d %.3f\n", k.NX, k.NY, k.preset, k.reg, k.m_log2,
+            k.nprob_log2, c.tile_x, c.tile_y, c.blocks_per_cu, c.paired, c.engine, c.fgp_tw, c.loop,
+            c.us_per_iter);
   }
   return fclose(f) == 0 ? 0 : MFISTA_E_ARG;
 }
@@ -182,13 +195,24 @@ static int load_table(const char* path) {
     if (line[0] == '#') continue;
     TuneKey k;
     TuneChoice c;
-    if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf", &k.NX, &k.NY, &k.preset, &k.reg, &k.m_log2,
-               &k.nprob_log2, &c.tile_x, &c.tile_y, &c.blocks_per_cu, &c.paired, &c.engine, &c.fgp_tw,
-               &c.us_per_iter) != 13)
+    double f13 = 0, f14 = 0;
+    const int nf = sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &k.NX, &k.NY, &k.preset, &k.reg,
+                          &k.m_log2, &k.nprob_log2, &c.tile_x, &c.tile_y, &c.blocks_per_cu, &c.paired, &c.engine,
+                          &c.fgp_tw, &f13, &f14);
+    if (nf == 14) {  // ... loop us_per_iter
+      c.loop = (int)f13;
+      if ((double)c.loop != f13) continue;
+      c.us_per_iter = f14;
+    } else if (nf == 13) {  // ... us_per_iter
+      c.loop = 0;
+      c.us_per_iter = f13;
+    } else {
       continue;
+    }
     if (!tile_supported(c.tile_x, c.tile_y) || c.blocks_per_cu < 1 || c.blocks_per_cu > 4 ||
         (c.engine != MFISTA_ENGINE_PERSISTENT && c.engine != MFISTA_ENGINE_STREAM) ||
-        (c.fgp_tw != 0 && c.fgp_tw != 16 && c.fgp_tw != 32 && c.fgp_tw != 64))
+        (c.fgp_tw != 0 && c.fgp_tw != 16 && c.fgp_tw != 32 && c.fgp_tw != 64) ||
+        (c.loop != 0 && c.loop != MFISTA_LOOP_GRIDDING && c.loop != MFISTA_LOOP_TOEPLITZ))
       continue;
     g_table[k] = c;
     n++;

@@ -54,8 +54,14 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out);
 // (toeplitz.hip; MFISTA_E_SIZE or MFISTA_E_FOV when it does not apply) and
 // the solve in one cooperative launch (persistent.hip)
 int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const double* d_vr, const double* d_vi,
-                   const double* d_sd, int M, int NX, int NY, int preset, double wv2);
+                   const double* d_sd, int M, int NX, int NY, int preset, double wv2, ToepOut& out);
 int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out);
+// independent Toeplitz-loop solves in one cooperative launch: problem i uses
+// sets[i] (its NUFFT set-up, for the geometry) and ops[i] (its set's operator
+// data from toeplitz_setup), C.bwork[i] its workspace (persistent.hip)
+int solve_nufft_toeplitz_batch(Context& C, const std::vector<NufftState*>& sets,
+                               const std::vector<const ToepOut*>& ops, const std::vector<RunParams>& P,
+                               std::vector<RunStats>& out, float* kernel_ms);
 int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       std::vector<RunStats>& out, float* kernel_ms);
 // independent solves of one image size and preset with every kernel launched

@@ -389,47 +389,57 @@ int solve_nufft_persistent(Context& C, const RunParams& P, RunStats* out) {
   return 0;
 }
 
-// Toeplitz loop of a single solve (toeplitz_body.h); toeplitz_setup has run
-// for the problem of C.nu.  Paired trial rounds always.
-int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
-  NufftState& N = C.nu;
-  ToepWork& T = C.tw;
-  PersistWork& W = C.pw;
-  hipStream_t st = C.stream;
-  const GridGeom& g = N.geom;
-  const int NX = P.NX, NY = P.NY, n = NX * NY, nky = NY + 1;
-  if (P.tv_mode || NX != g.NX || NY != g.NY) return MFISTA_E_ARG;
-  // launch configuration: [FP64 twiddle tables][stage scratch: two FP64
-  // sequences of the longer axis, double-buffered]
+// Launch configuration of the Toeplitz loop (toeplitz_body.h) for problems of
+// the given geometries: [FP64 twiddle tables][stage scratch: two FP64
+// sequences of the longer axis, double-buffered]
+static int toeplitz_config(Context& C, const std::vector<const NufftState*>& sets, bool batch, PLaunch* L) {
   int dev = 0, ncu = 0, lds_max = 0;
   MF_CHECK(hipGetDevice(&dev));
   MF_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   MF_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   const size_t static_lds = 16 * kPW * 8 + sizeof(BlockState) + (kTD + 1) * 8 + sizeof(BarCtx) + 64;
-  const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
-  PLaunch L;
-  L.sh = (size_t)4 * Lm * sizeof(double2);
-  const size_t tables = (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255;
-  L.lds_tables = (tables + L.sh + static_lds <= (size_t)lds_max) ? 1 : 0;
-  if (L.lds_tables) L.sh += tables;
-  if (L.sh + static_lds > (size_t)lds_max) return MFISTA_E_SIZE;
-  L.fn = L.lds_tables ? (const void*)k_mfista_toeplitz<true> : (const void*)k_mfista_toeplitz<false>;
+  size_t sh = 0, tables = 0;
+  for (const NufftState* N : sets) {
+    const GridGeom& g = N->geom;
+    const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
+    sh = std::max(sh, (size_t)4 * Lm * sizeof(double2));
+    tables = std::max(tables, (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255);
+  }
+  L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;
+  if (L->lds_tables) sh += tables;
+  if (sh + static_lds > (size_t)lds_max) return MFISTA_E_SIZE;
+  L->sh = sh;
+  if (batch)
+    L->fn = L->lds_tables ? (const void*)k_mfista_toeplitz_batch<true> : (const void*)k_mfista_toeplitz_batch<false>;
+  else
+    L->fn = L->lds_tables ? (const void*)k_mfista_toeplitz<true> : (const void*)k_mfista_toeplitz<false>;
   int per_cu = 0;
-  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L.fn, MF_BLOCK, L.sh));
+  MF_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, L->fn, MF_BLOCK, sh));
   if (per_cu < 1) return MFISTA_E_HIP;
   const int want = C.persist_blocks_per_cu > 0 ? C.persist_blocks_per_cu : 2;
-  L.nblk = ncu * (per_cu < want ? per_cu : want);
-  L.pair = 1;
-  // workspace
+  L->nblk = ncu * (per_cu < want ? per_cu : want);
+  L->pair = 1;
+  return 0;
+}
+
+// Allocates one Toeplitz-loop problem's workspace and fills its PArgs for nblk
+// workgroups; op: its visibility set's operator data.  P.xinit and P.xout are
+// device pointers.
+static int toeplitz_prepare(const NufftState& N, const ToepOut& op, PersistWork& W, const RunParams& P, int nblk,
+                            int lds_tables, int hier, hipStream_t st, PArgs* out) {
+  const GridGeom& g = N.geom;
+  const int NX = P.NX, NY = P.NY, n = NX * NY, nky = NY + 1;
+  if (P.tv_mode || NX != g.NX || NY != g.NY) return MFISTA_E_ARG;
   const size_t t64 = (size_t)nky * NX * sizeof(double2);
-  if (W.px.ensure((size_t)n * 8) || W.pz.ensure((size_t)n * 8) || W.T64.ensure(2 * t64) || W.part.ensure((size_t)L.nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
+  if (W.px.ensure((size_t)n * 8) || W.pz.ensure((size_t)n * 8) || W.T64.ensure(2 * t64) ||
+      W.part.ensure((size_t)nblk * kPW * 8) || W.state.ensure(sizeof(DevState)) ||
       W.cost.ensure((size_t)(P.maxiter > 0 ? P.maxiter : 1) * 8) || W.bar.ensure(kBarBytes) ||
-      T.tvec.ensure((size_t)4 * n * 8) || T.u64.ensure(2 * t64) || T.xrot.ensure((size_t)4 * n * 8))
+      W.tvec.ensure((size_t)4 * n * 8) || W.u64.ensure(2 * t64) || W.xrot.ensure((size_t)4 * n * 8))
     return MFISTA_E_HIP;
   if (P.xinit != W.px.p)
     MF_CHECK(hipMemcpyAsync(W.px.p, P.xinit, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
   MF_CHECK(hipMemsetAsync(W.bar.p, 0, kBarBytes, st));
-  PArgs A;
+  PArgs& A = *out;
   memset(&A, 0, sizeof(A));
   A.g = g;
   A.Px = N.Px;
@@ -453,19 +463,33 @@ int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
   A.part = (double*)W.part.p;
   A.n_row = (NX + 1) / 2;
   A.n_ky = nky;
-  A.lds_tables = L.lds_tables;
-  A.nblk = L.nblk;
+  A.lds_tables = lds_tables;
+  A.nblk = nblk;
   A.cost = (double*)W.cost.p;
   A.S = (DevState*)W.state.p;
   A.bar = (unsigned long long*)W.bar.p;
-  A.hier_barrier = C.persist_hier_barrier;
-  A.kh = (const double*)T.kh.p;
-  A.bimg = (const double*)T.bimg.p;
-  for (int q = 0; q < 4; q++) A.tv[q] = (double*)T.tvec.p + (size_t)q * n;
-  A.u64a = (double2*)T.u64.p;
-  A.u64b = (double2*)((char*)T.u64.p + t64);
-  A.wv2 = T.wv2;
-  for (int q = 0; q < 4; q++) A.xb[q] = (double*)T.xrot.p + (size_t)q * n;
+  A.hier_barrier = hier;
+  A.kh = (const double*)op.kh.p;
+  A.bimg = (const double*)op.bimg.p;
+  for (int q = 0; q < 4; q++) A.tv[q] = (double*)W.tvec.p + (size_t)q * n;
+  A.u64a = (double2*)W.u64.p;
+  A.u64b = (double2*)((char*)W.u64.p + t64);
+  A.wv2 = op.wv2;
+  for (int q = 0; q < 4; q++) A.xb[q] = (double*)W.xrot.p + (size_t)q * n;
+  return 0;
+}
+
+// Toeplitz loop of a single solve (toeplitz_body.h); toeplitz_setup has run
+// for the problem of C.nu into C.tw.out.  Paired trial rounds always.
+int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
+  NufftState& N = C.nu;
+  PersistWork& W = C.pw;
+  hipStream_t st = C.stream;
+  PLaunch L;
+  int rc;
+  if ((rc = toeplitz_config(C, {&N}, false, &L))) return rc;
+  PArgs A;
+  if ((rc = toeplitz_prepare(N, C.tw.out, W, P, L.nblk, L.lds_tables, C.persist_hier_barrier, st, &A))) return rc;
   const char* trace_path = getenv("MFISTA_PERSISTENT_TRACE");
   const int tcap = 16384;
   if (trace_path && *trace_path) {
@@ -480,7 +504,6 @@ int solve_nufft_toeplitz(Context& C, const RunParams& P, RunStats* out) {
   const PArgs* dA = (const PArgs*)W.pargs.p;
   void* args[] = {&dA};
   float kms = 0;
-  int rc;
   if ((rc = timed_launch(L, args, L.nblk, st, C.no_alloc ? nullptr : &kms))) return rc;
   out->kernel_ms = kms;
   C.persist_blocks = L.nblk;
@@ -562,6 +585,39 @@ int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const st
     for (int i = 0; i < pn; i++)
       if ((rc = persistent_prepare(*sets[p0 + i], C.bwork[p0 + i], P[p0 + i], nb[i], L.lds_tables,
                                    C.persist_hier_barrier, L.pair, st, &A[i])))
+        return rc;
+    if ((rc = launch_dealt(C, L, A, blk, P, p0, out, kernel_ms))) return rc;
+  }
+  return 0;
+}
+
+// Independent Toeplitz-loop solves (L1 / L1+TSV, square images) in cooperative
+// launches: problem i uses sets[i], ops[i] and C.bwork[i]; problems beyond one
+// workgroup each go to further launches.
+int solve_nufft_toeplitz_batch(Context& C, const std::vector<NufftState*>& sets,
+                               const std::vector<const ToepOut*>& ops, const std::vector<RunParams>& P,
+                               std::vector<RunStats>& out, float* kernel_ms) {
+  const int np = (int)P.size();
+  hipStream_t st = C.stream;
+  if (np == 0) return 0;
+  if ((int)ops.size() != np || (int)sets.size() != np) return MFISTA_E_ARG;
+  std::vector<const NufftState*> cs(sets.begin(), sets.end());
+  PLaunch L;
+  int rc;
+  if ((rc = toeplitz_config(C, cs, true, &L))) return rc;
+  C.persist_blocks = L.nblk;
+  if ((int)C.bwork.size() < np) C.bwork.resize(np);
+  out.assign(np, RunStats());
+  *kernel_ms = 0;
+  for (int p0 = 0; p0 < np; p0 += L.nblk) {
+    const int pn = std::min(np - p0, L.nblk);
+    std::vector<int2> blk;
+    std::vector<int> nb;
+    if ((rc = deal_workgroups(sets, P, p0, pn, L.nblk, blk, nb))) return rc;
+    std::vector<PArgs> A(pn);
+    for (int i = 0; i < pn; i++)
+      if ((rc = toeplitz_prepare(*sets[p0 + i], *ops[p0 + i], C.bwork[p0 + i], P[p0 + i], nb[i], L.lds_tables,
+                                 C.persist_hier_barrier, st, &A[i])))
         return rc;
     if ((rc = launch_dealt(C, L, A, blk, P, p0, out, kernel_ms))) return rc;
   }

@@ -68,15 +68,19 @@ struct PersistWork {
   DevMem pchunks;          // the gridding chunk list re-cut at the workgroups' range boundaries
   DevMem x0, xo, box;      // batched host-pointer solves: staged xinit, xout, cl_box
   DevMem pargs;            // the solver kernel's arguments (device copy)
+  DevMem tvec, u64, xrot;  // Toeplitz loop: T applied to x / xnew / z, column-pass outputs, image buffers
 };
 
-// The Toeplitz loop of a single solve (toeplitz.hip, toeplitz_body.h).
+// The Toeplitz loop's operator data of one visibility set (toeplitz.hip).
+struct ToepOut {
+  DevMem kh, bimg;  // the kernel's spectrum Kh, b = Re A^H W^2 V
+  double wv2 = 0;   // |W V|^2
+};
+// The Toeplitz loop's set-up scratch and the single solve's operator data.
 struct ToepWork {
   NufftState nu;                                // the set-up's 2L x 2L, 24-tap FP64 transform
   DevMem rb, rp, imgp, imgb, circ, tbuf, fbuf;  // set-up scratch
-  DevMem kh, bimg, tvec, u64;                   // Kh, b, T applied to x / xnew / z, column-pass outputs
-  DevMem xrot;                                  // the four image buffers of x and the trials' xnew
-  double wv2 = 0;                               // |W V|^2
+  ToepOut out;                                  // a single solve's operator data
 };
 
 // One visibility set of a batch, with its NUFFT set-up.
@@ -84,6 +88,7 @@ struct BatchSet {
   DevMem u, v, vr, vi, sd;
   NufftState nu;
   double vis_power = 0;  // sum |V|^2 (eps scaling)
+  ToepOut toep;          // the Toeplitz loop's operator data for this set
 };
 
 // Device workspace of one problem of the wide engine (wide.hip).
@@ -134,6 +139,7 @@ struct Context {
   int def_blocks_per_cu = 2, def_pair = 1;  // the knobs' defaults (environment overrides included)
   int tune_engine = -1;  // batches: the autotuner's engine (-1: the caller's opts->engine)
   int fgp_tw = 0;        // FGP block shape of the persistent engine (autotuner; 0 = along the flat image)
+  int tune_loop = 0;     // single solves: the autotuner's loop (MFISTA_LOOP_*; 0 = by rule)
   // uploads of the host-pointer entry points
   DevMem h_u, h_v, h_vr, h_vi, h_sd, h_x0, h_x1, h_box, h_model, h_A, h_y, h_idx;
 };

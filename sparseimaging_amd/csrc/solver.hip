@@ -463,6 +463,17 @@ struct NufftOp : Operator<double> {
   }
 };
 
+// Whether a single persistent solve may run the Toeplitz loop (opts->loop
+// permitting): L1 and L1+TSV, paired rounds, floating-point atomics allowed in
+// its set-up, square images up to 512^2 (the presets size both axes' kernels
+// from NX, so for NX != NY the reference's transform is not the exact one the
+// loop applies), and not the allocation-free entry point.
+static bool toeplitz_applies(const Context& C, const SolveArgs& A, const mfista_opts* opts) {
+  const bool tv = A.lambda_tv != 0;
+  return !tv && !C.no_alloc && !(opts && (opts->single_trials || opts->deterministic)) &&
+         (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512;
+}
+
 // Times the autotuner's candidates on this problem: `iters` iterations from
 // its xinit, no stopping rule, xout overwritten.
 static int tune_single(Context& C, const SolveArgs& A, const mfista_opts* opts, int iters, TuneChoice* best,
@@ -488,7 +499,7 @@ static int tune_single(Context& C, const SolveArgs& A, const mfista_opts* opts, 
     *us = o.kernel_ms * 1e3 / (r.ITER > 0 ? r.ITER : 1);
     return 0;
   };
-  return tune_search(C, tune_reg(A.lambda_tv, A.lambda_tsv), false, run, best, all);
+  return tune_search(C, tune_reg(A.lambda_tv, A.lambda_tsv), false, run, best, all, toeplitz_applies(C, A, opts));
 }
 
 int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) {
@@ -594,14 +605,11 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     rc = kCoopRefused;
     int used = MFISTA_ENGINE_PERSISTENT, loop_used = 0;
     if (engine == MFISTA_ENGINE_PERSISTENT && coop) {
-      // the Toeplitz loop where it applies: L1 and L1+TSV, paired rounds,
-      // floating-point atomics allowed in its set-up, square images up to
-      // 512^2 (the presets size both axes' kernels from NX, so for NX != NY
-      // the reference's transform is not the exact one the loop applies)
-      bool toep = !tv_mode && !C.no_alloc && !P.single_trials && !P.deterministic &&
-                  (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512;
+      // the Toeplitz loop where it applies, unless the autotuner's choice for
+      // this shape is the gridding loop
+      bool toep = toeplitz_applies(C, A, opts) && C.tune_loop != MFISTA_LOOP_GRIDDING;
       if (toep) {
-        const int trc = toeplitz_setup(C, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY, preset, wv2);
+        const int trc = toeplitz_setup(C, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY, preset, wv2, C.tw.out);
         if (trc == MFISTA_E_SIZE || trc == MFISTA_E_FOV)
           toep = false;  // the set-up transform does not apply: the gridding loop
         else if (trc)
@@ -800,6 +808,16 @@ int mfista_nufft_solve_device(const double* d_u, const double* d_v, const double
 // problem's workspace, the launch, and (results = true) the FP64 RESULT pass.
 // iters > 0 runs every problem for exactly that many iterations with no
 // stopping rule and writes no results (the autotuner's candidates).
+// Whether a batch's persistent launch may run the Toeplitz loop (the rule of
+// single solves, for every problem).
+static bool toeplitz_batch_applies(int nprob, const mfista_nufft_problem* pr, const std::vector<int>& tv,
+                                   const mfista_opts* opts) {
+  if (opts && (opts->single_trials || opts->deterministic || opts->loop == MFISTA_LOOP_GRIDDING)) return false;
+  for (int i = 0; i < nprob; i++)
+    if (tv[i] || pr[i].NX != pr[i].NY || pr[i].NX > 512) return false;
+  return nprob > 0;
+}
+
 static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, const std::vector<int>& tv,
                      mfista_opts* opts, int iters, bool results, RESULT* res, double* us_per_iter) {
   const int preset = opts ? opts->preset : MFISTA_PRESET_C24;
@@ -889,27 +907,58 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   }
   std::vector<RunStats> st;
   float kms = 0;
-  // the wide engine (every kernel launched once for all problems) takes
-  // batches of L1 / L1+TSV problems of one image size; others, and the
-  // default, run in one persistent launch.  Target objectives are a control
-  // rule of the wide engine only.
+  // Engines: the wide engine (every kernel launched once for all problems)
+  // takes batches of L1 / L1+TSV problems of one image size; the persistent
+  // engine runs the batch in cooperative launches, on the Toeplitz loop when
+  // every problem is L1 / L1+TSV on a square image up to 512^2 (the rule of
+  // single solves), else on the gridding loop.  Target objectives are a
+  // control rule of the wide engine only: its objective is the preset's own
+  // (the RESULT pass's), where the Toeplitz loop's applies the exact operator.
   const int engine = C->tune_engine >= 0 ? C->tune_engine : (opts ? opts->engine : MFISTA_ENGINE_PERSISTENT);
   bool targets = false, wide_ok = !P.empty();
+  bool toep = !P.empty() && !(opts && (opts->single_trials || opts->deterministic)) &&
+              !(opts && opts->loop == MFISTA_LOOP_GRIDDING) && C->tune_loop != MFISTA_LOOP_GRIDDING;
   for (size_t j = 0; j < P.size(); j++) {
     if (P[j].target_F > 0) targets = true;
     if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide_ok = false;
+    if (P[j].tv_mode || P[j].NX != P[j].NY || P[j].NX > 512) toep = false;
   }
   if (targets && !wide_ok) {
     fprintf(stderr, "[mfista_amd] target objectives need L1 or L1+TSV problems of one image size\n");
     return MFISTA_E_ARG;
   }
   const bool wide = wide_ok && (targets || engine == MFISTA_ENGINE_STREAM);
-  if (P.empty())
+  if (!wide && toep) {  // the operator data of every visibility set a problem uses
+    std::vector<char> used(nset, 0);
+    for (int i : which) used[set_of[i]] = 1;
+    for (int k = 0; k < nset && toep; k++) {
+      if (!used[k]) continue;
+      const mfista_nufft_problem& q = pr[set_rep[k]];
+      double wv2 = 0;
+      for (int m = 0; m < q.M; m++)
+        wv2 += (q.vis_r[m] * q.vis_r[m] + q.vis_i[m] * q.vis_i[m]) / (q.vis_std[m] * q.vis_std[m]);
+      BatchSet& S = C->bset[k];
+      std::swap(C->nu, S.nu);
+      rc = toeplitz_setup(*C, (const double*)S.u.p, (const double*)S.v.p, (const double*)S.vr.p,
+                          (const double*)S.vi.p, (const double*)S.sd.p, q.M, q.NX, q.NY, preset, wv2, S.toep);
+      std::swap(C->nu, S.nu);
+      if (rc == MFISTA_E_SIZE || rc == MFISTA_E_FOV)
+        toep = false;  // the set-up transform does not apply: the gridding loop
+      else if (rc)
+        return rc;
+    }
+  }
+  if (P.empty()) {
     st.clear();
-  else if (wide)
+  } else if (wide) {
     rc = solve_nufft_wide(*C, sets, P, st, &kms);
-  else
+  } else if (toep) {
+    std::vector<const ToepOut*> ops;
+    for (int i : which) ops.push_back(&C->bset[set_of[i]].toep);
+    rc = solve_nufft_toeplitz_batch(*C, sets, ops, P, st, &kms);
+  } else {
     rc = solve_nufft_batch(*C, sets, P, st, &kms);
+  }
   if (rc) return rc;
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   int total_trials = 0, max_trials = 0, rounds = 0;
@@ -941,6 +990,7 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   }
   if (opts) {
     opts->engine_used = wide ? MFISTA_ENGINE_STREAM : MFISTA_ENGINE_PERSISTENT;
+    opts->loop_used = wide ? 0 : (toep ? MFISTA_LOOP_TOEPLITZ : MFISTA_LOOP_GRIDDING);
     opts->solve_ms = secs * 1e3;
     opts->kernel_ms = kms;
     opts->total_trials = total_trials;
@@ -999,7 +1049,8 @@ int mfista_imaging_core_nufft_batch(int nprob, const mfista_nufft_problem* pr, s
     if (!tune_lookup(k, &ch)) {
       const int iters = opts->tune_iters > 0 ? opts->tune_iters : 30;
       auto run = [&](double* us) { return run_batch(C, nprob, pr, tv, opts, iters, false, nullptr, us); };
-      if ((rc = tune_search(*C, reg, true, run, &ch, nullptr))) return rc;
+      if ((rc = tune_search(*C, reg, true, run, &ch, nullptr, toeplitz_batch_applies(nprob, pr, tv, opts))))
+        return rc;
       tune_store(k, ch);
     }
     tune_apply(*C, ch);
@@ -1033,7 +1084,7 @@ int mfista_autotune_nufft_batch(int nprob, const mfista_nufft_problem* pr, const
   auto run = [&](double* us) { return run_batch(C, nprob, pr, tv, &o, it, false, nullptr, us); };
   TuneChoice ch;
   std::vector<TuneChoice> cand;
-  if ((rc = tune_search(*C, reg, true, run, &ch, &cand))) return rc;
+  if ((rc = tune_search(*C, reg, true, run, &ch, &cand, toeplitz_batch_applies(nprob, pr, tv, &o)))) return rc;
   const TuneKey k = tune_key(pr[0].NX, pr[0].NY, o.preset, reg, mmax, nprob);
   tune_store(k, ch);
   to_public(k, ch, best);

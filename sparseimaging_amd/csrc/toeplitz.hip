@@ -58,7 +58,7 @@ __global__ void k_toep_khat(const double2* __restrict__ F, int n, double scale, 
 }
 
 int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const double* d_vr, const double* d_vi,
-                   const double* d_sd, int M, int NX, int NY, int preset, double wv2) {
+                   const double* d_sd, int M, int NX, int NY, int preset, double wv2, ToepOut& out) {
   ToepWork& T = C.tw;
   const int L = std::max(NX, NY), L2 = 2 * L;
   if (L2 > 1024 || !C.nu.ready || C.nu.NX != NX || C.nu.NY != NY) return MFISTA_E_SIZE;
@@ -85,20 +85,20 @@ int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const doubl
   const int Lx = 2 * NX, Ly = 2 * NY, nky = NY + 1;
   const size_t ngrid = (size_t)Lx * Ly, nspec = (size_t)nky * Lx;
   if (T.circ.ensure(ngrid * 8) || T.tbuf.ensure(nspec * 16) || T.fbuf.ensure(nspec * 16) ||
-      T.kh.ensure(nspec * 8) || T.bimg.ensure((size_t)NX * NY * 8))
+      out.kh.ensure(nspec * 8) || out.bimg.ensure((size_t)NX * NY * 8))
     return MFISTA_E_HIP;
   const int cy = (preset == MFISTA_PRESET_C24) ? NX / 2 : NY / 2;  // the C library's centring
   hipLaunchKernelGGL(k_toep_embed, dim3((unsigned)((ngrid + 255) / 256)), dim3(256), 0, st,
                      (const double*)T.imgp.p, (const double*)T.imgb.p, L, NX, NY, cy, (double*)T.circ.p,
-                     (double*)T.bimg.p);
+                     (double*)out.bimg.p);
   launch_rowfft_fwd<double>((const double*)T.circ.p, nullptr, nullptr, (double2*)T.tbuf.p, Lx, Ly, Ly, 0, N.Py,
                             (const double2*)N.twyd.p, st, nullptr);
   launch_colfft_fwd<double>((const double2*)T.tbuf.p, (double2*)T.fbuf.p, Lx, nky, Lx, 0, 1, N.MSP, N.Px,
                             (const double2*)N.twxd.p, st, nullptr);
   hipLaunchKernelGGL(k_toep_khat, dim3((unsigned)((nspec + 255) / 256)), dim3(256), 0, st,
-                     (const double2*)T.fbuf.p, (int)nspec, 1.0 / ((double)Lx * Ly), (double*)T.kh.p);
+                     (const double2*)T.fbuf.p, (int)nspec, 1.0 / ((double)Lx * Ly), (double*)out.kh.p);
   MF_CHECK(hipGetLastError());
-  T.wv2 = wv2;
+  out.wv2 = wv2;
   return 0;
 }
 

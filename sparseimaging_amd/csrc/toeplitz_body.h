@@ -514,4 +514,15 @@ __global__ void __launch_bounds__(MF_BLOCK) k_mfista_toeplitz(const PArgs* __res
   toeplitz_body<TAB>(*A, blockIdx.x, gridDim.x);
 }
 
+// independent solves in one cooperative launch: blk[b] = (problem, workgroup
+// index within the problem) for workgroup b
+template <bool TAB>
+__global__ void __launch_bounds__(MF_BLOCK) k_mfista_toeplitz_batch(const PArgs* __restrict__ args,
+                                                                    const int2* __restrict__ blk) {
+  const int2 pb = blk[blockIdx.x];
+  if (pb.x < 0) return;  // a workgroup slot no problem was dealt
+  const PArgs& A = args[pb.x];
+  toeplitz_body<TAB>(A, pb.y, (unsigned)A.nblk);
+}
+
 }  // namespace mf

@@ -96,17 +96,21 @@ def test_tune_table_file_round_trip(tmp_path):
     lines with an unsupported tile or a bad field count are skipped."""
     f = tmp_path / "tune.txt"
     f.write_text("# header\n256 256 1 1 14 0 32 16 2 1 0 0 88.5\n256 256 1 1 14 3 24 16 2 1 0 0 1.0\n"
-                 "256 256 1 1 14 4 16 16 2 1 7 0 1.0\n256 256 0 2 14 0 16 16 2 0 0 12 1.0\n1 2 3\n")
+                 "256 256 1 1 14 4 16 16 2 1 7 0 1.0\n256 256 0 2 14 0 16 16 2 0 0 12 1.0\n1 2 3\n"
+                 "128 128 1 1 13 0 16 16 1 1 0 0 2 70.25\n128 128 1 1 12 0 16 16 1 1 0 0 5 70.25\n")
     sa.tune_clear()
-    assert sa.tune_load(str(f)) == 1
+    assert sa.tune_load(str(f)) == 2  # a 13-field line (loop by rule) and a 14-field one
     t = sa.tune_table()
-    assert len(t) == 1 and t[0]["tile_x"] == 32 and t[0]["tile_y"] == 16 and t[0]["blocks_per_cu"] == 2
+    t = sorted(t, key=lambda e: e["NX"], reverse=True)
+    assert len(t) == 2 and t[0]["tile_x"] == 32 and t[0]["tile_y"] == 16 and t[0]["blocks_per_cu"] == 2
     assert t[0]["m_log2"] == 14 and t[0]["engine"] == sa.ENGINE_PERSISTENT and abs(t[0]["us_per_iter"] - 88.5) < 1e-9
+    assert t[0]["loop"] == 0
+    assert t[1]["NX"] == 128 and t[1]["loop"] == sa.LOOP_TOEPLITZ and abs(t[1]["us_per_iter"] - 70.25) < 1e-9
     g = tmp_path / "tune2.txt"
     sa.tune_save(str(g))
     sa.tune_clear()
     assert sa.tune_table() == []
-    assert sa.tune_load(str(g)) == 1 and sa.tune_table() == t
+    assert sa.tune_load(str(g)) == 2 and sorted(sa.tune_table(), key=lambda e: e["NX"], reverse=True) == t
     sa.tune_clear()
 
 

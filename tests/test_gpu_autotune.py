@@ -61,7 +61,7 @@ def test_tiles_l1_tsv_match_oracle(tmp_path, preset, tile, bpc, paired):
     _force(tmp_path, 64, 64, preset, 1, 2000, 1, tile, bpc, paired)
     runs = sa.tune_runs()
     x, r, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
-                                              preset=preset, autotune=True, **kw)
+                                              preset=preset, autotune=True, loop=sa.LOOP_GRIDDING, **kw)
     assert sa.tune_runs() == runs  # the loaded entry was used, nothing was timed
     assert info["engine_used"] == sa.ENGINE_PERSISTENT
     if paired == 0:
@@ -136,7 +136,8 @@ def test_batch_with_forced_tile_matches_oracle(tmp_path):
     probs = [dict(base, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
     print("case batch of 3 (64^2, M=1500) with 32x32 tiles, 3000 iterations, against the oracle", flush=True)
     _force(tmp_path, 64, 64, sa.PRESET_CXX12, 1, 1500, 3, (32, 32))
-    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, autotune=True)
+    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, autotune=True,
+                                                      loop=sa.LOOP_GRIDDING)
     assert info["engine_used"] == sa.ENGINE_PERSISTENT
     for p, xb, rb in zip(probs, xs, rs):
         assert rb["ITER"] == 3000
@@ -154,10 +155,12 @@ def test_autotune_benchmark_workload():
     print("case autotune 256^2 M=20000 12 taps L1+TSV, 30 iterations per candidate", flush=True)
     best, cand = sa.autotune_nufft(*args, lambda_l1=1.0, lambda_tsv=100.0, preset=sa.PRESET_CXX12, iters=30)
     for c in cand:
-        print(f"  tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} paired {c['paired']}: "
-              f"{c['us_per_iter']:.1f} us/iteration", flush=True)
+        print(f"  loop {c['loop']} tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} "
+              f"paired {c['paired']}: {c['us_per_iter']:.1f} us/iteration", flush=True)
     print(f"  chosen: {best}", flush=True)
-    assert len(cand) == 12 and sa.tune_runs() - runs == 24
+    # 12 gridding-loop candidates and the Toeplitz loop at 1 and 2 workgroups per CU
+    assert len(cand) == 14 and sa.tune_runs() - runs == 28
+    assert sorted(c["loop"] for c in cand) == [sa.LOOP_GRIDDING] * 12 + [sa.LOOP_TOEPLITZ] * 2
     assert best["us_per_iter"] == min(c["us_per_iter"] for c in cand)
     assert (best["NX"], best["NY"], best["preset"], best["reg"], best["m_log2"]) == (256, 256, sa.PRESET_CXX12, 1, 14)
     assert len(sa.tune_table()) == 1
@@ -165,7 +168,8 @@ def test_autotune_benchmark_workload():
     x, r, info = sa.mfista_imaging_core_nufft(*args, maxiter=50000, eps=0.0, lambda_l1=1.0, lambda_tsv=100.0,
                                               cinit=1e4, preset=sa.PRESET_CXX12, rel_tol=1e-6, autotune=True)
     assert sa.tune_runs() == runs
-    print(f"  tuned solve: ITER {r['ITER']} {info['kernel_ms']:.1f} ms", flush=True)
+    assert info["loop_used"] == best["loop"]  # the tuned solve runs the chosen loop
+    print(f"  tuned solve: ITER {r['ITER']} {info['kernel_ms']:.1f} ms, loop {info['loop_used']}", flush=True)
     o = oracle.imaging_core_nufft(*args[:5], 256, 256, r["ITER"], 0.0, 1.0, 0.0, 100.0, 1e4, nonneg=1,
                                   preset=oracle.PRESET_CXX12, rel_tol=0.0)
     _cmp("tuned vs oracle at the same iteration count", x, r, o["x"], o["result"]["finalcost"])
@@ -173,8 +177,9 @@ def test_autotune_benchmark_workload():
 
 
 def test_autotune_lambda_grid_batch():
-    """The tuner on the 6x6 lambda grid at 128^2 (15 candidates: 12 persistent
-    launches and the wide engine with 3 tiles); the tuned batch at a fixed
+    """The tuner on the 6x6 lambda grid at 128^2 (17 candidates: 12 persistent
+    launches of the gridding loop, the Toeplitz loop at 1 and 2 workgroups per
+    CU and the wide engine with 3 tiles); the tuned batch at a fixed
     3000 iterations against the oracle's solutions of two grid points
     (tests/golden/batch_oracle.npz) to the parity bars, and every problem
     against the default persistent batch (the two engines' trajectories differ
@@ -194,7 +199,8 @@ def test_autotune_lambda_grid_batch():
         print(f"  tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} paired {c['paired']} "
               f"engine {c['engine']}: {c['us_per_iter']:.1f} us/iteration", flush=True)
     print(f"  chosen: {best}", flush=True)
-    assert best["nprob_log2"] == 5 and len(cand) == 15
+    assert best["nprob_log2"] == 5 and len(cand) == 17
+    assert sum(c["loop"] == sa.LOOP_TOEPLITZ for c in cand) == 2
     assert best["us_per_iter"] == min(c["us_per_iter"] for c in cand)
     xa, ra, ia = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12)
     xt, rt, it = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, autotune=True)

@@ -122,20 +122,29 @@ def test_both_regularisers_rejected():
     assert e.value.status == -1
 
 
-def test_persistent_trace_summary(tmp_path, monkeypatch):
+@pytest.mark.parametrize("loop", [sa.LOOP_GRIDDING, sa.LOOP_AUTO])
+def test_persistent_trace_summary(tmp_path, monkeypatch, loop):
     """MFISTA_PERSISTENT_TRACE=<file>: the persistent solver timestamps every
-    grid barrier; the summary's stage and tail times add up to the launch."""
+    grid barrier; the summary's stage and tail times add up to the launch
+    (the gridding loop: five stages per iteration; the Toeplitz loop, the
+    default here: three per trial round)."""
     P = _problem(64, 2000, 7, 0.35)
     path = tmp_path / "trace.txt"
     monkeypatch.setenv("MFISTA_PERSISTENT_TRACE", str(path))
     x, r, info = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64,
                                               maxiter=300, eps=0.0, lambda_l1=1.0, lambda_tsv=10.0, cinit=1e4,
-                                              nonneg_flag=1, preset=sa.PRESET_CXX12)
+                                              nonneg_flag=1, preset=sa.PRESET_CXX12, loop=loop)
     text = path.read_text()
     print(text, flush=True)
-    for stage in ("P1-grid", "P2-colinv", "T1-rows", "T2-cols", "T3-degrid"):
-        assert stage in text
-    counts = {ln.split()[0]: int(ln.split()[1]) for ln in text.splitlines() if ln.startswith(("P1", "P2", "T1"))}
-    assert counts["P1-grid"] == 300 and counts["P2-colinv"] == 300 and counts["T1-rows"] == info["trial_rounds"]
+    counts = {ln.split()[0]: int(ln.split()[1]) for ln in text.splitlines() if ln.startswith(("P1", "P2", "T"))}
+    if loop == sa.LOOP_GRIDDING:
+        assert info["loop_used"] == sa.LOOP_GRIDDING and "gridding loop" in text
+        for stage in ("P1-grid", "P2-colinv", "T1-rows", "T2-cols", "T3-degrid"):
+            assert stage in text
+        assert counts["P1-grid"] == 300 and counts["P2-colinv"] == 300 and counts["T1-rows"] == info["trial_rounds"]
+    else:
+        assert info["loop_used"] == sa.LOOP_TOEPLITZ and "Toeplitz loop" in text
+        for stage in ("T1-rows", "T2-conv", "T3-c2r"):
+            assert counts[stage] == info["trial_rounds"]
     total = float(text.split("sum over traced barriers")[1].split()[0])
     assert 0.7 * info["kernel_ms"] <= total <= 1.05 * info["kernel_ms"]

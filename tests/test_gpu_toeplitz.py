@@ -123,3 +123,30 @@ def test_loop_selection():
                                             lambda_tsv=1.0, preset=sa.PRESET_CXX12, **base)
     assert i0["loop_used"] == sa.LOOP_TOEPLITZ
     assert [i["loop_used"] for i in (i1, i2, i3, i4, i5)] == [sa.LOOP_GRIDDING] * 5
+
+
+def test_toeplitz_batch_matches_single_solves_and_oracle():
+    """Three lambda_tsv values over one visibility set as one persistent batch
+    (the Toeplitz loop, one set-up shared by the batch) and as single solves,
+    4000 iterations: every problem against its single solve and the middle one
+    against the oracle, to the parity bars."""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
+                maxiter=4000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
+    print("case toeplitz batch of 3 (64^2, M=2000), 4000 iterations", flush=True)
+    xs, rs, info = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12)
+    assert info["engine_used"] == sa.ENGINE_PERSISTENT and info["loop_used"] == sa.LOOP_TOEPLITZ
+    for p, xb, rb in zip(probs, xs, rs):
+        xg, rg, ig = _gpu(P, 64, 64, sa.PRESET_CXX12, 4000, 0.0, 1.0, p["lambda_tsv"])
+        l2 = np.linalg.norm(xb - xg) / np.linalg.norm(xg)
+        dc = abs(rb["finalcost"] - rg["finalcost"]) / rg["finalcost"]
+        print(f"  lambda_tsv {p['lambda_tsv']:g}: ITER {rb['ITER']} batch vs single relL2={l2:.2e} cost rel={dc:.2e}")
+        assert rb["ITER"] == 4000 and ig["loop_used"] == sa.LOOP_TOEPLITZ
+        assert l2 <= 1e-3 and dc <= 1e-6
+    o = _oracle(P, 64, 64, sa.PRESET_CXX12, 4000, 0.0, 1.0, 10.0)
+    l2, dc = _compare(xs[1], rs[1], o)
+    print(f"  lambda_tsv 10 vs oracle: relL2={l2:.2e} cost rel={dc:.2e}")
+    assert l2 <= 1e-3 and dc <= 1e-6
+    xg, rg, ig = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, loop=sa.LOOP_GRIDDING)
+    assert ig["loop_used"] == sa.LOOP_GRIDDING
