@@ -168,8 +168,9 @@ typedef struct mfista_opts {
                         up to 512 x 512 (not with deterministic, single_trials or the
                         allocation-free entry point), the gridding loop otherwise;
                         MFISTA_LOOP_GRIDDING always runs the gridding loop */
-  int loop_used;     /* out: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ (persistent
-                        engine), 0 when the stream engine ran */
+  int loop_used;     /* out: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ, the loop that ran
+                        (batches: the steps of the persistent or the wide engine); 0 when a
+                        single solve ran on the stream engine */
 } mfista_opts;
 
 void mfista_default_opts(mfista_opts *opts);

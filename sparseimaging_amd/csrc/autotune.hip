@@ -106,14 +106,22 @@ std::vector<TuneChoice> tune_candidates(int reg, bool batch, bool toep) {
       c.loop = MFISTA_LOOP_TOEPLITZ;
       v.push_back(c);
     }
-  if (batch && reg != 2)  // the wide engine: the gridding tile is its only knob
+  if (batch && reg != 2) {  // the wide engine: the gridding tile is its only knob (gridding steps)
     for (const auto& t : tiles) {
       TuneChoice c;
       c.tile_x = t[0];
       c.tile_y = t[1];
       c.engine = MFISTA_ENGINE_STREAM;
+      if (toep) c.loop = MFISTA_LOOP_GRIDDING;
       v.push_back(c);
     }
+    if (toep) {  // Toeplitz steps
+      TuneChoice c;
+      c.engine = MFISTA_ENGINE_STREAM;
+      c.loop = MFISTA_LOOP_TOEPLITZ;
+      v.push_back(c);
+    }
+  }
   return v;
 }
 

@@ -66,9 +66,11 @@ int solve_nufft_batch(Context& C, const std::vector<NufftState*>& sets, const st
                       std::vector<RunStats>& out, float* kernel_ms);
 // independent solves of one image size and preset with every kernel launched
 // once for all problems, one step captured as a graph (wide.hip); L1 and
-// L1+TSV only
+// L1+TSV only.  ops (square images): the Toeplitz steps, with problem i's
+// visibility set's operator data ops[i] (toeplitz_setup)
 int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
-                     std::vector<RunStats>& out, float* kernel_ms);
+                     std::vector<RunStats>& out, float* kernel_ms,
+                     const std::vector<const ToepOut*>* ops = nullptr);
 // the frames of a coupled movie in one launch with one control step (persistent.hip)
 int solve_nufft_movie(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
                       double lambda_t, const std::vector<double>& w, std::vector<RunStats>& out,

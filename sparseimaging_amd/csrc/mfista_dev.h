@@ -196,9 +196,9 @@ __device__ inline void dev_control(DevState* S, double* cost, const double* __re
   const int it = S->iter;
   if (prologue) {
     cost[it] = S->costtmp;
-    S->Qcore = chi_z / 2 + (tsv ? S->lambda_tsv * tsv_z : 0.0);
+    S->Qcore = (chi_z + S->chi_add) / 2 + (tsv ? S->lambda_tsv * tsv_z : 0.0);
   }
-  double F = chi_x / 2 + (tsv ? S->lambda_tsv * tsvn : 0.0);
+  double F = (chi_x + S->chi_add) / 2 + (tsv ? S->lambda_tsv * tsvn : 0.0);
   const double Q = -s_dg + S->c * s_d2 / 2 + S->Qcore;
   S->last_F = F;
   S->last_Q = Q;
@@ -290,7 +290,7 @@ __device__ inline void dev_init_control(DevState* S, const double* __restrict__ 
   const double tsv = block_reduce_range(stats + 1, nsb, 4, red);
   const double tv = block_reduce_range(stats + 2, nsb, 4, red);
   if (threadIdx.x != 0) return;
-  double c0 = chi / 2 + S->lambda_l1 * l1;
+  double c0 = (chi + S->chi_add) / 2 + S->lambda_l1 * l1;
   if (S->tv_mode)
     c0 += S->lambda_tv * tv;
   else if (S->lambda_tsv > 0)

@@ -34,6 +34,8 @@ struct DevState {
   int gate_prologue, gate_trial, gate_momentum;  // nonzero: the kernel returns at once
   int rounds;                                    // persistent engine: trial rounds evaluated
   double target_F;  // > 0: also stop once the objective is at or below it (stream and wide engines)
+  double chi_add;   // added to every chi^2 sum: |W V|^2 in the Toeplitz loop (whose partials
+                    // are <x, T x> - 2 <x, b>), 0 otherwise
 };
 
 // offsets (in doubles) of the partial-sum regions written by the kernels

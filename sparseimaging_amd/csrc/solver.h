@@ -93,6 +93,7 @@ struct BatchSet {
 
 // Device workspace of one problem of the wide engine (wide.hip).
 struct WideWork {
+  DevMem tvec;               // Toeplitz steps: T x, T xnew, T z
   DevMem x, z, xnew, gr, d;  // FP64 iterates, gradient, step
   DevMem T64, Gf64, mod;     // FP64 forward transform; models W A x, W A xnew, W A z
   DevMem rz32, Hg, U;        // FP32 adjoint

@@ -909,11 +909,12 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   float kms = 0;
   // Engines: the wide engine (every kernel launched once for all problems)
   // takes batches of L1 / L1+TSV problems of one image size; the persistent
-  // engine runs the batch in cooperative launches, on the Toeplitz loop when
-  // every problem is L1 / L1+TSV on a square image up to 512^2 (the rule of
-  // single solves), else on the gridding loop.  Target objectives are a
-  // control rule of the wide engine only: its objective is the preset's own
-  // (the RESULT pass's), where the Toeplitz loop's applies the exact operator.
+  // engine runs the batch in cooperative launches.  Both run Toeplitz steps
+  // when every problem is L1 / L1+TSV on a square image up to 512^2 (the rule
+  // of single solves), else gridding steps.  Target objectives are a control
+  // rule of the wide engine; with Toeplitz steps the objective they are
+  // compared with applies the exact operator, within ~1e-8 (relative) of the
+  // preset's objective that the RESULT pass reports.
   const int engine = C->tune_engine >= 0 ? C->tune_engine : (opts ? opts->engine : MFISTA_ENGINE_PERSISTENT);
   bool targets = false, wide_ok = !P.empty();
   bool toep = !P.empty() && !(opts && (opts->single_trials || opts->deterministic)) &&
@@ -928,7 +929,7 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
     return MFISTA_E_ARG;
   }
   const bool wide = wide_ok && (targets || engine == MFISTA_ENGINE_STREAM);
-  if (!wide && toep) {  // the operator data of every visibility set a problem uses
+  if (toep) {  // the operator data of every visibility set a problem uses
     std::vector<char> used(nset, 0);
     for (int i : which) used[set_of[i]] = 1;
     for (int k = 0; k < nset && toep; k++) {
@@ -951,7 +952,9 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   if (P.empty()) {
     st.clear();
   } else if (wide) {
-    rc = solve_nufft_wide(*C, sets, P, st, &kms);
+    std::vector<const ToepOut*> ops;
+    for (int i : which) ops.push_back(&C->bset[set_of[i]].toep);
+    rc = solve_nufft_wide(*C, sets, P, st, &kms, toep ? &ops : nullptr);
   } else if (toep) {
     std::vector<const ToepOut*> ops;
     for (int i : which) ops.push_back(&C->bset[set_of[i]].toep);
@@ -990,7 +993,7 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   }
   if (opts) {
     opts->engine_used = wide ? MFISTA_ENGINE_STREAM : MFISTA_ENGINE_PERSISTENT;
-    opts->loop_used = wide ? 0 : (toep ? MFISTA_LOOP_TOEPLITZ : MFISTA_LOOP_GRIDDING);
+    opts->loop_used = toep ? MFISTA_LOOP_TOEPLITZ : MFISTA_LOOP_GRIDDING;
     opts->solve_ms = secs * 1e3;
     opts->kernel_ms = kms;
     opts->total_trials = total_trials;

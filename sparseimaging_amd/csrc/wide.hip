@@ -23,6 +23,13 @@
 // Everything a kernel needs is in device memory (per-problem descriptors and
 // states), so replays need no host work.  No memset is captured: the adjoint
 // grids are cleared by the column pass that reads them.
+//
+// Toeplitz steps (toeplitz_body.h): for L1 / L1+TSV problems on square images
+// the data term's normal operator is applied as an FP64 FFT convolution, so
+// the images T x, T xnew and T z replace the visibility-domain models, the
+// gradient at z is b - T z (one elementwise kernel instead of the gridding
+// adjoint) and a trial's forward transform is the row FFT of its step, the
+// column product with Kh and the c2r (no degridding).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -30,9 +37,9 @@
 
 #include "driver.h"
 #include "mfista_dev.h"
-#include "persistent_body.h"
 #include "solver.h"
 #include "stages.h"
+#include "toeplitz_body.h"
 
 namespace mf {
 
@@ -54,6 +61,9 @@ struct WProb {
   DevState* S;
   const float* box;
   double lambda_tsv;
+  // Toeplitz steps: the set's kernel spectrum and b, and T x, T xnew, T z
+  const double *kh, *bimg;
+  double *tx, *txn, *tz;
 };
 
 // What the problems of a batch share: image size, preset, FFT plans, twiddle
@@ -68,6 +78,7 @@ struct WShared {
   int nseq_r64, nseq_c64, nseq_ci32, nseq_c2r32;
   float adj_scale;  // 1 / (8 NX NY)
   int* ndone;       // problems finished
+  int toep;         // Toeplitz steps (no E4 weighting in the row pass)
 };
 
 // which transform a forward kernel computes: of the start (no gate) or of a
@@ -84,7 +95,11 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_rowfft64(const WProb* __restrict_
   extern __shared__ double4 smem_d4[];
   const double* x = (which == kWInit) ? p.x : p.d;
   const int NY = sh.NY;
-  auto load = [&](int i, int j) -> double { return x[(size_t)i * NY + j] * sh.e4x64[i] * sh.e4y64[j]; };
+  const bool e4 = !sh.toep;
+  auto load = [&](int i, int j) -> double {
+    const double v = x[(size_t)i * NY + j];
+    return e4 ? v * sh.e4x64[i] * sh.e4y64[j] : v;
+  };
   st_rowfft_fwd<double>(blockIdx.x, sh.nseq_r64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64, p.T64,
                         reinterpret_cast<double2*>(smem_d4), load);
 }
@@ -386,6 +401,88 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_init_control(const WProb* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Toeplitz steps.
+
+// the gradient at z without its TSV term, b - T z, and the partials of
+// chi^2(z) - |W V|^2 = <z, T z> - 2 <z, b> (L.chi_z, one per element block)
+__global__ void __launch_bounds__(MF_BLOCK) kwt_grad(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  if (p.S->gate_prologue) return;
+  __shared__ double red[16];
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {0.0};
+  if (f < sh.NX * sh.NY) {
+    const double b = p.bimg[f], tz = p.tz[f], z = p.z[f];
+    p.gr[f] = b - tz;
+    v[0] = z * (tz - 2.0 * b);
+  }
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) p.part[p.L.chi_z + blockIdx.x] = v[0];
+}
+
+// column pass of the Toeplitz product of the start (init) or of a trial's
+// step: task ky < NY (row 0 carrying the Nyquist row), the output in Gf64
+__global__ void __launch_bounds__(MF_BLOCK) kwt_colconv(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if (w_closed(p.S, which)) return;
+  extern __shared__ double4 smem_d4[];
+  st_colconv1<double>(blockIdx.x, p.T64, p.kh, p.Gf64, sh.NX, sh.NY + 1, sh.Lx, sh.NX / 2, sh.Px, sh.twx64,
+                      reinterpret_cast<double2*>(smem_d4));
+}
+
+// c2r of the column pass's output, two image rows per task:
+//   init:  T x = T z = the result; partials of <x, T x> - 2 <x, b> (L.chi_x)
+//   trial: T xnew = T z + T d; partials of <xnew, T xnew> - 2 <xnew, b>
+//          (L.chi_x) and of <d, T d> = |W A d|^2 (L.lip)
+__global__ void __launch_bounds__(MF_BLOCK) kwt_c2r(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if (w_closed(p.S, which)) return;
+  extern __shared__ double4 smem_d4[];
+  __shared__ double red[32];
+  const int NY = sh.NY;
+  double v[2] = {0.0, 0.0};
+  const bool init = which == kWInit;
+  auto store = [&](int i, int j, double td) {
+    const size_t f = (size_t)i * NY + j;
+    if (init) {
+      p.tx[f] = td;
+      p.tz[f] = td;
+      v[0] += p.x[f] * (td - 2.0 * p.bimg[f]);
+    } else {
+      const double xv = p.xnew[f], tn = p.tz[f] + td;
+      p.txn[f] = tn;
+      v[0] += xv * (tn - 2.0 * p.bimg[f]);
+      v[1] += p.d[f] * td;
+    }
+  };
+  st_rowfft_c2r<double>(blockIdx.x, 1, p.Gf64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64,
+                        reinterpret_cast<double2*>(smem_d4), store);
+  block_sum<2>(v, red);
+  if (threadIdx.x == 0) {
+    p.part[p.L.chi_x + blockIdx.x] = v[0];
+    if (!init) p.part[p.L.lip + blockIdx.x] = v[1];
+  }
+}
+
+// the momentum step in both images: z = a x + b xnew, T z = a T x + b T xnew;
+// x <- xnew and T x <- T xnew on acceptance
+__global__ void __launch_bounds__(MF_BLOCK) kwt_momentum(const WProb* __restrict__ W, WShared sh) {
+  const WProb& p = W[blockIdx.y];
+  const DevState* S = p.S;
+  if (S->gate_momentum) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= sh.NX * sh.NY) return;
+  const double a = S->mom_a, b = S->mom_b;
+  const double xf = p.x[f], xn = p.xnew[f], tf = p.tx[f], tn = p.txn[f];
+  p.z[f] = xf * a + b * xn;
+  p.tz[f] = tf * a + b * tn;
+  if (S->accept_x) {
+    p.x[f] = xn;
+    p.tx[f] = tn;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 
 static int nseq_for(int L, size_t csize, int nseq_total) {
@@ -399,8 +496,8 @@ static int nseq_for(int L, size_t csize, int nseq_total) {
 }
 
 struct WLaunch {
-  dim3 rowf, colf, deg, grid, colinv, c2r, elem, one, vis;
-  size_t sh_rowf, sh_colf, sh_grid, sh_colinv, sh_c2r;
+  dim3 rowf, colf, deg, grid, colinv, c2r, elem, one, vis, conv, c2r64;
+  size_t sh_rowf, sh_colf, sh_grid, sh_colinv, sh_c2r, sh_conv, sh_c2r64;
 };
 
 static void launch_forward(const WLaunch& L, const WProb* W, const WShared& sh, int which, int MSP,
@@ -432,10 +529,24 @@ static void launch_step(const WLaunch& L, const WProb* W, const WShared& sh, int
   hipLaunchKernelGGL(kw_momentum, dim3(L.elem.x + L.vis.x, L.elem.y), dim3(MF_BLOCK), 0, st, W, sh, (int)L.elem.x);
 }
 
+// one Toeplitz trial step of every problem
+static void launch_step_toep(const WLaunch& L, const WProb* W, const WShared& sh, hipStream_t st) {
+  hipLaunchKernelGGL(kwt_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_tsv_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_prox, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+  hipLaunchKernelGGL(kw_rowfft64, L.rowf, dim3(MF_BLOCK), L.sh_rowf, st, W, sh, (int)kWTrial);
+  hipLaunchKernelGGL(kwt_colconv, L.conv, dim3(MF_BLOCK), L.sh_conv, st, W, sh, (int)kWTrial);
+  hipLaunchKernelGGL(kwt_c2r, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWTrial);
+  hipLaunchKernelGGL(kw_control, L.one, dim3(MF_BLOCK), 0, st, W, sh.ndone);
+  hipLaunchKernelGGL(kwt_momentum, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
+}
+
 int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std::vector<RunParams>& P,
-                     std::vector<RunStats>& out, float* kernel_ms) {
+                     std::vector<RunStats>& out, float* kernel_ms, const std::vector<const ToepOut*>* ops) {
   const int B = (int)P.size();
   if (B <= 0 || (int)sets.size() != B) return MFISTA_E_ARG;
+  const bool toep = ops != nullptr;
+  if (toep && ((int)ops->size() != B || P[0].NX != P[0].NY)) return MFISTA_E_ARG;
   NufftState& N0 = *sets[0];
   const int NX = P[0].NX, NY = P[0].NY, n = NX * NY, MSP = N0.MSP;
   for (int b = 0; b < B; b++) {
@@ -471,6 +582,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   sh.nseq_ci32 = nseq_for(g0.Lx, sizeof(float2), nky);
   sh.nseq_c2r32 = nseq_for(g0.Ly, sizeof(float2), (NX + 1) / 2);
   sh.adj_scale = (float)(1.0 / (8.0 * (double)NX * NY));
+  sh.toep = toep ? 1 : 0;
   // per-problem workspaces and descriptors
   if ((int)C.wwork.size() < B) C.wwork.resize(B);
   std::vector<WProb> hw(B);
@@ -481,7 +593,9 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     NufftState& N = *sets[b];
     WideWork& Wk = C.wwork[b];
     const RunParams& R = P[b];
-    const int ndeg = degrid_blocks(N.M, MSP);
+    // partials of a trial's chi^2 and |W A d|^2: one per degrid task, or per
+    // c2r task (two image rows) in Toeplitz steps
+    const int ndeg = toep ? (NX + 1) / 2 : degrid_blocks(N.M, MSP);
     PartialLayout L;
     L.n_degrid = ndeg;
     L.n_elem = n_elem;
@@ -492,19 +606,25 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     L.prox = L.tsv_z + n_elem;
     L.total = L.prox + kProxW * n_elem;
     // chi^2 at z comes from the visibility-domain momentum, one partial per
-    // 256 visibilities, after the other regions
-    L.n_chiz = (N.M + MF_BLOCK - 1) / MF_BLOCK;
+    // 256 visibilities (Toeplitz steps: from the gradient kernel, one per
+    // element block), after the other regions
+    L.n_chiz = toep ? n_elem : (N.M + MF_BLOCK - 1) / MF_BLOCK;
     L.chi_z = L.total;
     L.total += L.n_chiz;
     if (Wk.x.ensure(rb) || Wk.z.ensure(rb) || Wk.xnew.ensure(rb) || Wk.gr.ensure(rb) || Wk.d.ensure(rb) ||
         Wk.T64.ensure((size_t)nky * NX * sizeof(double2)) ||
-        Wk.Gf64.ensure(((size_t)NY + MSP) * g0.Lx * sizeof(double2)) ||
-        Wk.mod.ensure((size_t)3 * N.M * sizeof(double2)) || Wk.rz32.ensure((size_t)N.M * sizeof(float2)) ||
-        Wk.Hg.ensure((size_t)nky * g0.Lx * sizeof(float2)) || Wk.U.ensure((size_t)NX * nky * sizeof(float2)) ||
-        Wk.part.ensure((size_t)L.total * 8) || Wk.stat.ensure((size_t)4 * n_elem * 8) ||
-        Wk.state.ensure(sizeof(DevState)) || Wk.cost.ensure((size_t)(R.maxiter > 0 ? R.maxiter : 1) * 8))
+        Wk.Gf64.ensure(((size_t)NY + MSP) * g0.Lx * sizeof(double2)) || Wk.part.ensure((size_t)L.total * 8) ||
+        Wk.stat.ensure((size_t)4 * n_elem * 8) || Wk.state.ensure(sizeof(DevState)) ||
+        Wk.cost.ensure((size_t)(R.maxiter > 0 ? R.maxiter : 1) * 8))
       return MFISTA_E_HIP;
-    MF_CHECK(hipMemsetAsync(Wk.Hg.p, 0, (size_t)nky * g0.Lx * sizeof(float2), st));
+    if (toep) {
+      if (Wk.tvec.ensure(3 * rb)) return MFISTA_E_HIP;
+    } else {
+      if (Wk.mod.ensure((size_t)3 * N.M * sizeof(double2)) || Wk.rz32.ensure((size_t)N.M * sizeof(float2)) ||
+          Wk.Hg.ensure((size_t)nky * g0.Lx * sizeof(float2)) || Wk.U.ensure((size_t)NX * nky * sizeof(float2)))
+        return MFISTA_E_HIP;
+      MF_CHECK(hipMemsetAsync(Wk.Hg.p, 0, (size_t)nky * g0.Lx * sizeof(float2), st));
+    }
     MF_CHECK(hipMemcpyAsync(Wk.x.p, R.xinit, rb, hipMemcpyDeviceToDevice, st));
     MF_CHECK(hipMemcpyAsync(Wk.z.p, R.xinit, rb, hipMemcpyDeviceToDevice, st));
     DevState h;
@@ -519,6 +639,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     h.nonneg = R.nonneg;
     h.lip_mode = 1;
     h.target_F = R.target_F;
+    h.chi_add = toep ? (*ops)[b]->wv2 : 0.0;
     MF_CHECK(hipMemcpyAsync(Wk.state.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
     WProb& w = hw[b];
     memset(&w, 0, sizeof(w));
@@ -550,6 +671,16 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     w.S = (DevState*)Wk.state.p;
     w.box = R.box_flag ? R.box : nullptr;
     w.lambda_tsv = R.lambda_tsv;
+    if (toep) {
+      w.kh = (const double*)(*ops)[b]->kh.p;
+      w.bimg = (const double*)(*ops)[b]->bimg.p;
+      w.tx = (double*)Wk.tvec.p;
+      w.txn = w.tx + n;
+      w.tz = w.txn + n;
+      w.mx = w.mxn = w.mz = nullptr;
+      w.rz32 = nullptr;
+      w.Hg = w.U = nullptr;
+    }
   }
   if (C.wdesc.ensure((size_t)B * sizeof(WProb)) || C.wdone.ensure(sizeof(int))) return MFISTA_E_HIP;
   MF_CHECK(hipMemcpyAsync(C.wdesc.p, hw.data(), (size_t)B * sizeof(WProb), hipMemcpyHostToDevice, st));
@@ -582,13 +713,23 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     L.elem = dim3(n_elem, n);
     L.one = dim3(1, n);
     L.vis = dim3(mvb, n);
+    L.conv = dim3(NY, n);  // st_colconv1: NY tasks (row 0 carries the Nyquist row)
+    L.sh_conv = 2 * (size_t)g0.Lx * sizeof(double2);
+    L.c2r64 = dim3((NX + 1) / 2, n);
+    L.sh_c2r64 = 2 * (size_t)g0.Ly * sizeof(double2);
     return L;
   };
   WLaunch L = shape(act);
   // start: the model of xinit, its cost (chi^2, |x|_1, TSV), z = x in both
   // domains, and the initial control state
-  launch_forward(L, W, sh, kWInit, MSP, st);
-  hipLaunchKernelGGL(kw_vis_momentum, L.vis, dim3(MF_BLOCK), 0, st, W, 1);
+  if (toep) {  // T x0 (= T z), its chi^2 partials
+    hipLaunchKernelGGL(kw_rowfft64, L.rowf, dim3(MF_BLOCK), L.sh_rowf, st, W, sh, (int)kWInit);
+    hipLaunchKernelGGL(kwt_colconv, L.conv, dim3(MF_BLOCK), L.sh_conv, st, W, sh, (int)kWInit);
+    hipLaunchKernelGGL(kwt_c2r, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWInit);
+  } else {
+    launch_forward(L, W, sh, kWInit, MSP, st);
+    hipLaunchKernelGGL(kw_vis_momentum, L.vis, dim3(MF_BLOCK), 0, st, W, 1);
+  }
   hipLaunchKernelGGL(kw_stats, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   hipLaunchKernelGGL(kw_init_control, L.one, dim3(MF_BLOCK), 0, st, W, n_elem);
   MF_CHECK(hipGetLastError());
@@ -604,7 +745,10 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
     exec = nullptr;
     graph = nullptr;
     MF_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    launch_step(L, W, sh, MSP, st);
+    if (toep)
+      launch_step_toep(L, W, sh, st);
+    else
+      launch_step(L, W, sh, MSP, st);
     if (hipStreamEndCapture(st, &graph) != hipSuccess) return MFISTA_E_HIP;
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) return MFISTA_E_HIP;
     return 0;

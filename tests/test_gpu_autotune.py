@@ -177,9 +177,10 @@ def test_autotune_benchmark_workload():
 
 
 def test_autotune_lambda_grid_batch():
-    """The tuner on the 6x6 lambda grid at 128^2 (17 candidates: 12 persistent
+    """The tuner on the 6x6 lambda grid at 128^2 (18 candidates: 12 persistent
     launches of the gridding loop, the Toeplitz loop at 1 and 2 workgroups per
-    CU and the wide engine with 3 tiles); the tuned batch at a fixed
+    CU, the wide engine's gridding steps with 3 tiles and its Toeplitz steps);
+    the tuned batch at a fixed
     3000 iterations against the oracle's solutions of two grid points
     (tests/golden/batch_oracle.npz) to the parity bars, and every problem
     against the default persistent batch (the two engines' trajectories differ
@@ -199,8 +200,8 @@ def test_autotune_lambda_grid_batch():
         print(f"  tile {c['tile_x']}x{c['tile_y']} blocks/CU {c['blocks_per_cu']} paired {c['paired']} "
               f"engine {c['engine']}: {c['us_per_iter']:.1f} us/iteration", flush=True)
     print(f"  chosen: {best}", flush=True)
-    assert best["nprob_log2"] == 5 and len(cand) == 17
-    assert sum(c["loop"] == sa.LOOP_TOEPLITZ for c in cand) == 2
+    assert best["nprob_log2"] == 5 and len(cand) == 18
+    assert sum(c["loop"] == sa.LOOP_TOEPLITZ for c in cand) == 3
     assert best["us_per_iter"] == min(c["us_per_iter"] for c in cand)
     xa, ra, ia = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12)
     xt, rt, it = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, autotune=True)

@@ -150,3 +150,51 @@ def test_toeplitz_batch_matches_single_solves_and_oracle():
     assert l2 <= 1e-3 and dc <= 1e-6
     xg, rg, ig = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, loop=sa.LOOP_GRIDDING)
     assert ig["loop_used"] == sa.LOOP_GRIDDING
+
+
+def test_toeplitz_wide_engine_matches_persistent_and_oracle():
+    """The wide engine's Toeplitz steps (every stage one launch for all
+    problems): three lambda_tsv values at a fixed 4000 iterations against the
+    persistent Toeplitz batch and the oracle, to the parity bars."""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
+                maxiter=4000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
+    print("case toeplitz wide engine, 3 problems (64^2, M=2000), 4000 iterations", flush=True)
+    xw, rw, iw = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM)
+    assert iw["engine_used"] == sa.ENGINE_STREAM and iw["loop_used"] == sa.LOOP_TOEPLITZ
+    xp, rp, ip = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12)
+    assert ip["loop_used"] == sa.LOOP_TOEPLITZ
+    for p, a, b, c, d in zip(probs, xw, rw, xp, rp):
+        l2 = np.linalg.norm(a - c) / np.linalg.norm(c)
+        dc = abs(b["finalcost"] - d["finalcost"]) / d["finalcost"]
+        print(f"  lambda_tsv {p['lambda_tsv']:g}: wide vs persistent relL2={l2:.2e} cost rel={dc:.2e}")
+        assert b["ITER"] == 4000 and l2 <= 1e-3 and dc <= 1e-6
+    o = _oracle(P, 64, 64, sa.PRESET_CXX12, 4000, 0.0, 1.0, 10.0)
+    l2, dc = _compare(xw[1], rw[1], o)
+    print(f"  wide lambda_tsv 10 vs oracle: relL2={l2:.2e} cost rel={dc:.2e}")
+    assert l2 <= 1e-3 and dc <= 1e-6
+
+
+def test_toeplitz_wide_target_objective_stop():
+    """Target objectives on the wide engine's Toeplitz steps: run 600
+    iterations, then the same problems with those objectives as targets.  The
+    steps compare the exact operator's objective with the target, which is
+    the 12-tap preset's objective (the RESULT pass's): the two differ by about
+    1e-8 (relative) near a solution, so each problem stops within a few
+    iterations of 600, at its target to within 1e-7."""
+    P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
+                eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, maxiter=600, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
+    print("case toeplitz wide engine target objective: 600 iterations, then the same objectives as targets",
+          flush=True)
+    _, r0, i0 = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM)
+    assert i0["loop_used"] == sa.LOOP_TOEPLITZ
+    tprobs = [dict(p, maxiter=20000, target_objective=r["finalcost"]) for p, r in zip(probs, r0)]
+    _, r1, info = sa.mfista_imaging_core_nufft_batch(tprobs, preset=sa.PRESET_CXX12)
+    assert info["engine_used"] == sa.ENGINE_STREAM and info["loop_used"] == sa.LOOP_TOEPLITZ
+    for a, b in zip(r0, r1):
+        print(f"  target {a['finalcost']:.10e}: ITER {b['ITER']} objective {b['finalcost']:.10e}", flush=True)
+        assert abs(b["ITER"] - 600) <= 20
+        assert b["finalcost"] <= a["finalcost"] * (1 + 1e-7)

@@ -159,16 +159,19 @@ def test_wide_target_objective_stop():
     objective is at or below it: run 600 iterations, then the same problems
     with that objective as the target (no stopping rule, maxiter 20000); each
     stops within a few iterations of 600 at or below its target.  A batch with
-    targets runs on the wide engine even when the persistent one is asked for."""
+    targets runs on the wide engine even when the persistent one is asked for.
+    (Gridding steps: their objective is the preset's own, as the RESULT
+    pass's; the Toeplitz steps' target test is in test_gpu_toeplitz.py.)"""
     P = synth.make_nufft_problem(N=64, M=2000, seed=1234, amax=0.35)
     base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=64, NY=64,
                 eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
     probs = [dict(base, maxiter=600, lambda_tsv=t) for t in (1.0, 10.0, 100.0)]
     print("case wide engine target objective: 600 iterations, then the same objectives as targets", flush=True)
-    _, r0, _ = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM)
+    _, r0, _ = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM,
+                                                  loop=sa.LOOP_GRIDDING)
     tprobs = [dict(p, maxiter=20000, target_objective=r["finalcost"]) for p, r in zip(probs, r0)]
-    _, r1, info = sa.mfista_imaging_core_nufft_batch(tprobs, preset=sa.PRESET_CXX12)
-    assert info["engine_used"] == sa.ENGINE_STREAM
+    _, r1, info = sa.mfista_imaging_core_nufft_batch(tprobs, preset=sa.PRESET_CXX12, loop=sa.LOOP_GRIDDING)
+    assert info["engine_used"] == sa.ENGINE_STREAM and info["loop_used"] == sa.LOOP_GRIDDING
     for a, b in zip(r0, r1):
         print(f"  target {a['finalcost']:.10e}: ITER {b['ITER']} objective {b['finalcost']:.10e}", flush=True)
         assert abs(b["ITER"] - 600) <= 5
