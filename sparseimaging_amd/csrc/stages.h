@@ -17,7 +17,18 @@ namespace mf {
 // input of pixel (i, j).  The rows are placed at b = (j - shift) mod Ly,
 // transformed, split into the two real rows' half spectra and written
 // transposed: T[ky][i], ky in [0, Ly/2].
-template <typename R, typename Load>
+// a complex value stored non-temporally (the line is not kept in the
+// writer's L2: less for a grid barrier's release to write back)
+template <typename R>
+__device__ __forceinline__ void nt_store(cplx<R>* p, const cplx<R>& v) {
+  typedef R v2 __attribute__((ext_vector_type(2)));
+  v2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+}
+
+template <typename R, typename Load, bool NTS = false>
 __device__ inline void st_rowfft_fwd(int task, int nseq, int NX, int NY, int Ly, int shift,
                                      const FFTPlan& P, const cplx<R>* __restrict__ tw,
                                      cplx<R>* __restrict__ T, cplx<R>* sm, Load load) {
@@ -51,7 +62,10 @@ __device__ inline void st_rowfft_fwd(int task, int nseq, int NX, int NY, int Ly,
       o = mk<R>((R)0.5 * (z.x + zm.x), (R)0.5 * (z.y - zm.y));
     else
       o = mk<R>((R)0.5 * (z.y + zm.y), (R)-0.5 * (z.x - zm.x));
-    T[(size_t)k * NX + i] = o;
+    if constexpr (NTS)
+      nt_store<R>(T + (size_t)k * NX + i, o);
+    else
+      T[(size_t)k * NX + i] = o;
   }
   __syncthreads();
 }

@@ -382,7 +382,10 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
           l1 += fabs(xv);
           return dv;
         };
-        st_rowfft_fwd<double>(rt, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd, k ? GP(A.T64b) : GP(A.T64), sm64, load);
+        // (non-temporal stores of the row transforms: measured T1 barrier
+        // tail 5.0 -> 3.6 us, the release has fewer dirty lines to write back)
+        st_rowfft_fwd<double, decltype(load), true>(rt, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd,
+                                                    k ? GP(A.T64b) : GP(A.T64), sm64, load);
         if (k) {  // (no runtime index into v: it stays in registers)
           v[2] += d2;
           v[3] += l1;
