@@ -476,6 +476,8 @@ static int toeplitz_prepare(const NufftState& N, const ToepOut& op, PersistWork&
   A.u64b = (double2*)((char*)W.u64.p + t64);
   A.wv2 = op.wv2;
   for (int q = 0; q < 4; q++) A.xb[q] = (double*)W.xrot.p + (size_t)q * n;
+  A.Pxh = op.Pxh;
+  A.twxh = (const double2*)op.twxh64.p;
   return 0;
 }
 

@@ -98,6 +98,8 @@ struct PArgs {
   double2 *u64a, *u64b;
   double wv2;
   double* xb[4];  // image buffers whose roles rotate (BlockState::xr)
+  FFTPlan Pxh;           // the NX-point plan of the pruned column transforms
+  const double2* twxh;   // and its twiddles
 };
 
 // Grid-wide barriers for a cooperative launch.  bar[] layout (64-bit words):

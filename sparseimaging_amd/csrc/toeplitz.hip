@@ -18,6 +18,8 @@
 
 namespace mf {
 
+int upload_twiddles_pub(const FFTPlan& P, DevMem& tf, DevMem& td, hipStream_t st);  // solver.hip
+
 // the two right-hand sides in the set-up transform's visibility frame
 __global__ void k_toep_rhs(VisParams<double> vp, int M, double2* __restrict__ rb, double2* __restrict__ rp) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,6 +101,9 @@ int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const doubl
                      (const double2*)T.fbuf.p, (int)nspec, 1.0 / ((double)Lx * Ly), (double*)out.kh.p);
   MF_CHECK(hipGetLastError());
   out.wv2 = wv2;
+  // the NX-point plan and twiddles of the loop's pruned column transforms
+  if (!make_fft_plan(NX, &out.Pxh)) return MFISTA_E_SIZE;
+  if (upload_twiddles_pub(out.Pxh, out.twxh32, out.twxh64, st)) return MFISTA_E_HIP;
   return 0;
 }
 

@@ -92,15 +92,25 @@ __device__ inline void st_colconv(int ky, int nseq, const cplx<R>* __restrict__ 
 // imaginary parts (stored with zero imaginary parts, which the c2r pass
 // ignores at ky = 0 and NY).  Tasks ky = 1 .. NY-1 are ordinary columns, so a
 // transform takes NY tasks instead of NY + 1.
+//
+// The Lx = 2N-point transforms are pruned.  The N image rows occupy positions
+// a = (i - N/2) mod 2N, one in each residue class mod N, so
+//   X(2m) = FFT_N(y)(m),  X(2m+1) = FFT_N(y W_2N^a)(m),  y(a mod N) = x(a),
+// two N-point transforms (one LDS stage fewer than one 2N-point transform);
+// and the N rows wanted back are e(r) + conj(W_2N^a) o(r), r = a mod N, with
+// e and o the N-point backward transforms of the even and odd halves of the
+// product (the last radix-2 stage folded into the store).  Ph / twh: the
+// N-point plan and twiddles; tw: the 2N-point twiddles.
 template <typename R>
 __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const R* __restrict__ Kh,
-                                   cplx<R>* __restrict__ U, int NX, int nky, int Lx, int shift, const FFTPlan& P,
-                                   const cplx<R>* __restrict__ tw, cplx<R>* sm) {
+                                   cplx<R>* __restrict__ U, int NX, int nky, int Lx, int shift, const FFTPlan& Ph,
+                                   const cplx<R>* __restrict__ twh, const cplx<R>* __restrict__ tw, cplx<R>* sm) {
   using C = cplx<R>;
   constexpr int kPre = 4;  // kernel values a thread prefetches (Lx <= kPre * blockDim)
+  const int N = Lx / 2, Nh = N / 2;
   const int kyN = nky - 1;
   const bool packed = (ky == 0);
-  C* b0 = sm;
+  C* b0 = sm;          // [even | odd] half sequences, N each
   C* b1 = sm + Lx;
   const R* kr = Kh + (size_t)ky * Lx;
   const R* kn = Kh + (size_t)kyN * Lx;
@@ -111,23 +121,28 @@ __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const 
     kv[m] = (k < Lx) ? kr[k] : (R)0;
     kw[m] = (packed && k < Lx) ? kn[k] : (R)0;
   }
-  for (int a = threadIdx.x; a < Lx; a += blockDim.x) {
-    int i = a + shift;
-    if (i >= Lx) i -= Lx;
+  // residue r of position a: a = r (r < N/2) or r + N; image row i = (r + N/2) mod N
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    int i = r + Nh;
+    if (i >= N) i -= N;
+    const int a = (r < Nh) ? r : r + N;
     C v = mk<R>(0, 0);
     if (i < NX) v = packed ? mk<R>(T[i].x, T[(size_t)kyN * NX + i].x) : T[(size_t)ky * NX + i];
-    b0[a] = v;
+    b0[r] = v;
+    b0[N + r] = cmul(v, tw[a]);
   }
+  (void)shift;  // the placement above is the shift NX / 2 of st_colfft_fwd, with Lx = 2 NX
   __syncthreads();
-  C* res = lds_fft<R>(b0, b1, 1, P, tw, -1);
+  C* res = lds_fft<R>(b0, b1, 2, Ph, twh, -1);
   C* oth = (res == b0) ? b1 : b0;
+  auto at = [&](int k) -> int { return (k & 1) ? N + (k >> 1) : (k >> 1); };  // X(k) in res
   if (!packed) {
 #pragma unroll
     for (int m = 0; m < kPre; m++) {
       const int k = threadIdx.x + m * blockDim.x;
       if (k < Lx) {
-        const C v = res[k];
-        res[k] = mk<R>(v.x * kv[m], v.y * kv[m]);
+        const C v = res[at(k)];
+        res[at(k)] = mk<R>(v.x * kv[m], v.y * kv[m]);
       }
     }
   } else {
@@ -135,9 +150,9 @@ __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const 
     for (int m = 0; m < kPre; m++) {
       const int k = threadIdx.x + m * blockDim.x;
       if (k < Lx) {
-        const C w = res[k], wm = res[k == 0 ? 0 : Lx - k];  // W(k), W(-k)
+        const C w = res[at(k)], wm = res[at(k == 0 ? 0 : Lx - k)];  // W(k), W(-k)
         const R sp = (R)0.5 * (kv[m] + kw[m]), sn = (R)0.5 * (kv[m] - kw[m]);
-        oth[k] = mk<R>(w.x * sp + wm.x * sn, w.y * sp - wm.y * sn);
+        oth[at(k)] = mk<R>(w.x * sp + wm.x * sn, w.y * sp - wm.y * sn);
       }
     }
     C* t = res;  // the product is in oth: transform it, res is the scratch
@@ -145,11 +160,13 @@ __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const 
     oth = t;
   }
   __syncthreads();
-  res = lds_fft<R>(res, oth, 1, P, tw, +1);
+  res = lds_fft<R>(res, oth, 2, Ph, twh, +1);
   for (int i = threadIdx.x; i < NX; i += blockDim.x) {
-    int a = i - shift;
-    if (a < 0) a += Lx;
-    const C v = res[a];
+    int r = i + Nh;
+    if (r >= N) r -= N;
+    const int a = (r < Nh) ? r : r + N;
+    const C o = res[N + r], wa = tw[a];
+    const C v = cadd(res[r], cmul(o, mk<R>(wa.x, -wa.y)));
     if (!packed) {
       U[(size_t)i * nky + ky] = v;
     } else {
@@ -160,8 +177,8 @@ __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const 
   __syncthreads();
 }
 
-// LDS bytes of the FP64 twiddle tables of both axes
-__host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) { return (size_t)(Lx + Ly) * 16; }
+// LDS bytes of the FP64 twiddle tables: both axes and the pruned column transforms' NX-point table
+__host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) { return (size_t)(Lx + Ly + Lx / 2) * 16; }
 
 // The solve by workgroups lb = 0 .. nb-1 of a cooperative launch.  TAB: the
 // twiddle tables are staged in LDS.
@@ -190,14 +207,17 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
   }
   __syncthreads();
   char* lds = reinterpret_cast<char*>(smem_d4);
-  const double2 *twxd = GP(A.twxd), *twyd = GP(A.twyd);
+  const double2 *twxd = GP(A.twxd), *twyd = GP(A.twyd), *twxh = GP(A.twxh);
   if constexpr (TAB) {
     double2* t64x = reinterpret_cast<double2*>(lds);
     double2* t64y = t64x + g.Lx;
+    double2* t64h = t64y + g.Ly;
     for (int i = threadIdx.x; i < g.Lx; i += blockDim.x) t64x[i] = GP(A.twxd)[i];
     for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) t64y[i] = GP(A.twyd)[i];
+    for (int i = threadIdx.x; i < g.Lx / 2; i += blockDim.x) t64h[i] = GP(A.twxh)[i];
     twxd = t64x;
     twyd = t64y;
+    twxh = t64h;
     lds += (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255;
     __syncthreads();
   }
@@ -408,7 +428,7 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
     for (int t = lb; t < 2 * NY; t += nb) {
       const int k = t & 1;
       st_colconv1<double>(t >> 1, k ? GP(A.T64b) : GP(A.T64), kh, k ? GP(A.u64b) : GP(A.u64a), NX, nky, g.Lx,
-                          NX / 2, A.Px, twxd, sm64);
+                          NX / 2, A.Pxh, twxh, twxd, sm64);
     }
     if (!grid_sync(B, epoch, 9)) return;
     // T3: c2r rows of both trials: T d, T xnew = T z + T d, chi^2(xnew) - |W V|^2

@@ -79,6 +79,8 @@ struct WShared {
   float adj_scale;  // 1 / (8 NX NY)
   int* ndone;       // problems finished
   int toep;         // Toeplitz steps (no E4 weighting in the row pass)
+  FFTPlan Pxh;      // Toeplitz steps: the NX-point plan and twiddles of the pruned column transforms
+  const double2* twxh64;
 };
 
 // which transform a forward kernel computes: of the start (no gate) or of a
@@ -426,8 +428,8 @@ __global__ void __launch_bounds__(MF_BLOCK) kwt_colconv(const WProb* __restrict_
   const WProb& p = W[blockIdx.y];
   if (w_closed(p.S, which)) return;
   extern __shared__ double4 smem_d4[];
-  st_colconv1<double>(blockIdx.x, p.T64, p.kh, p.Gf64, sh.NX, sh.NY + 1, sh.Lx, sh.NX / 2, sh.Px, sh.twx64,
-                      reinterpret_cast<double2*>(smem_d4));
+  st_colconv1<double>(blockIdx.x, p.T64, p.kh, p.Gf64, sh.NX, sh.NY + 1, sh.Lx, sh.NX / 2, sh.Pxh, sh.twxh64,
+                      sh.twx64, reinterpret_cast<double2*>(smem_d4));
 }
 
 // c2r of the column pass's output, two image rows per task:
@@ -583,6 +585,10 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   sh.nseq_c2r32 = nseq_for(g0.Ly, sizeof(float2), (NX + 1) / 2);
   sh.adj_scale = (float)(1.0 / (8.0 * (double)NX * NY));
   sh.toep = toep ? 1 : 0;
+  if (toep) {  // every problem has NX = NY = the batch's size: the first set's plan serves all
+    sh.Pxh = (*ops)[0]->Pxh;
+    sh.twxh64 = (const double2*)(*ops)[0]->twxh64.p;
+  }
   // per-problem workspaces and descriptors
   if ((int)C.wwork.size() < B) C.wwork.resize(B);
   std::vector<WProb> hw(B);
