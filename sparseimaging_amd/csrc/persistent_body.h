@@ -98,7 +98,7 @@ struct PArgs {
   double2 *u64a, *u64b;
   double wv2;
   double* xb[4];  // image buffers whose roles rotate (BlockState::xr)
-  FFTPlan Pxh;           // the NX-point plan of the pruned column transforms
+  FFTPlan Pxh;           // the NX-point plan of the pruned row and column transforms
   const double2* twxh;   // and its twiddles
 };
 

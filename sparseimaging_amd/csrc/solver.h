@@ -75,7 +75,7 @@ struct PersistWork {
 struct ToepOut {
   DevMem kh, bimg;  // the kernel's spectrum Kh, b = Re A^H W^2 V
   double wv2 = 0;   // |W V|^2
-  FFTPlan Pxh;                 // the NX-point plan of the pruned column transforms
+  FFTPlan Pxh;                 // the NX-point plan of the pruned row and column transforms
   DevMem twxh32, twxh64;       // and its twiddles
 };
 // The Toeplitz loop's set-up scratch and the single solve's operator data.

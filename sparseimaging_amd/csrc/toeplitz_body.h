@@ -177,6 +177,90 @@ __device__ inline void st_colconv1(int ky, const cplx<R>* __restrict__ T, const 
   __syncthreads();
 }
 
+// The row passes of the loop, pruned like st_colconv1's transforms (square
+// images, so the NX-point plan Ph / twiddles twh serve the rows too; tw: the
+// Ly-point twiddles).  Pixel j of a row sits at b = (j - NY/2) mod Ly, one per
+// residue class r = b mod NY, with j = (r + NY/2) mod NY.
+__device__ __forceinline__ int toep_pos(int r, int Nh) { return (r < Nh) ? r : r + 2 * Nh; }
+
+// st_rowfft_fwd with nseq = 1 (rows 2 task, 2 task + 1 packed as one complex
+// sequence): the even and odd outputs X(2m), X(2m+1) of the Ly-point transform
+// are the NY-point transforms of y and y W_Ly^b.  Each pixel is loaded once.
+template <typename R, typename Load, bool NTS>
+__device__ inline void st_rowfft_fwd_h(int task, int NX, int NY, const FFTPlan& Ph, const cplx<R>* __restrict__ twh,
+                                       const cplx<R>* __restrict__ tw, cplx<R>* __restrict__ T, cplx<R>* sm,
+                                       Load load) {
+  using C = cplx<R>;
+  const int N = NY, Nh = N / 2;
+  C* b0 = sm;  // [even | odd], N each
+  C* b1 = sm + 2 * N;
+  const int i0 = 2 * task, i1 = i0 + 1;
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    int j = r + Nh;
+    if (j >= N) j -= N;
+    R re = 0, im = 0;
+    if (i0 < NX) re = load(i0, j);
+    if (i1 < NX) im = load(i1, j);
+    const C v = mk<R>(re, im);
+    b0[r] = v;
+    b0[N + r] = cmul(v, tw[toep_pos(r, Nh)]);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 2, Ph, twh, -1);
+  auto at = [&](int k) -> int { return (k & 1) ? N + (k >> 1) : (k >> 1); };  // X(k) in res
+  // half spectra of the two rows, T[k][i], k in [0, NY]
+  for (int k = threadIdx.x; k <= N; k += blockDim.x) {
+    const C z = res[at(k)], zm = res[at(k == 0 ? 0 : 2 * N - k)];
+    const C o0 = mk<R>((R)0.5 * (z.x + zm.x), (R)0.5 * (z.y - zm.y));
+    const C o1 = mk<R>((R)0.5 * (z.y + zm.y), (R)-0.5 * (z.x - zm.x));
+    cplx<R>* out = T + (size_t)k * NX + i0;
+    if constexpr (NTS) {
+      nt_store<R>(out, o0);
+      if (i1 < NX) nt_store<R>(out + 1, o1);
+    } else {
+      out[0] = o0;
+      if (i1 < NX) out[1] = o1;
+    }
+  }
+  __syncthreads();
+}
+
+// st_rowfft_c2r with nseq = 1: the Hermitian-completed spectrum of rows
+// 2 task, 2 task + 1 (packed as A + i B) is split into its even and odd
+// entries, both go through the NY-point backward transform, and pixel j is
+// e(r) + conj(W_Ly^b) o(r).  store(i, j, v) as in st_rowfft_c2r.
+template <typename R, typename Store>
+__device__ inline void st_rowfft_c2r_h(int task, const cplx<R>* __restrict__ U, int NX, int NY, const FFTPlan& Ph,
+                                       const cplx<R>* __restrict__ twh, const cplx<R>* __restrict__ tw, cplx<R>* sm,
+                                       Store store) {
+  using C = cplx<R>;
+  const int N = NY, Nh = N / 2, Ly = 2 * N, nky = N + 1;
+  C* b0 = sm;
+  C* b1 = sm + Ly;
+  const int i0 = 2 * task, i1 = i0 + 1;
+  for (int b = threadIdx.x; b < Ly; b += blockDim.x) {
+    const int k = (b <= N) ? b : Ly - b;
+    C A = (i0 < NX) ? U[(size_t)i0 * nky + k] : mk<R>(0, 0);
+    C B = (i1 < NX) ? U[(size_t)i1 * nky + k] : mk<R>(0, 0);
+    if (k == 0 || k == N) {
+      A.y = 0;
+      B.y = 0;
+    }
+    b0[(b & 1) ? N + (b >> 1) : (b >> 1)] = (b <= N) ? mk<R>(A.x - B.y, A.y + B.x) : mk<R>(A.x + B.y, -A.y + B.x);
+  }
+  __syncthreads();
+  C* res = lds_fft<R>(b0, b1, 2, Ph, twh, +1);
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    int r = j + Nh;
+    if (r >= N) r -= N;
+    const C o = res[N + r], wa = tw[toep_pos(r, Nh)];
+    const C z = cadd(res[r], cmul(o, mk<R>(wa.x, -wa.y)));
+    if (i0 < NX) store(i0, j, z.x);
+    if (i1 < NX) store(i1, j, z.y);
+  }
+  __syncthreads();
+}
+
 // LDS bytes of the FP64 twiddle tables: both axes and the pruned column transforms' NX-point table
 __host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) { return (size_t)(Lx + Ly + Lx / 2) * 16; }
 
@@ -404,8 +488,8 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
         };
         // (non-temporal stores of the row transforms: measured T1 barrier
         // tail 5.0 -> 3.6 us, the release has fewer dirty lines to write back)
-        st_rowfft_fwd<double, decltype(load), true>(rt, 1, NX, NY, g.Ly, NY / 2, A.Py, twyd,
-                                                    k ? GP(A.T64b) : GP(A.T64), sm64, load);
+        st_rowfft_fwd_h<double, decltype(load), true>(rt, NX, NY, A.Pxh, twxh, twyd, k ? GP(A.T64b) : GP(A.T64),
+                                                      sm64, load);
         if (k) {  // (no runtime index into v: it stays in registers)
           v[2] += d2;
           v[3] += l1;
@@ -471,7 +555,7 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
           lip += (xv - zs[f]) * td;
           chi += xv * (tn - 2.0 * bv[f]);
         };
-        st_rowfft_c2r<double>(rt, 1, k ? GP(A.u64b) : GP(A.u64a), NX, NY, g.Ly, NY / 2, A.Py, twyd, sm64, store);
+        st_rowfft_c2r_h<double>(rt, k ? GP(A.u64b) : GP(A.u64a), NX, NY, A.Pxh, twxh, twyd, sm64, store);
         if (k) {  // (no runtime index into v: it stays in registers)
           v[2] += chi;
           v[3] += lip;

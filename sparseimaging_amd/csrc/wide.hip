@@ -78,8 +78,7 @@ struct WShared {
   int nseq_r64, nseq_c64, nseq_ci32, nseq_c2r32;
   float adj_scale;  // 1 / (8 NX NY)
   int* ndone;       // problems finished
-  int toep;         // Toeplitz steps (no E4 weighting in the row pass)
-  FFTPlan Pxh;      // Toeplitz steps: the NX-point plan and twiddles of the pruned column transforms
+  FFTPlan Pxh;      // Toeplitz steps: the NX-point plan and twiddles of the pruned transforms
   const double2* twxh64;
 };
 
@@ -97,11 +96,7 @@ __global__ void __launch_bounds__(MF_BLOCK) kw_rowfft64(const WProb* __restrict_
   extern __shared__ double4 smem_d4[];
   const double* x = (which == kWInit) ? p.x : p.d;
   const int NY = sh.NY;
-  const bool e4 = !sh.toep;
-  auto load = [&](int i, int j) -> double {
-    const double v = x[(size_t)i * NY + j];
-    return e4 ? v * sh.e4x64[i] * sh.e4y64[j] : v;
-  };
+  auto load = [&](int i, int j) -> double { return x[(size_t)i * NY + j] * sh.e4x64[i] * sh.e4y64[j]; };
   st_rowfft_fwd<double>(blockIdx.x, sh.nseq_r64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64, p.T64,
                         reinterpret_cast<double2*>(smem_d4), load);
 }
@@ -422,6 +417,19 @@ __global__ void __launch_bounds__(MF_BLOCK) kwt_grad(const WProb* __restrict__ W
   if (threadIdx.x == 0) p.part[p.L.chi_z + blockIdx.x] = v[0];
 }
 
+// row pass of the Toeplitz product of the start (init) or of a trial's step,
+// two image rows per task (the pruned transform of toeplitz_body.h)
+__global__ void __launch_bounds__(MF_BLOCK) kwt_rowfft(const WProb* __restrict__ W, WShared sh, int which) {
+  const WProb& p = W[blockIdx.y];
+  if (w_closed(p.S, which)) return;
+  extern __shared__ double4 smem_d4[];
+  const double* x = (which == kWInit) ? p.x : p.d;
+  const int NY = sh.NY;
+  auto load = [&](int i, int j) -> double { return x[(size_t)i * NY + j]; };
+  st_rowfft_fwd_h<double, decltype(load), false>(blockIdx.x, sh.NX, NY, sh.Pxh, sh.twxh64, sh.twy64, p.T64,
+                                                 reinterpret_cast<double2*>(smem_d4), load);
+}
+
 // column pass of the Toeplitz product of the start (init) or of a trial's
 // step: task ky < NY (row 0 carrying the Nyquist row), the output in Gf64
 __global__ void __launch_bounds__(MF_BLOCK) kwt_colconv(const WProb* __restrict__ W, WShared sh, int which) {
@@ -457,8 +465,8 @@ __global__ void __launch_bounds__(MF_BLOCK) kwt_c2r(const WProb* __restrict__ W,
       v[1] += p.d[f] * td;
     }
   };
-  st_rowfft_c2r<double>(blockIdx.x, 1, p.Gf64, sh.NX, NY, sh.Ly, NY / 2, sh.Py, sh.twy64,
-                        reinterpret_cast<double2*>(smem_d4), store);
+  st_rowfft_c2r_h<double>(blockIdx.x, p.Gf64, sh.NX, NY, sh.Pxh, sh.twxh64, sh.twy64,
+                          reinterpret_cast<double2*>(smem_d4), store);
   block_sum<2>(v, red);
   if (threadIdx.x == 0) {
     p.part[p.L.chi_x + blockIdx.x] = v[0];
@@ -536,7 +544,7 @@ static void launch_step_toep(const WLaunch& L, const WProb* W, const WShared& sh
   hipLaunchKernelGGL(kwt_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   hipLaunchKernelGGL(kw_tsv_grad, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
   hipLaunchKernelGGL(kw_prox, L.elem, dim3(MF_BLOCK), 0, st, W, sh);
-  hipLaunchKernelGGL(kw_rowfft64, L.rowf, dim3(MF_BLOCK), L.sh_rowf, st, W, sh, (int)kWTrial);
+  hipLaunchKernelGGL(kwt_rowfft, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWTrial);
   hipLaunchKernelGGL(kwt_colconv, L.conv, dim3(MF_BLOCK), L.sh_conv, st, W, sh, (int)kWTrial);
   hipLaunchKernelGGL(kwt_c2r, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWTrial);
   hipLaunchKernelGGL(kw_control, L.one, dim3(MF_BLOCK), 0, st, W, sh.ndone);
@@ -584,7 +592,6 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   sh.nseq_ci32 = nseq_for(g0.Lx, sizeof(float2), nky);
   sh.nseq_c2r32 = nseq_for(g0.Ly, sizeof(float2), (NX + 1) / 2);
   sh.adj_scale = (float)(1.0 / (8.0 * (double)NX * NY));
-  sh.toep = toep ? 1 : 0;
   if (toep) {  // every problem has NX = NY = the batch's size: the first set's plan serves all
     sh.Pxh = (*ops)[0]->Pxh;
     sh.twxh64 = (const double2*)(*ops)[0]->twxh64.p;
@@ -729,7 +736,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   // start: the model of xinit, its cost (chi^2, |x|_1, TSV), z = x in both
   // domains, and the initial control state
   if (toep) {  // T x0 (= T z), its chi^2 partials
-    hipLaunchKernelGGL(kw_rowfft64, L.rowf, dim3(MF_BLOCK), L.sh_rowf, st, W, sh, (int)kWInit);
+    hipLaunchKernelGGL(kwt_rowfft, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWInit);
     hipLaunchKernelGGL(kwt_colconv, L.conv, dim3(MF_BLOCK), L.sh_conv, st, W, sh, (int)kWInit);
     hipLaunchKernelGGL(kwt_c2r, L.c2r64, dim3(MF_BLOCK), L.sh_c2r64, st, W, sh, (int)kWInit);
   } else {
