@@ -523,6 +523,8 @@ def main():
     ap.add_argument("--loop", choices=["auto", "gridding"], default="auto",
                     help="single solves on the persistent engine: the Toeplitz loop where it applies (auto) or "
                          "the gridding loop")
+    ap.add_argument("--compare-loops", type=int, choices=[0, 1], default=1,
+                    help="also time the same solve on the gridding loop (when the Toeplitz loop ran)")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--warm", type=int, default=32,
@@ -593,6 +595,25 @@ def main():
     rounds = info["trial_rounds"]
     log(f"bench: ITER={ITER} trials={trials} rounds={rounds} finalcost={res['finalcost']:.10e} "
         f"ms/solve={ms_per_step:.2f} iters/s={ITER / (ms_per_step / 1e3):.1f}")
+
+    # the same solve on the FP32 gridding loop (what the Toeplitz loop replaces
+    # for single solves), timed the same way, for comparison
+    grid_loop = None
+    if args.compare_loops and args.engine == "persistent" and info.get("loop_used") == sa.LOOP_TOEPLITZ:
+        cfg_g = dict(cfg, loop=sa.LOOP_GRIDDING)
+        xgl = torch.zeros_like(xo)  # (xo keeps the timed solve's image for the parity check)
+        sa.solve_nufft_device(t["u"], t["v"], t["vis_r"], t["vis_i"], t["std"], N, N, x0, xgl, **cfg_g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rg, _ = sa.solve_nufft_device(t["u"], t["v"], t["vis_r"], t["vis_i"], t["std"], N, N, x0, xgl, **cfg_g)
+        torch.cuda.synchronize()
+        dtg = time.perf_counter() - t0
+        grid_loop = {"solves_per_s": round(args.steps / dtg, 4), "ms_per_step": round(dtg / args.steps * 1e3, 3),
+                     "iterations": rg["ITER"], "objective": rg["finalcost"], "dtype": "f32/f64",
+                     "toeplitz_speedup": round(value / (args.steps / dtg), 3)}
+        log("bench: gridding loop " + json.dumps(grid_loop))
+        del xgl
 
     # per-kernel durations of the stream engine's kernels (same problem, HIP events)
     prof = sa.profile_stages(args.profile_reps)
@@ -668,6 +689,8 @@ def main():
                   "objective_rel": abs(res["finalcost"] - Fr) / Fr,
                   "objective_gpu": res["finalcost"], "objective_oracle": Fr, "iterations_oracle": int(ref["ITER_rule"]),
                   "gap_to_converged_objective": (res["finalcost"] - Fc) / Fc}
+        if grid_loop is not None:
+            grid_loop["objective_rel_vs_oracle"] = abs(grid_loop["objective"] - Fr) / Fr
         if same_iter is not None:
             # the rule can stop the two solvers a few iterations apart while the
             # iterate still moves ~3e-4 per iteration; at equal counts the
@@ -691,7 +714,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64",  # FP32 adjoint and gridding; FP64 step transform, iterates, reductions
+        # Toeplitz loop: FP64 throughout (FFT convolution, iterates, reductions);
+        # gridding loop: FP32 adjoint and gridding, FP64 step transform, iterates, reductions
+        "dtype": "f64" if info.get("loop_used") == sa.LOOP_TOEPLITZ else "f32/f64",
         "data": "synthetic (seeded ring + 3 Gaussians, 12 elliptical uv arcs)",
         "config": {"workload": "nufft_l1tsv_256_m20000_12tap", "N": N, "M": M, "taps": 12,
                    "lambda_l1": 1.0, "lambda_tsv": 100.0, "rel_tol": 1e-6, "start": "cold",
@@ -700,6 +725,7 @@ def main():
                        info.get("loop_used"), "?")},
         "iterations_per_s": round(ITER / (ms_per_step / 1e3), 2),
         "roofline": roofline,
+        "gridding_loop": grid_loop,
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": batch or None,

@@ -449,6 +449,28 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
         const int r0 = 2 * rt - 1;
         const double cik = k ? ci1 : ci0, ek = k ? e1 : e0;
         double* xk_out = k ? xb_out : xa_out;
+        // T z and b at this thread's pixels of the row transform's load (r =
+        // threadIdx.x + m blockDim, pixel j = (r + NY/2) mod NY, rows 2 rt + q),
+        // read before the staging barrier so that their latency overlaps the
+        // staging loads (NY <= 2 blockDim)
+        double pt[2][2] = {{0, 0}, {0, 0}}, pb[2][2] = {{0, 0}, {0, 0}};
+#pragma unroll
+        for (int m = 0; m < 2; m++) {
+          const int r = threadIdx.x + m * blockDim.x;
+          if (r < NY) {
+            int j = r + NY / 2;
+            if (j >= NY) j -= NY;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+              const int i = 2 * rt + q;
+              if (i < NX) {
+                const int f = i * NY + j;
+                pt[m][q] = mom ? ma * tx[f] + mb * tq[f] : tz[f];
+                pb[m][q] = bv[f];
+              }
+            }
+          }
+        }
         for (int q = threadIdx.x; q < 4 * NY; q += blockDim.x) {
           const int r = r0 + q / NY, j = q - (q / NY) * NY;
           double zv = 0.0;
@@ -464,12 +486,16 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
           const int f = i * NY + j;
           const int l = (i - r0) * NY + j;  // (i, j) in the staged rows
           const double zf = zl[l];
-          const double tzf = mom ? ma * tx[f] + mb * tq[f] : tz[f];
+          int rr = j + NY / 2;  // the load's r for pixel j
+          if (rr >= NY) rr -= NY;
+          const bool m1 = rr >= (int)blockDim.x, q1 = i != 2 * rt;
+          const double tzf = m1 ? (q1 ? pt[1][1] : pt[1][0]) : (q1 ? pt[0][1] : pt[0][0]);
+          const double bf = m1 ? (q1 ? pb[1][1] : pb[1][0]) : (q1 ? pb[0][1] : pb[0][0]);
           if (mom && !k) {  // the trial-0 workgroup of the pair stores z and T z
             zs[f] = zf;
             tz[f] = tzf;
           }
-          double gf = bv[f] - tzf;
+          double gf = bf - tzf;
           if (tsv) {  // - lambda_tsv grad TSV(z); square images: column j, row i
             double d = 0.0;
             if (j < W - 1) d += zf - zl[l + 1];
