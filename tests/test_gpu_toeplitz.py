@@ -70,6 +70,46 @@ def test_toeplitz_and_gridding_loops_at_fixed_iterations(preset):
     assert lg <= 1e-3 and dg <= 1e-6
 
 
+@pytest.mark.parametrize("N", [384, 512])
+def test_toeplitz_large_images_match_gridding_loop(N):
+    """Images wider than a workgroup (each thread loads two pixels of a row
+    and forms two entries of a column in the pruned transforms; 384 = 3 x 128
+    also takes the radix-3 stage): the Toeplitz loop, the persistent Toeplitz
+    batch and the wide engine's Toeplitz steps against the gridding loop (whose
+    stages do not share that code) for the same 4000 iterations, to the parity
+    bars.  Both loops follow the oracle to those bars at 96^2 above; an oracle
+    solve at this size takes minutes.  At this M the gridding loop is the
+    faster one beyond 320^2, so LOOP_AUTO picks it and the Toeplitz loop is
+    asked for."""
+    P = synth.make_nufft_problem(N=N, M=20000, seed=4242)
+    print(f"case toeplitz large image N={N} M=20000 4000 iterations", flush=True)
+    _, _, ia = _gpu(P, N, N, sa.PRESET_CXX12, 50, 0.0, 1.0, 100.0)
+    assert ia["loop_used"] == sa.LOOP_GRIDDING
+    xt, rt, it = _gpu(P, N, N, sa.PRESET_CXX12, 4000, 0.0, 1.0, 100.0, loop=sa.LOOP_TOEPLITZ)
+    xg, rg, ig = _gpu(P, N, N, sa.PRESET_CXX12, 4000, 0.0, 1.0, 100.0, loop=sa.LOOP_GRIDDING)
+    assert it["loop_used"] == sa.LOOP_TOEPLITZ and ig["loop_used"] == sa.LOOP_GRIDDING
+    l2 = np.linalg.norm(xt - xg) / np.linalg.norm(xg)
+    dc = abs(rt["finalcost"] - rg["finalcost"]) / rg["finalcost"]
+    print(f"  toeplitz vs gridding: relL2={l2:.2e} cost rel={dc:.2e} ({it['kernel_ms']:.1f} / "
+          f"{ig['kernel_ms']:.1f} ms)", flush=True)
+    assert rt["ITER"] == 4000 and rg["ITER"] == 4000
+    assert l2 <= 1e-3 and dc <= 1e-6
+    base = dict(u_dx=P["u"], v_dy=P["v"], vis_r=P["vis_r"], vis_i=P["vis_i"], vis_std=P["std"], NX=N, NY=N,
+                maxiter=4000, eps=0.0, cinit=1e4, nonneg_flag=1, lambda_tv=0.0, lambda_l1=1.0)
+    probs = [dict(base, lambda_tsv=t) for t in (100.0, 300.0)]
+    xp, rp, ip = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, loop=sa.LOOP_TOEPLITZ)
+    xw, rw, iw = sa.mfista_imaging_core_nufft_batch(probs, preset=sa.PRESET_CXX12, engine=sa.ENGINE_STREAM)
+    assert ip["loop_used"] == sa.LOOP_TOEPLITZ
+    assert iw["engine_used"] == sa.ENGINE_STREAM and iw["loop_used"] == sa.LOOP_TOEPLITZ
+    for name, xs, rs in (("persistent batch", xp, rp), ("wide", xw, rw)):
+        l2 = np.linalg.norm(xs[0] - xt) / np.linalg.norm(xt)
+        dc = abs(rs[0]["finalcost"] - rt["finalcost"]) / rt["finalcost"]
+        l2b = np.linalg.norm(xs[1] - xp[1]) / np.linalg.norm(xp[1])
+        print(f"  {name}: lambda_tsv 100 vs single relL2={l2:.2e} cost rel={dc:.2e}; lambda_tsv 300 vs the "
+              f"persistent batch relL2={l2b:.2e}", flush=True)
+        assert rs[0]["ITER"] == 4000 and l2 <= 1e-3 and dc <= 1e-6 and l2b <= 1e-3
+
+
 @pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
 def test_toeplitz_signed_with_box(preset):
     """nonneg_flag = 0 (the signed soft threshold, c/mfista_tools.c:81-112)
@@ -108,7 +148,8 @@ def test_loop_selection():
     """TV solves, deterministic mode, single-trial rounds and rectangular
     images keep the gridding loop (the presets size both axes' kernels from
     NX, so for NX != NY the reference's transform is not the exact one the
-    Toeplitz loop applies); LOOP_GRIDDING forces it."""
+    Toeplitz loop applies), also when LOOP_TOEPLITZ asks for it; LOOP_GRIDDING
+    forces it."""
     P = synth.make_nufft_problem(N=32, M=600, seed=5, amax=0.3)
     base = dict(maxiter=50, eps=0.0, lambda_l1=1.0, cinit=1e4, nonneg_flag=1)
     args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 32, 32)
@@ -121,8 +162,10 @@ def test_loop_selection():
                                             **base)
     _, _, i5 = sa.mfista_imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 32, 48,
                                             lambda_tsv=1.0, preset=sa.PRESET_CXX12, **base)
+    _, _, i6 = sa.mfista_imaging_core_nufft(*args, lambda_tv=1.0, preset=sa.PRESET_C24, loop=sa.LOOP_TOEPLITZ,
+                                            **base)
     assert i0["loop_used"] == sa.LOOP_TOEPLITZ
-    assert [i["loop_used"] for i in (i1, i2, i3, i4, i5)] == [sa.LOOP_GRIDDING] * 5
+    assert [i["loop_used"] for i in (i1, i2, i3, i4, i5, i6)] == [sa.LOOP_GRIDDING] * 6
 
 
 def test_toeplitz_batch_matches_single_solves_and_oracle():
