@@ -166,13 +166,11 @@ typedef struct mfista_opts {
   int loop;          /* in: NUFFT solves on the persistent engine: the Toeplitz loop applies
                         to L1 and L1+TSV square images up to 512 x 512 (not with
                         deterministic, single_trials or the allocation-free entry point).
-                        MFISTA_LOOP_AUTO (default) runs it where it applies and is the
-                        faster loop (images up to 320 x 320 at any M, larger ones from
-                        M = 32000; a tuned shape runs its tuned loop), the gridding loop
-                        otherwise; MFISTA_LOOP_TOEPLITZ runs it wherever it applies;
-                        MFISTA_LOOP_GRIDDING always runs the gridding loop.  Batches on
-                        the wide engine run Toeplitz steps wherever they apply unless
-                        MFISTA_LOOP_GRIDDING */
+                        MFISTA_LOOP_AUTO (default) runs it where it applies (a shape the
+                        autotuner timed runs its tuned loop), the gridding loop otherwise;
+                        MFISTA_LOOP_TOEPLITZ runs it wherever it applies, also on a shape
+                        tuned to the gridding loop; MFISTA_LOOP_GRIDDING always runs the
+                        gridding loop.  Batches follow the same rule */
   int loop_used;     /* out: MFISTA_LOOP_GRIDDING or MFISTA_LOOP_TOEPLITZ, the loop that ran
                         (batches: the steps of the persistent or the wide engine); 0 when a
                         single solve ran on the stream engine */

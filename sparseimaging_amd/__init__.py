@@ -27,7 +27,7 @@ PRESET_C24 = 0
 PRESET_CXX12 = 1
 ENGINE_PERSISTENT = 0  # one cooperative launch per NUFFT solve
 ENGINE_STREAM = 1      # one launch per kernel, device-side gating
-LOOP_AUTO = 0          # the Toeplitz loop where it applies and is the faster loop, else gridding
+LOOP_AUTO = 0          # the Toeplitz loop where it applies (or the tuned loop), else gridding
 LOOP_GRIDDING = 1      # grid the residual every iteration, degrid every trial (the reference's structure)
 LOOP_TOEPLITZ = 2      # the normal operator as an FFT convolution, no gridding in the loop (wherever it applies)
 MAXITER, MINITER, FGPITER, TD, ETA, EPS = 50000, 100, 100, 50, 1.1, 1.0e-5
@@ -268,9 +268,9 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     repeated solves, persistent engine); autotune=True uses the autotuner's
     launch configuration for this shape (tuned first, with tune_iters
     iterations per candidate, when the table has none); loop=LOOP_AUTO runs
-    the Toeplitz loop for L1 / L1+TSV square images up to 512^2 where it is
-    the faster loop (up to 320^2 at any M, above from M = 32000),
-    LOOP_TOEPLITZ wherever it applies, LOOP_GRIDDING never."""
+    the Toeplitz loop for L1 / L1+TSV square images up to 512^2 (a tuned
+    shape: its tuned loop), LOOP_TOEPLITZ wherever it applies, LOOP_GRIDDING
+    never."""
     u, pu = _d(u_dx)
     v, pv = _d(v_dy)
     vr, pvr = _d(vis_r)
@@ -350,9 +350,8 @@ def mfista_imaging_core_nufft_batch(problems, preset=PRESET_C24, fgp_iter=FGPITE
     the autotuner's launch configuration for the batch's shape.
     engine=ENGINE_STREAM: the wide engine (every stage one launch for all
     problems; L1 and L1+TSV of one image size).  Both run Toeplitz steps when
-    every problem is L1 / L1+TSV on a square image up to 512^2; the
-    persistent launch only where single solves' LOOP_AUTO rule picks them too
-    (or loop=LOOP_TOEPLITZ); loop=LOOP_GRIDDING: gridding steps.  Returns
+    every problem is L1 / L1+TSV on a square image up to 512^2
+    (loop=LOOP_GRIDDING: gridding steps).  Returns
     (list of x[NX,NY], list of RESULT dicts, opts dict for the whole batch)."""
     n = len(problems)
     arr, keep, outs = _pack_problems(problems)

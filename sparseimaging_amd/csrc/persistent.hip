@@ -401,8 +401,10 @@ static int toeplitz_config(Context& C, const std::vector<const NufftState*>& set
   size_t sh = 0, tables = 0;
   for (const NufftState* N : sets) {
     const GridGeom& g = N->geom;
+    // every stage's transform takes two buffers of one padded row or column
+    // (T1 stages its z rows in the second one before its first FFT stage)
     const int Lm = g.Lx > g.Ly ? g.Lx : g.Ly;
-    sh = std::max(sh, (size_t)4 * Lm * sizeof(double2));
+    sh = std::max(sh, (size_t)2 * Lm * sizeof(double2));
     tables = std::max(tables, (toep_table_bytes(g.Lx, g.Ly) + 255) & ~(size_t)255);
   }
   L->lds_tables = (tables + sh + static_lds <= (size_t)lds_max) ? 1 : 0;

@@ -474,19 +474,6 @@ static bool toeplitz_applies(const Context& C, const SolveArgs& A, const mfista_
          (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512;
 }
 
-// Whether MFISTA_LOOP_AUTO (with no tuned choice) takes the Toeplitz loop on
-// the persistent engine: where it is the faster loop.  Its iteration cost
-// depends on the image size only; the gridding loop's grows with M.  Measured
-// per iteration (MI355X, L1+TSV, 12 taps, 600 iterations, microseconds):
-//   N                           128  192  256  320  384  512
-//   Toeplitz loop                56   63   66   91  132  173
-//   gridding loop, M = 5000      75   81   83  109  112
-// plus about 0.8 us per 1000 visibilities for the gridding loop (512^2 at
-// M = 20000: 160).  Beyond 320^2 the column pass has more tasks than the
-// launch has workgroups.  So the Toeplitz loop up to 320^2 at every M, and
-// above from M = 32000 (crossovers near 29000 at 384^2 and 35000 at 512^2).
-static bool toeplitz_by_rule(int N, long M) { return N <= 320 || M >= 32000; }
-
 // Times the autotuner's candidates on this problem: `iters` iterations from
 // its xinit, no stopping rule, xout overwritten.
 static int tune_single(Context& C, const SolveArgs& A, const mfista_opts* opts, int iters, TuneChoice* best,
@@ -618,11 +605,11 @@ int solve_nufft(Context& C, const SolveArgs& A, RESULT* res, mfista_opts* opts) 
     rc = kCoopRefused;
     int used = MFISTA_ENGINE_PERSISTENT, loop_used = 0;
     if (engine == MFISTA_ENGINE_PERSISTENT && coop) {
-      // the Toeplitz loop where it applies and is asked for: by opts->loop,
-      // by the autotuner's choice for this shape, else by the measured rule
-      const bool asked = (opts && opts->loop == MFISTA_LOOP_TOEPLITZ) || C.tune_loop == MFISTA_LOOP_TOEPLITZ ||
-                         (C.tune_loop == 0 && toeplitz_by_rule(A.NX, A.M));
-      bool toep = toeplitz_applies(C, A, opts) && C.tune_loop != MFISTA_LOOP_GRIDDING && asked;
+      // the Toeplitz loop where it applies (the faster loop at every size and M
+      // measured), unless the autotuner's choice for this shape is the
+      // gridding loop and opts->loop does not ask for the Toeplitz loop
+      const bool forced = opts && opts->loop == MFISTA_LOOP_TOEPLITZ;
+      bool toep = toeplitz_applies(C, A, opts) && (forced || C.tune_loop != MFISTA_LOOP_GRIDDING);
       if (toep) {
         const int trc = toeplitz_setup(C, A.u, A.v, A.vr, A.vi, A.sd, A.M, A.NX, A.NY, preset, wv2, C.tw.out);
         if (trc == MFISTA_E_SIZE || trc == MFISTA_E_FOV)
@@ -925,17 +912,16 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   // Engines: the wide engine (every kernel launched once for all problems)
   // takes batches of L1 / L1+TSV problems of one image size; the persistent
   // engine runs the batch in cooperative launches.  Both run Toeplitz steps
-  // when every problem is L1 / L1+TSV on a square image up to 512^2, else
-  // gridding steps; the persistent engine also applies the single solves'
-  // rule of the faster loop (toeplitz_by_rule), the wide engine (any number
-  // of workgroups per pass) does not.  Target objectives are a control
+  // when every problem is L1 / L1+TSV on a square image up to 512^2 (the rule
+  // of single solves), else gridding steps.  Target objectives are a control
   // rule of the wide engine; with Toeplitz steps the objective they are
   // compared with applies the exact operator, within ~1e-8 (relative) of the
   // preset's objective that the RESULT pass reports.
   const int engine = C->tune_engine >= 0 ? C->tune_engine : (opts ? opts->engine : MFISTA_ENGINE_PERSISTENT);
   bool targets = false, wide_ok = !P.empty();
   bool toep = !P.empty() && !(opts && (opts->single_trials || opts->deterministic)) &&
-              !(opts && opts->loop == MFISTA_LOOP_GRIDDING) && C->tune_loop != MFISTA_LOOP_GRIDDING;
+              !(opts && opts->loop == MFISTA_LOOP_GRIDDING) &&
+              ((opts && opts->loop == MFISTA_LOOP_TOEPLITZ) || C->tune_loop != MFISTA_LOOP_GRIDDING);
   for (size_t j = 0; j < P.size(); j++) {
     if (P[j].target_F > 0) targets = true;
     if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide_ok = false;
@@ -946,9 +932,6 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
     return MFISTA_E_ARG;
   }
   const bool wide = wide_ok && (targets || engine == MFISTA_ENGINE_STREAM);
-  if (toep && !wide && !(opts && opts->loop == MFISTA_LOOP_TOEPLITZ) && C->tune_loop != MFISTA_LOOP_TOEPLITZ)
-    for (size_t j = 0; j < P.size(); j++)  // persistent launches: the single solves' rule
-      if (!toeplitz_by_rule(P[j].NX, pr[which[j]].M)) toep = false;
   if (toep) {  // the operator data of every visibility set a problem uses
     std::vector<char> used(nset, 0);
     for (int i : which) used[set_of[i]] = 1;

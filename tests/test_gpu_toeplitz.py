@@ -78,13 +78,12 @@ def test_toeplitz_large_images_match_gridding_loop(N):
     batch and the wide engine's Toeplitz steps against the gridding loop (whose
     stages do not share that code) for the same 8000 iterations, to the parity
     bars.  Both loops follow the oracle to those bars at 96^2 above; an oracle
-    solve at this size takes minutes.  At this M the gridding loop is the
-    faster one beyond 320^2, so LOOP_AUTO picks it and the Toeplitz loop is
-    asked for."""
+    solve at this size takes minutes.  LOOP_AUTO picks the Toeplitz loop here
+    too; LOOP_TOEPLITZ asks for it."""
     P = synth.make_nufft_problem(N=N, M=20000, seed=4242)
     print(f"case toeplitz large image N={N} M=20000 8000 iterations", flush=True)
     _, _, ia = _gpu(P, N, N, sa.PRESET_CXX12, 50, 0.0, 1.0, 100.0)
-    assert ia["loop_used"] == sa.LOOP_GRIDDING
+    assert ia["loop_used"] == sa.LOOP_TOEPLITZ
     xt, rt, it = _gpu(P, N, N, sa.PRESET_CXX12, 8000, 0.0, 1.0, 100.0, loop=sa.LOOP_TOEPLITZ)
     xg, rg, ig = _gpu(P, N, N, sa.PRESET_CXX12, 8000, 0.0, 1.0, 100.0, loop=sa.LOOP_GRIDDING)
     assert it["loop_used"] == sa.LOOP_TOEPLITZ and ig["loop_used"] == sa.LOOP_GRIDDING
