@@ -164,7 +164,7 @@ typedef struct mfista_opts {
                         first (on the problem itself) when the table has none */
   int tune_iters;    /* in: iterations per autotuner candidate (default 30) */
   int loop;          /* in: NUFFT solves on the persistent engine: the Toeplitz loop applies
-                        to L1 and L1+TSV square images up to 512 x 512 (not with
+                        to L1 and L1+TSV square images of even size up to 512 x 512 (not with
                         deterministic, single_trials or the allocation-free entry point).
                         MFISTA_LOOP_AUTO (default) runs it where it applies (a shape the
                         autotuner timed runs its tuned loop), the gridding loop otherwise;

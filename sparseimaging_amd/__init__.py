@@ -268,7 +268,7 @@ def mfista_imaging_core_nufft(u_dx, v_dy, vis_r, vis_i, vis_std, NX, NY, maxiter
     repeated solves, persistent engine); autotune=True uses the autotuner's
     launch configuration for this shape (tuned first, with tune_iters
     iterations per candidate, when the table has none); loop=LOOP_AUTO runs
-    the Toeplitz loop for L1 / L1+TSV square images up to 512^2 (a tuned
+    the Toeplitz loop for L1 / L1+TSV square images of even size up to 512^2 (a tuned
     shape: its tuned loop), LOOP_TOEPLITZ wherever it applies, LOOP_GRIDDING
     never."""
     u, pu = _d(u_dx)

@@ -467,11 +467,12 @@ struct NufftOp : Operator<double> {
 // permitting): L1 and L1+TSV, paired rounds, floating-point atomics allowed in
 // its set-up, square images up to 512^2 (the presets size both axes' kernels
 // from NX, so for NX != NY the reference's transform is not the exact one the
-// loop applies), and not the allocation-free entry point.
+// loop applies) of even size (the pruned transforms place pixel i at
+// (i - N/2) mod 2N with N/2 exact), and not the allocation-free entry point.
 static bool toeplitz_applies(const Context& C, const SolveArgs& A, const mfista_opts* opts) {
   const bool tv = A.lambda_tv != 0;
   return !tv && !C.no_alloc && !(opts && (opts->single_trials || opts->deterministic)) &&
-         (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512;
+         (!opts || opts->loop != MFISTA_LOOP_GRIDDING) && A.NX == A.NY && A.NX <= 512 && A.NX % 2 == 0;
 }
 
 // Times the autotuner's candidates on this problem: `iters` iterations from
@@ -816,7 +817,7 @@ static bool toeplitz_batch_applies(int nprob, const mfista_nufft_problem* pr, co
                                    const mfista_opts* opts) {
   if (opts && (opts->single_trials || opts->deterministic || opts->loop == MFISTA_LOOP_GRIDDING)) return false;
   for (int i = 0; i < nprob; i++)
-    if (tv[i] || pr[i].NX != pr[i].NY || pr[i].NX > 512) return false;
+    if (tv[i] || pr[i].NX != pr[i].NY || pr[i].NX > 512 || pr[i].NX % 2) return false;
   return nprob > 0;
 }
 
@@ -925,7 +926,7 @@ static int run_batch(Context* C, int nprob, const mfista_nufft_problem* pr, cons
   for (size_t j = 0; j < P.size(); j++) {
     if (P[j].target_F > 0) targets = true;
     if (P[j].tv_mode || P[j].NX != P[0].NX || P[j].NY != P[0].NY) wide_ok = false;
-    if (P[j].tv_mode || P[j].NX != P[j].NY || P[j].NX > 512) toep = false;
+    if (P[j].tv_mode || P[j].NX != P[j].NY || P[j].NX > 512 || P[j].NX % 2) toep = false;
   }
   if (targets && !wide_ok) {
     fprintf(stderr, "[mfista_amd] target objectives need L1 or L1+TSV problems of one image size\n");

@@ -63,7 +63,7 @@ int toeplitz_setup(Context& C, const double* d_u, const double* d_v, const doubl
                    const double* d_sd, int M, int NX, int NY, int preset, double wv2, ToepOut& out) {
   ToepWork& T = C.tw;
   const int L = std::max(NX, NY), L2 = 2 * L;
-  if (L2 > 1024 || !C.nu.ready || C.nu.NX != NX || C.nu.NY != NY) return MFISTA_E_SIZE;
+  if (L2 > 1024 || NX != NY || NX % 2 || !C.nu.ready || C.nu.NX != NX || C.nu.NY != NY) return MFISTA_E_SIZE;
   hipStream_t st = C.stream;
   const size_t img2 = (size_t)L2 * L2 * 8;
   if (T.rb.ensure((size_t)M * 16) || T.rp.ensure((size_t)M * 16) || T.imgp.ensure(img2) || T.imgb.ensure(img2))

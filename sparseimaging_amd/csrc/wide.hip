@@ -556,7 +556,7 @@ int solve_nufft_wide(Context& C, const std::vector<NufftState*>& sets, const std
   const int B = (int)P.size();
   if (B <= 0 || (int)sets.size() != B) return MFISTA_E_ARG;
   const bool toep = ops != nullptr;
-  if (toep && ((int)ops->size() != B || P[0].NX != P[0].NY)) return MFISTA_E_ARG;
+  if (toep && ((int)ops->size() != B || P[0].NX != P[0].NY || P[0].NX % 2)) return MFISTA_E_ARG;
   NufftState& N0 = *sets[0];
   const int NX = P[0].NX, NY = P[0].NY, n = NX * NY, MSP = N0.MSP;
   for (int b = 0; b < B; b++) {

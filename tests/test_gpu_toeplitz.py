@@ -147,8 +147,8 @@ def test_loop_selection():
     """TV solves, deterministic mode, single-trial rounds and rectangular
     images keep the gridding loop (the presets size both axes' kernels from
     NX, so for NX != NY the reference's transform is not the exact one the
-    Toeplitz loop applies), also when LOOP_TOEPLITZ asks for it; LOOP_GRIDDING
-    forces it."""
+    Toeplitz loop applies), also when LOOP_TOEPLITZ asks for the other;
+    LOOP_GRIDDING forces it."""
     P = synth.make_nufft_problem(N=32, M=600, seed=5, amax=0.3)
     base = dict(maxiter=50, eps=0.0, lambda_l1=1.0, cinit=1e4, nonneg_flag=1)
     args = (P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 32, 32)
