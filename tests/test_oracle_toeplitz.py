@@ -8,7 +8,9 @@ w^2 V.  The loop takes p and b from the 24-tap adjoint on a 2N x 2N image
 (toeplitz.hip); here the same construction in numpy, from the oracle's
 adjoint, is compared with the oracle's own A^H W^2 A x, for both presets, and
 with the exact DFT.  A wrong centring, a transposed offset or a dropped weight
-fails these by orders of magnitude.
+fails these by orders of magnitude.  The last tests check the even/odd split
+the loop's pruned 2N-point transforms rest on against the plain padded
+convolution (numpy FFTs only).
 """
 import numpy as np
 import pytest
@@ -98,3 +100,64 @@ def test_rectangular_preset_is_not_the_exact_transform():
     for preset in (O.PRESET_CXX12, O.PRESET_C24):
         y = O.nufft_forward(u, v, x, preset=preset)
         assert np.linalg.norm(y - ye) / np.linalg.norm(ye) > 1e-2
+
+
+def _pruned_conv(x, K):
+    """The loop's pruned 2N-point convolution of one padded row or column
+    (st_colconv1, st_rowfft_fwd_h / st_rowfft_c2r_h): pixel i at
+    a = (i - N//2) mod 2N, one per residue class r = a mod N; the even and
+    odd outputs of the 2N-point transform are the N-point transforms of y(r)
+    and y(r) W_2N^a; the N wanted outputs of the backward transform are
+    e(r) + conj(W_2N^a) o(r).  Written from those formulas, not from the
+    kernels' code."""
+    N = x.size
+    h = N // 2
+    L = 2 * N
+    tw = np.exp(-2j * np.pi * np.arange(L) / L)
+    r = np.arange(N)
+    i = (r + h) % N
+    a = np.where(r < h, r, r + N)
+    y = x[i]
+    X = np.empty(L, complex)
+    X[0::2] = np.fft.fft(y)
+    X[1::2] = np.fft.fft(y * tw[a])
+    P = X * K
+    e, o = np.fft.ifft(P[0::2]) * N, np.fft.ifft(P[1::2]) * N
+    out = np.empty(N, complex)
+    out[i] = e + o * np.conj(tw[a])
+    return out
+
+
+def _padded_conv(x, K):
+    N = x.size
+    L = 2 * N
+    z = np.zeros(L, complex)
+    z[(np.arange(N) - N // 2) % L] = x
+    full = np.fft.ifft(np.fft.fft(z) * K) * L
+    return full[(np.arange(N) - N // 2) % L]
+
+
+@pytest.mark.parametrize("N", [16, 48, 96, 256, 320, 384, 512])
+def test_pruned_transforms_equal_the_padded_convolution(N):
+    """The identity behind the loop's pruned transforms holds for the even
+    sizes the loop takes (powers of two and the radix-3 / radix-5 sizes),
+    against the direct 2N-point convolution of the zero-padded sequence."""
+    rng = np.random.default_rng(N)
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    K = rng.normal(size=2 * N)
+    K = 0.5 * (K + K[(-np.arange(2 * N)) % (2 * N)])  # real and even, as the loop's Kh
+    ref = _padded_conv(x, K)
+    assert np.abs(_pruned_conv(x, K) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_pruned_placement_needs_even_sizes():
+    """For odd N the placement a = r for r < N//2 misses the residue class
+    r = N//2 (pixel N-1 sits at a = N//2, not at N//2 + N), so the identity
+    fails: the loop takes even sizes only."""
+    N = 45
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    K = rng.normal(size=2 * N)
+    K = 0.5 * (K + K[(-np.arange(2 * N)) % (2 * N)])
+    ref = _padded_conv(x, K)
+    assert np.abs(_pruned_conv(x, K) - ref).max() > 1e-3 * np.abs(ref).max()
