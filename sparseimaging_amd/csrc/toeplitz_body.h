@@ -261,8 +261,11 @@ __device__ inline void st_rowfft_c2r_h(int task, const cplx<R>* __restrict__ U, 
   __syncthreads();
 }
 
-// LDS bytes of the FP64 twiddle tables: both axes and the pruned column transforms' NX-point table
-__host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) { return (size_t)(Lx + Ly + Lx / 2) * 16; }
+// LDS bytes of the FP64 twiddle tables: the 2N-point table (square images:
+// rows and columns share it) and the pruned transforms' N-point table
+__host__ __device__ inline size_t toep_table_bytes(int Lx, int Ly) {
+  return (size_t)(Lx + (Ly != Lx ? Ly : 0) + Lx / 2) * 16;
+}
 
 // The solve by workgroups lb = 0 .. nb-1 of a cooperative launch.  TAB: the
 // twiddle tables are staged in LDS.
@@ -293,11 +296,15 @@ __device__ __forceinline__ void toeplitz_body(const PArgs& A, const int lb, cons
   char* lds = reinterpret_cast<char*>(smem_d4);
   const double2 *twxd = GP(A.twxd), *twyd = GP(A.twyd), *twxh = GP(A.twxh);
   if constexpr (TAB) {
+    // (the loop takes square images only: the Lx- and Ly-point tables are
+    // the same table, staged once; the layout keeps a second one otherwise)
+    const bool sq = g.Lx == g.Ly;
     double2* t64x = reinterpret_cast<double2*>(lds);
-    double2* t64y = t64x + g.Lx;
-    double2* t64h = t64y + g.Ly;
+    double2* t64y = sq ? t64x : t64x + g.Lx;
+    double2* t64h = t64x + g.Lx + (sq ? 0 : g.Ly);
     for (int i = threadIdx.x; i < g.Lx; i += blockDim.x) t64x[i] = GP(A.twxd)[i];
-    for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) t64y[i] = GP(A.twyd)[i];
+    if (!sq)
+      for (int i = threadIdx.x; i < g.Ly; i += blockDim.x) t64y[i] = GP(A.twyd)[i];
     for (int i = threadIdx.x; i < g.Lx / 2; i += blockDim.x) t64h[i] = GP(A.twxh)[i];
     twxd = t64x;
     twyd = t64y;
