@@ -20,9 +20,11 @@ def _problem(N, M, seed, amax):
 @pytest.mark.parametrize("engine", [sa.ENGINE_PERSISTENT, sa.ENGINE_STREAM])
 @pytest.mark.parametrize("preset", [sa.PRESET_CXX12, sa.PRESET_C24])
 def test_solve_l1_tsv_matches_oracle(preset, engine):
+    # a fixed 5000 iterations, past convergence (a 1e-9 rule stops near 3700):
+    # at that rule the FP32 gridding's objective noise can end a solve early
     P = _problem(64, 2000, 1234, 0.35)
-    kw = dict(maxiter=20000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=10.0, cinit=1e4,
-              nonneg_flag=1, rel_tol=1e-9)
+    kw = dict(maxiter=5000, eps=0.0, lambda_l1=1.0, lambda_tv=0.0, lambda_tsv=10.0, cinit=1e4,
+              nonneg_flag=1, rel_tol=0.0)
     print(f"case preset={preset} engine={engine} N=64 M=2000", flush=True)
     o = oracle.imaging_core_nufft(P["u"], P["v"], P["vis_r"], P["vis_i"], P["std"], 64, 64, kw["maxiter"],
                                   kw["eps"], kw["lambda_l1"], 0.0, kw["lambda_tsv"], kw["cinit"],
@@ -34,6 +36,7 @@ def test_solve_l1_tsv_matches_oracle(preset, engine):
     dcost = abs(rg["finalcost"] - ro["finalcost"]) / abs(ro["finalcost"])
     print(f"  ITER gpu={rg['ITER']} oracle={ro['ITER']}  image relL2={l2:.3e}  cost rel={dcost:.3e}"
           f"  cost gpu={rg['finalcost']:.10e} oracle={ro['finalcost']:.10e}")
+    assert rg["ITER"] == 5000
     assert l2 <= 1e-3
     assert dcost <= 1e-6
     assert rg["M"] == (2000 if preset == sa.PRESET_CXX12 else 1000)
