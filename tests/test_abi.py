@@ -121,3 +121,13 @@ def test_product_does_not_import_the_oracle():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in text and "liboracle" not in text, f
+
+
+def test_python_constants_match_the_header():
+    """The Python layer's engine, loop and preset values are the header's."""
+    text = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include",
+                             "mfista.h")).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r"^#define MFISTA_(\w+) (-?\d+)\b", text, re.M)}
+    for name in ("ENGINE_PERSISTENT", "ENGINE_STREAM", "LOOP_AUTO", "LOOP_GRIDDING", "LOOP_TOEPLITZ",
+                 "PRESET_C24", "PRESET_CXX12"):
+        assert getattr(sa, name) == defs[name], name
