@@ -161,3 +161,49 @@ def test_pruned_placement_needs_even_sizes():
     K = 0.5 * (K + K[(-np.arange(2 * N)) % (2 * N)])
     ref = _padded_conv(x, K)
     assert np.abs(_pruned_conv(x, K) - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def test_step_quadratic_from_the_half_spectrum():
+    """The identities a two-barrier round would use instead of the c2r of both
+    trials: <d, T d> = sum_k Kh(k) |D(k)|^2 over the padded grid (D the
+    transform of the padded step, Kh the circulant's spectrum over (2N)^2),
+    taken over the half spectrum ky = 0 .. N with weights 1, 2, ..., 2, 1;
+    and chi^2(z + d) - chi^2(z) = 2 <d, T z - b> + <d, T d>.  T is the
+    embedding of p(D) = sum_m w_m^2 cos(u_m . D) from the exact sums, checked
+    first against the exact normal operator Re A^H W^2 A of the direct DFT."""
+    N, M = 16, 300
+    u, v, w, V, _ = _problem(N, M, 21)
+    rng = np.random.default_rng(5)
+    z = np.abs(rng.normal(size=(N, N)))
+    d = rng.normal(size=(N, N))
+    L = 2 * N
+    D = np.arange(-N, N)
+    p2 = np.einsum("m,ma,mb->ab", w ** 2, np.exp(1j * np.outer(u, D)), np.exp(1j * np.outer(v, D))).real
+    T = lambda x: _toeplitz_apply(p2, x)  # noqa: E731
+    E, F = np.exp(1j * np.outer(u, np.arange(N))), np.exp(1j * np.outer(v, np.arange(N)))
+    y = np.einsum("ij,mi,mj->m", z, E.conj(), F.conj())
+    Tz_direct = np.einsum("m,mi,mj->ij", w ** 2 * y, E, F).real
+    assert np.linalg.norm(T(z) - Tz_direct) <= 1e-12 * np.linalg.norm(Tz_direct)
+    c = np.zeros((L, L))
+    for a in range(L):
+        for b in range(L):
+            if a != N and b != N:
+                c[a, b] = p2[(a if a < N else a - L) + N, (b if b < N else b - L) + N]
+    Kh = np.fft.fft2(c).real / (L * L)
+    dp = np.zeros((L, L))
+    dp[:N, :N] = d
+    Dk = np.fft.fft2(dp)
+    full = (Kh * np.abs(Dk) ** 2).sum()
+    wgt = np.full(L, 2.0)
+    wgt[0] = wgt[N] = 1.0
+    wgt[N + 1:] = 0.0  # the half spectrum along the row transform's axis
+    half = (Kh * np.abs(Dk) ** 2 * wgt[None, :]).sum()
+    dTd = (d * T(d)).sum()
+    assert abs(full - dTd) <= 1e-10 * abs(dTd)
+    assert abs(half - dTd) <= 1e-10 * abs(dTd)
+    # chi^2(x) - |W V|^2 = <x, T x> - 2 <x, b>
+    b = rng.normal(size=(N, N))
+    q = lambda x: (x * T(x)).sum() - 2 * (x * b).sum()  # noqa: E731
+    lhs = q(z + d) - q(z)
+    rhs = 2 * (d * (T(z) - b)).sum() + dTd
+    assert abs(lhs - rhs) <= 1e-10 * max(abs(lhs), 1.0)
